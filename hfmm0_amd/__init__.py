@@ -1,0 +1,5 @@
+"""MI355X-native Helmholtz FMM (HIP kernels for gfx950 behind a C ABI).
+
+``from hfmm0_amd import setup, evaluate`` mirrors the reference's ``HFMM`` module.
+"""
+from .api import HFMMState, benchmark, direct, evaluate, evaluate_device, setup  # noqa: F401

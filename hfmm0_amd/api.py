@@ -1,0 +1,159 @@
+"""Drop-in counterpart of the reference's ``HFMM.setup`` / ``HFMM.evaluate``.
+
+``setup(positions, k, p=4, N_max=16, max_depth=None)`` and ``evaluate(state, charges)``
+keep the reference signatures and ValueError messages (``HFMM.py:55-91``).  The
+evaluate path runs entirely in libhfmm's HIP kernels; there is no CPU fallback.
+
+Differences by design (the reference path is a kernel-independent FMM):
+* ``p`` is the truncation degree of the solid-harmonic expansions ((p+1)^2
+  coefficients per box), not the number of surface points per edge;
+* ``evaluate`` also accepts complex charges (the reference accepts real ones only,
+  ``HFMM.py:86-87``); real charges give the reference semantics, including an exactly
+  real result when k == 0 (``HFMM.py:71-72, 90``).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+PRECISIONS = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED}
+
+
+class HFMMState:
+    """Plan handle returned by :func:`setup`; owns all device buffers."""
+
+    def __init__(self, handle, n, k, opts):
+        self._h = handle
+        self.n = n
+        self.k = k
+        self.opts = opts
+        info = L.PlanInfo()
+        L.check(L.lib().hfmm_plan_info_get(handle, ctypes.byref(info)))
+        self.info = info
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("the HFMM state has been released")
+        return self._h
+
+    @property
+    def depth(self):
+        return self.info.depth
+
+    @property
+    def p_level(self):
+        return [self.info.p_level[i] for i in range(1, self.info.depth + 1)]
+
+    def close(self):
+        if self._h is not None:
+            L.lib().hfmm_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tree(self):
+        """Tree and lists of this plan as NumPy arrays (canonical box order)."""
+        from .tree import _export
+        lib = L.lib()
+        return _export(self.handle, self.info, lib.hfmm_plan_export_tree, lib.hfmm_plan_export_list)
+
+
+def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, use_graph=True):
+    """Build the octree, lists and operator tables; upload everything to the GPU.
+
+    Validation order and messages follow ``HFMM.py:57-64`` and ``HFMM.py:14-16``.
+    ``p_level`` optionally gives the truncation degree per tree level (index 0 is
+    level 1); ``precision`` is ``"fp64"`` (plain FMM) or ``"mixed"`` (FP32 near field).
+    """
+    x = np.asarray(positions, dtype=np.float64)
+    if x.ndim != 2 or x.shape[1] != 3 or len(x) == 0 or not np.isfinite(x).all():
+        raise ValueError("positions must be a nonempty finite (N,3) array")
+    if p < 2 or int(p) != p or N_max < 1 or int(N_max) != N_max:
+        raise ValueError("p must be an integer >=2; N_max must be a positive integer")
+    if max_depth is not None and (max_depth < 1 or int(max_depth) != max_depth):
+        raise ValueError("max_depth must be a positive integer or None")
+    k = float(k)
+    if not np.isfinite(k):
+        raise ValueError("k must be a finite real number")
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+    o = L.default_opts()
+    o.p = int(p)
+    o.n_max = int(N_max)
+    o.max_depth = int(max_depth) if max_depth is not None else 0
+    o.precision = PRECISIONS[precision]
+    o.use_graph = 1 if use_graph else 0
+    if p_level is not None:
+        for i, v in enumerate(p_level):
+            if i + 1 >= L.MAX_LEVELS:
+                break
+            o.p_level[i + 1] = int(v)
+    x = np.ascontiguousarray(x)
+    h = ctypes.c_void_p()
+    L.check(L.lib().hfmm_plan_create(L.ptr(x), len(x), k, ctypes.byref(o), ctypes.byref(h)))
+    return HFMMState(h, len(x), k, o)
+
+
+def evaluate(state, charges):
+    """phi at every point, complex128, input order (``HFMM.py:83-91``)."""
+    q = np.asarray(charges)
+    if q.shape != (state.n,):
+        raise ValueError("charges must be a real array of shape (N,)")
+    if np.iscomplexobj(q):
+        qc = np.ascontiguousarray(q, dtype=np.complex128)
+        qt = L.C128
+    else:
+        qc = np.ascontiguousarray(q, dtype=np.float64)
+        qt = L.F64
+    out = np.empty(state.n, dtype=np.complex128)
+    L.check(L.lib().hfmm_evaluate(state.handle, L.ptr(qc), qt, L.ptr(out), L.HOST, None))
+    return out
+
+
+def evaluate_device(state, q_dev, out_dev, stream=None):
+    """Evaluate with device-resident torch tensors: ``q_dev`` (N,) float64 or complex128,
+    ``out_dev`` (N,) complex128, both on the GPU."""
+    import torch
+    if q_dev.shape != (state.n,) or out_dev.shape != (state.n,) or out_dev.dtype != torch.complex128:
+        raise ValueError("charges must be a real array of shape (N,)")
+    if not (q_dev.is_cuda and out_dev.is_cuda and q_dev.is_contiguous() and out_dev.is_contiguous()):
+        raise ValueError("device evaluate needs contiguous GPU tensors")
+    qt = L.C128 if q_dev.is_complex() else L.F64
+    if qt == L.F64 and q_dev.dtype != torch.float64:
+        raise ValueError("real charges must be float64")
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    L.check(L.lib().hfmm_evaluate(state.handle, ctypes.c_void_p(q_dev.data_ptr()), qt,
+                                  ctypes.c_void_p(out_dev.data_ptr()), L.DEVICE, ctypes.c_void_p(stream)))
+    return out_dev
+
+
+def benchmark(state, q_dev, out_dev, warmup=3, reps=20, phases=False):
+    """Median milliseconds of one device-resident evaluate (hipEvent timing of graph
+    replays) and optionally per-phase milliseconds of one instrumented run."""
+    qt = L.C128 if q_dev.is_complex() else L.F64
+    med = ctypes.c_double()
+    ph = np.zeros(7)
+    L.check(L.lib().hfmm_benchmark(state.handle, ctypes.c_void_p(q_dev.data_ptr()), qt,
+                                   ctypes.c_void_p(out_dev.data_ptr()), int(warmup), int(reps),
+                                   ctypes.byref(med), L.ptr(ph) if phases else None))
+    if phases:
+        names = ["gather", "p2m", "m2m", "m2l", "l2l", "l2p", "p2p"]
+        return med.value, dict(zip(names, ph.tolist()))
+    return med.value
+
+
+def direct(positions, charges, k, targets):
+    """FP64 direct sum on the GPU at ``positions[targets]`` (sampled-target oracle)."""
+    x = np.ascontiguousarray(positions, dtype=np.float64)
+    q = np.ascontiguousarray(np.asarray(charges).astype(np.complex128))
+    t = np.ascontiguousarray(targets, dtype=np.int64)
+    out = np.empty(len(t), dtype=np.complex128)
+    L.check(L.lib().hfmm_direct(L.ptr(x), len(x), L.ptr(q), float(k), L.ptr(t), len(t), L.ptr(out)))
+    return out

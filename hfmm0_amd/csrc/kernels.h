@@ -1,0 +1,72 @@
+// Launchers of the HIP kernels (plain pointers; called from the host plan code).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hfmm {
+
+// Per-box data, device side (canonical box order).
+struct DevBoxes {
+    const double* centre;      // 3 per box
+    const int* level;
+    const int* pstart;
+    const int* count;
+    const int64_t* exp_off;    // offset (in complex coefficients) of the box's expansion
+};
+
+// Per-level data, device side.
+struct DevLevels {
+    const int* p;              // truncation degree per level
+    const double* R;           // expansion radius sqrt(3) h per level
+};
+
+// Displacement information for one translation direction.
+struct DispInfo {
+    int theta;                 // index into the rotation tables
+    int dist;                  // index of the distance within a level's coaxial tables
+};
+
+// Copies the Legendre recurrence coefficients to constant memory (once per process).
+void upload_legendre_coefficients();
+
+void launch_fill_zero(double2* buf, int64_t n, hipStream_t s);
+void launch_gather_charges(const void* q, int q_type, const int* perm, int64_t n, int conj, double2* q_sorted,
+                           hipStream_t s);
+
+void launch_p2m(const int* leaves, int n_items, int chunk, DevBoxes bx, DevLevels lv, const double* X,
+                const double* Y, const double* Z, const double2* q, double k, int pmax, double2* M, hipStream_t s);
+
+void launch_l2p(const int* leaves, int n_items, DevBoxes bx, DevLevels lv, const double* X, const double* Y,
+                const double* Z, const double2* L, double k, int pmax, double2* far, hipStream_t s);
+
+// Generic rotate / coaxial / rotate-back translation over a CSR pair list.
+// Target t = targets[blockIdx.x]; sources src[ptr[t_i] .. ptr[t_i+1]) with direction dir[].
+// dst[exp_off[t]] (+)= sum over sources of the translated expansion.
+struct XlateArgs {
+    const int* targets;        // n_targets box ids
+    const int* ptr;            // CSR offsets indexed by target ordinal
+    const int* src;            // source box ids
+    const unsigned short* dir; // direction index per entry
+    const DispInfo* dinfo;     // per direction
+    const double2* phase;      // per direction, e^{i m phi}, m = 0..pmax
+    const double* rot;         // rotation tables, rot_size(pmax) doubles per theta index
+    const double2* coax;       // coaxial tables of this launch, indexed by dinfo.dist
+    int64_t coax_stride;       // complex entries per coaxial table
+    int ps, pt, pmax;
+    int accumulate;            // 1: dst += result, 0: dst = result
+    const double2* srcbuf;
+    double2* dstbuf;
+    const int64_t* exp_off;
+};
+void launch_xlate(const XlateArgs& a, int n_targets, hipStream_t s);
+
+void launch_p2p(int precision, int n_leaves, const int* leaf_box, DevBoxes bx, const int* rng_ptr,
+                const int* rng_start, const int* rng_pref, const int* leaf_src_total, const double* X,
+                const double* Y, const double* Z, const double2* q, double k, const double2* far, const int* perm,
+                int out_mode /* bit 0: conjugate, bit 1: zero imaginary part */, double2* out, hipStream_t s);
+
+void launch_direct(const double* X, const double* Y, const double* Z, const double2* q, int64_t n, double k,
+                   const double* tx, const double* ty, const double* tz, int64_t nt, double2* out, hipStream_t s);
+
+}  // namespace hfmm

@@ -1,0 +1,672 @@
+// HIP kernels of the Helmholtz FMM for gfx950 (CDNA4, wave64).
+//
+// Expansion conventions (shared with tables.cpp): box radius R = sqrt(3) h;
+//   Mh_n^m = sum_j q_j jhat_n(k rho_j)(rho_j/R)^n conj(Y_n^m(y_j))
+//   phi(x) = sum Lh_n^m jhat_n(k r)(r/R)^n Y_n^m(x)
+// with jhat_n(x) = j_n(x)(2n+1)!!/x^n, orthonormal Y_n^m (Condon-Shortley phase) and
+// coefficient index n*n + n + m.  Expansions are complex FP64, one contiguous
+// (p+1)^2 vector per box.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace hfmm {
+
+namespace {
+
+constexpr int kXlateThreads = 256;
+constexpr int kAccPerThread = ((kMaxP + 1) * (kMaxP + 1) + kXlateThreads - 1) / kXlateThreads;
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cmadd(double2 acc, double2 a, double2 b) {
+    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
+    return acc;
+}
+__device__ inline double2 conjd(double2 a) { return make_double2(a.x, -a.y); }
+
+__device__ inline void nm_decode(int c, int& n, int& m) {
+    n = (int)sqrtf((float)c + 0.5f);
+    if ((n + 1) * (n + 1) <= c) ++n;
+    if (n * n > c) --n;
+    m = c - n * n - n;
+}
+
+// jhat_n(x) for n = 0..p by Miller's downward recurrence
+//   jhat_{n-1} = jhat_n - x^2 jhat_{n+1} / ((2n+1)(2n+3)),
+// normalised with jhat_0 = sin(x)/x (or jhat_1 = 3(sin x - x cos x)/x^3 when that is
+// the larger one).  out[n * stride].
+__device__ void jhat_dev(double x, int p, double* out, int stride) {
+    x = fabs(x);
+    if (x == 0.0) {
+        for (int n = 0; n <= p; ++n) out[n * stride] = 1.0;
+        return;
+    }
+    const int N = p + (int)(1.5 * x) + 30;
+    double a_next = 0.0, a = 1.0;
+    const double x2 = x * x;
+    for (int n = N; n >= 1; --n) {
+        const double am1 = a - x2 * a_next / ((2.0 * n + 1.0) * (2.0 * n + 3.0));
+        a_next = a;
+        a = am1;
+        if (n - 1 <= p) out[(n - 1) * stride] = a;
+        if (fabs(a) > 1e250) {
+            a *= 1e-250;
+            a_next *= 1e-250;
+            for (int j = n - 1; j <= p; ++j)
+                if (j >= 0) out[j * stride] *= 1e-250;
+        }
+    }
+    double s, c;
+    sincos(x, &s, &c);
+    const double e0 = s / x;
+    double scale;
+    if (x < 1.0) {
+        scale = e0 / out[0];
+    } else {
+        const double e1 = 3.0 * (s - x * c) / (x * x * x);
+        scale = fabs(e0) >= fabs(e1) ? e0 / out[0] : e1 / out[stride];
+    }
+    for (int n = 0; n <= p; ++n) out[n * stride] *= scale;
+}
+
+// Legendre recurrence coefficients (fully normalised, Condon-Shortley phase):
+//   P_m^m = -d_m sin(theta) P_{m-1}^{m-1},  d_m = sqrt((2m+1)/(2m))
+//   P_{m+1}^m = e_m cos(theta) P_m^m,       e_m = sqrt(2m+3)
+//   P_n^m = a_nm (cos(theta) P_{n-1}^m - b_nm P_{n-2}^m)
+struct LegCoef {
+    double a[kMaxP + 1][kMaxP + 1];
+    double b[kMaxP + 1][kMaxP + 1];
+    double d[kMaxP + 1];
+    double e[kMaxP + 1];
+};
+__constant__ LegCoef c_leg;
+
+// ---------------------------------------------------------------------------------
+// small utility kernels
+// ---------------------------------------------------------------------------------
+__global__ void fill_zero_kernel(double2* buf, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) buf[i] = make_double2(0.0, 0.0);
+}
+
+__global__ void gather_charges_kernel(const void* q, int q_type, const int* perm, int64_t n, int conj,
+                                      double2* qs) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int o = perm[i];
+    double2 v;
+    if (q_type == 0)
+        v = make_double2(((const double*)q)[o], 0.0);
+    else
+        v = ((const double2*)q)[o];
+    if (conj) v.y = -v.y;
+    qs[i] = v;
+}
+
+// ---------------------------------------------------------------------------------
+// P2M: one workgroup per leaf; points processed in chunks staged in LDS.
+// Phase A: work items (point j, order m) build P_n^m(cos beta_j), e^{-i m alpha_j}
+//          and (m == 0) the radial factors q_j jhat_n(k rho_j)(rho_j/R)^n.
+// Phase B: one thread per coefficient (n, m) accumulates over the chunk.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void p2m_kernel(const int* leaves, int chunk, DevBoxes bx, DevLevels lv,
+                                                  const double* X, const double* Y, const double* Z,
+                                                  const double2* q, double k, double2* M) {
+    extern __shared__ double smem[];
+    const int b = leaves[blockIdx.x];
+    const int L = bx.level[b];
+    const int p = lv.p[L];
+    const double R = lv.R[L];
+    const int P1 = p + 1, tri = P1 * (P1 + 1) / 2, nc = P1 * P1;
+    double2* s_qrad = (double2*)smem;
+    double2* s_ph = s_qrad + chunk * P1;
+    double* s_leg = (double*)(s_ph + chunk * P1);
+    const double cx = bx.centre[3 * b], cy = bx.centre[3 * b + 1], cz = bx.centre[3 * b + 2];
+    const int ps = bx.pstart[b], cnt = bx.count[b];
+    double2 acc[kAccPerThread];
+#pragma unroll
+    for (int t = 0; t < kAccPerThread; ++t) acc[t] = make_double2(0.0, 0.0);
+
+    for (int j0 = 0; j0 < cnt; j0 += chunk) {
+        const int nch = min(chunk, cnt - j0);
+        for (int w = threadIdx.x; w < nch * P1; w += blockDim.x) {
+            const int j = w / P1, m = w - j * P1;
+            const int i = ps + j0 + j;
+            const double dx = X[i] - cx, dy = Y[i] - cy, dz = Z[i] - cz;
+            const double rho = sqrt(dx * dx + dy * dy + dz * dz);
+            const double rxy = sqrt(dx * dx + dy * dy);
+            const double ct = rho > 0.0 ? dz / rho : 1.0;
+            const double st = rho > 0.0 ? rxy / rho : 0.0;
+            const double alpha = atan2(dy, dx);
+            double sa, ca;
+            sincos(m * alpha, &sa, &ca);
+            s_ph[j * P1 + m] = make_double2(ca, -sa);
+            double* Pl = s_leg + j * tri;
+            double pmm = 0.28209479177387814;   // 1/sqrt(4 pi)
+            for (int i2 = 1; i2 <= m; ++i2) pmm *= -c_leg.d[i2] * st;
+            Pl[m * (m + 1) / 2 + m] = pmm;
+            if (m < p) {
+                double p2 = pmm, p1 = c_leg.e[m] * ct * pmm;
+                Pl[(m + 1) * (m + 2) / 2 + m] = p1;
+                for (int n = m + 2; n <= p; ++n) {
+                    const double pn = c_leg.a[n][m] * (ct * p1 - c_leg.b[n][m] * p2);
+                    Pl[n * (n + 1) / 2 + m] = pn;
+                    p2 = p1;
+                    p1 = pn;
+                }
+            }
+            if (m == 0) {
+                double* rad = &s_qrad[j * P1].x;
+                jhat_dev(k * rho, p, rad, 2);
+                const double2 qq = q[i];
+                const double rr = rho / R;
+                double pw = 1.0;
+                for (int n = 0; n <= p; ++n) {
+                    const double g = rad[2 * n] * pw;
+                    s_qrad[j * P1 + n] = make_double2(qq.x * g, qq.y * g);
+                    pw *= rr;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < kAccPerThread; ++t) {
+            const int c = threadIdx.x + t * blockDim.x;
+            if (c < nc) {
+                int n, m;
+                nm_decode(c, n, m);
+                const int am = m < 0 ? -m : m;
+                double2 a = acc[t];
+                for (int j = 0; j < nch; ++j) {
+                    const double2 qr = s_qrad[j * P1 + n];
+                    double pl = s_leg[j * tri + n * (n + 1) / 2 + am];
+                    double2 ph = s_ph[j * P1 + am];
+                    if (m < 0) {
+                        ph.y = -ph.y;
+                        if (am & 1) pl = -pl;
+                    }
+                    a = cmadd(a, qr, make_double2(pl * ph.x, pl * ph.y));
+                }
+                acc[t] = a;
+            }
+        }
+        __syncthreads();
+    }
+    double2* out = M + bx.exp_off[b];
+#pragma unroll
+    for (int t = 0; t < kAccPerThread; ++t) {
+        const int c = threadIdx.x + t * blockDim.x;
+        if (c < nc) out[c] = acc[t];
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// L2P: one workgroup (one wave) per leaf, one thread per target point.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void l2p_kernel(const int* leaves, DevBoxes bx, DevLevels lv, const double* X,
+                                                 const double* Y, const double* Z, const double2* Lexp, double k,
+                                                 double2* far) {
+    extern __shared__ double smem[];
+    const int b = leaves[blockIdx.x];
+    const int L = bx.level[b];
+    const int p = lv.p[L];
+    const double R = lv.R[L];
+    const int P1 = p + 1, nc = P1 * P1;
+    double2* s_L = (double2*)smem;
+    double* s_rad = (double*)(s_L + nc);
+    const double2* src = Lexp + bx.exp_off[b];
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) s_L[c] = src[c];
+    __syncthreads();
+    const double cx = bx.centre[3 * b], cy = bx.centre[3 * b + 1], cz = bx.centre[3 * b + 2];
+    const int ps = bx.pstart[b], cnt = bx.count[b];
+    double* rad = s_rad + threadIdx.x * P1;
+    for (int i0 = 0; i0 < cnt; i0 += blockDim.x) {
+        const int il = i0 + threadIdx.x;
+        if (il >= cnt) break;
+        const int i = ps + il;
+        const double dx = X[i] - cx, dy = Y[i] - cy, dz = Z[i] - cz;
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        const double rxy = sqrt(dx * dx + dy * dy);
+        const double ct = r > 0.0 ? dz / r : 1.0;
+        const double st = r > 0.0 ? rxy / r : 0.0;
+        const double2 eia = rxy > 0.0 ? make_double2(dx / rxy, dy / rxy) : make_double2(1.0, 0.0);
+        jhat_dev(k * r, p, rad, 1);
+        const double rr = r / R;
+        double pw = 1.0;
+        for (int n = 0; n <= p; ++n) {
+            rad[n] *= pw;
+            pw *= rr;
+        }
+        double2 phi = make_double2(0.0, 0.0);
+        double pmm = 0.28209479177387814;
+        double2 eim = make_double2(1.0, 0.0);
+        for (int m = 0; m <= p; ++m) {
+            if (m > 0) {
+                pmm *= -c_leg.d[m] * st;
+                eim = cmul(eim, eia);
+            }
+            const double sgn = (m & 1) ? -1.0 : 1.0;
+            double p2 = 0.0, p1 = 0.0;
+            for (int n = m; n <= p; ++n) {
+                double pn;
+                if (n == m)
+                    pn = pmm;
+                else if (n == m + 1)
+                    pn = c_leg.e[m] * ct * pmm;
+                else
+                    pn = c_leg.a[n][m] * (ct * p1 - c_leg.b[n][m] * p2);
+                p2 = p1;
+                p1 = pn;
+                const double g = rad[n] * pn;
+                const double2 lp = s_L[n * n + n + m];
+                double2 term = cmul(lp, eim);
+                if (m > 0) {
+                    const double2 lm = s_L[n * n + n - m];
+                    const double2 t2 = cmul(lm, conjd(eim));
+                    term.x += sgn * t2.x;
+                    term.y += sgn * t2.y;
+                }
+                phi.x = fma(g, term.x, phi.x);
+                phi.y = fma(g, term.y, phi.y);
+            }
+        }
+        far[i] = phi;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Translation (M2M, M2L, L2L): rotate the source expansion so the translation axis
+// is +z, apply the coaxial operator (block diagonal in m), rotate back, accumulate.
+// One workgroup per target box; thread per coefficient; the three stages exchange
+// data through LDS.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(kXlateThreads) void xlate_kernel(XlateArgs a) {
+    extern __shared__ double2 sm2[];
+    const int ps = a.ps, pt = a.pt;
+    const int ncs = (ps + 1) * (ps + 1), nct = (pt + 1) * (pt + 1);
+    const int nmx = max(ncs, nct);
+    double2* sIn = sm2;          // stage-0 output, reused for the stage-2 output
+    double2* sA = sIn + nmx;
+    double2* sB = sIn;
+    int* s_off = (int*)(sA + nmx);
+    const int mmax = min(ps, pt);
+    if (threadIdx.x == 0) {
+        int o = 0;
+        for (int m = 0; m <= mmax; ++m) {
+            s_off[m] = o;
+            o += (pt + 1 - m) * (ps + 1 - m);
+        }
+    }
+    const int t_ord = blockIdx.x;
+    const int tbox = a.targets[t_ord];
+    const int rsz = (a.pmax + 1) * (2 * a.pmax + 1) * (2 * a.pmax + 3) / 3;
+    double2 acc[kAccPerThread];
+#pragma unroll
+    for (int t = 0; t < kAccPerThread; ++t) acc[t] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int e = a.ptr[t_ord]; e < a.ptr[t_ord + 1]; ++e) {
+        const int s = a.src[e];
+        const int d = a.dir[e];
+        const DispInfo di = a.dinfo[d];
+        const double2* ph = a.phase + (size_t)d * (a.pmax + 1);
+        const double* rot = a.rot + (size_t)di.theta * rsz;
+        const double2* cx = a.coax + (size_t)di.dist * a.coax_stride;
+        const double2* src = a.srcbuf + a.exp_off[s];
+        for (int c = threadIdx.x; c < ncs; c += blockDim.x) {
+            int n, m;
+            nm_decode(c, n, m);
+            const double2 phm = m >= 0 ? ph[m] : conjd(ph[-m]);
+            sIn[c] = cmul(src[c], phm);
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c < ncs; c += blockDim.x) {
+            int n, m;
+            nm_decode(c, n, m);
+            const int w = 2 * n + 1;
+            const double* E = rot + n * (2 * n - 1) * (2 * n + 1) / 3;
+            double2 v = make_double2(0.0, 0.0);
+            const double2* in = sIn + n * n;
+            for (int j = 0; j < w; ++j) {
+                const double e2 = E[j * w + (m + n)];
+                v.x = fma(e2, in[j].x, v.x);
+                v.y = fma(e2, in[j].y, v.y);
+            }
+            sA[c] = v;
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c < nct; c += blockDim.x) {
+            int n, m;
+            nm_decode(c, n, m);
+            const int am = m < 0 ? -m : m;
+            double2 v = make_double2(0.0, 0.0);
+            if (am <= ps) {
+                const double2* T = cx + s_off[am] + (n - am) * (ps + 1 - am);
+                for (int n2 = am; n2 <= ps; ++n2) v = cmadd(v, T[n2 - am], sA[n2 * n2 + n2 + m]);
+            }
+            sB[c] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < kAccPerThread; ++t) {
+            const int c = threadIdx.x + t * blockDim.x;
+            if (c < nct) {
+                int n, m;
+                nm_decode(c, n, m);
+                const int w = 2 * n + 1;
+                const double* E = rot + n * (2 * n - 1) * (2 * n + 1) / 3 + (m + n) * w;
+                const double2* in = sB + n * n;
+                double2 v = make_double2(0.0, 0.0);
+                for (int j = 0; j < w; ++j) {
+                    v.x = fma(E[j], in[j].x, v.x);
+                    v.y = fma(E[j], in[j].y, v.y);
+                }
+                const double2 phm = m >= 0 ? ph[m] : conjd(ph[-m]);
+                const double2 r = cmul(v, conjd(phm));
+                acc[t].x += r.x;
+                acc[t].y += r.y;
+            }
+        }
+        __syncthreads();
+    }
+    double2* dst = a.dstbuf + a.exp_off[tbox];
+#pragma unroll
+    for (int t = 0; t < kAccPerThread; ++t) {
+        const int c = threadIdx.x + t * blockDim.x;
+        if (c < nct) {
+            if (a.accumulate) {
+                double2 v = dst[c];
+                v.x += acc[t].x;
+                v.y += acc[t].y;
+                dst[c] = v;
+            } else {
+                dst[c] = acc[t];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// P2P near field.
+// ---------------------------------------------------------------------------------
+// sin and cos of x in FP32: Cody-Waite reduction by pi/2 with a two-term split of
+// pi/2 (float(pi/2) and its remainder), then the Taylor polynomials of degree 9 (sin)
+// and 10 (cos) on [-pi/4, pi/4] (truncation error below 2e-9, under half an FP32 ulp
+// of the results), and a quadrant swap.
+__device__ inline void sincos_rr(float x, float& s, float& c) {
+    const float kn = rintf(x * 0.636619772367581343f);
+    float y = fmaf(-kn, 1.57079637050628662109375f, x);
+    y = fmaf(-kn, -4.37113900018624283e-8f, y);
+    const float y2 = y * y;
+    float ps = fmaf(y2, 2.7557319223985893e-6f, -1.9841269841269841e-4f);
+    ps = fmaf(y2, ps, 8.3333333333333333e-3f);
+    ps = fmaf(y2, ps, -1.6666666666666667e-1f);
+    const float sn = fmaf(y * y2, ps, y);
+    float pc = fmaf(y2, -2.7557319223985891e-7f, 2.4801587301587302e-5f);
+    pc = fmaf(y2, pc, -1.3888888888888889e-3f);
+    pc = fmaf(y2, pc, 4.1666666666666667e-2f);
+    pc = fmaf(y2, pc, -0.5f);
+    const float cs = fmaf(y2, pc, 1.0f);
+    const int q = ((int)kn) & 3;
+    const float a = (q & 1) ? cs : sn;
+    const float b = (q & 1) ? sn : cs;
+    s = (q & 2) ? -a : a;
+    c = ((q + 1) & 2) ? -b : b;
+}
+
+template <typename T>
+struct P2PMath;
+
+template <>
+struct P2PMath<float> {
+    __device__ static inline void kernel(float r2, float k, float& wr, float& wi) {
+        const float rinv = __builtin_amdgcn_rsqf(r2);
+        const float r = r2 * rinv;
+        float s, c;
+        sincos_rr(k * r, s, c);
+        wr = c * rinv;
+        wi = s * rinv;
+    }
+};
+
+template <>
+struct P2PMath<double> {
+    __device__ static inline void kernel(double r2, double k, double& wr, double& wi) {
+        const double r = sqrt(r2);
+        const double rinv = 1.0 / r;
+        double s, c;
+        sincos(k * r, &s, &c);
+        wr = c * rinv;
+        wi = s * rinv;
+    }
+};
+
+// One wave per target leaf.  Targets of a chunk of up to 64 points are spread over
+// lane groups: with T' = next power of two >= chunk size, lane l holds target l % T'
+// and scans the sources j == l / T' (mod 64/T') of each 64-point source batch staged
+// in LDS (coordinates relative to the target leaf centre, in T).  Lane-group partial
+// sums are combined with wavefront shuffles.  Sources of a leaf are a list of point
+// ranges (adjacent leaves, and the W/X interactions evaluated directly).
+template <typename T>
+__global__ __launch_bounds__(64) void p2p_kernel(const int* leaf_box, DevBoxes bx, const int* rng_ptr,
+                                                 const int* rng_start, const int* rng_pref,
+                                                 const int* leaf_src_total, const double* X, const double* Y,
+                                                 const double* Z, const double2* q, double k, const double2* far,
+                                                 const int* perm, int out_mode, double2* out) {
+    __shared__ T sx[64], sy[64], sz[64], sqr[64], sqi[64];
+    const int leaf = blockIdx.x;
+    const int b = leaf_box[leaf];
+    const int tstart = bx.pstart[b], tcnt = bx.count[b];
+    const double cx = bx.centre[3 * b], cy = bx.centre[3 * b + 1], cz = bx.centre[3 * b + 2];
+    const int r0 = rng_ptr[leaf], r1 = rng_ptr[leaf + 1];
+    const int S = leaf_src_total[leaf];
+    const int lane = threadIdx.x;
+    const T kk = (T)k;
+    for (int t0 = 0; t0 < tcnt; t0 += 64) {
+        const int tc = min(64, tcnt - t0);
+        int Tp = 1;
+        while (Tp < tc) Tp <<= 1;
+        const int G = 64 / Tp;
+        const int ti = lane & (Tp - 1), g = lane / Tp;
+        const bool has_t = ti < tc;
+        const int it = tstart + t0 + (has_t ? ti : 0);
+        const T xt = (T)(X[it] - cx), yt = (T)(Y[it] - cy), zt = (T)(Z[it] - cz);
+        T ar = 0, ai = 0;
+        for (int base = 0; base < S; base += 64) {
+            const int v = base + lane;
+            if (v < S) {
+                int lo = r0, hi = r1 - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (rng_pref[mid] <= v)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                const int idx = rng_start[lo] + (v - rng_pref[lo]);
+                sx[lane] = (T)(X[idx] - cx);
+                sy[lane] = (T)(Y[idx] - cy);
+                sz[lane] = (T)(Z[idx] - cz);
+                const double2 qq = q[idx];
+                sqr[lane] = (T)qq.x;
+                sqi[lane] = (T)qq.y;
+            }
+            __syncthreads();
+            const int nb = min(64, S - base);
+            for (int j = g; j < nb; j += G) {
+                const T dx = xt - sx[j], dy = yt - sy[j], dz = zt - sz[j];
+                const T r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > (T)0) {
+                    T wr, wi;
+                    P2PMath<T>::kernel(r2, kk, wr, wi);
+                    const T qr = sqr[j], qi = sqi[j];
+                    ar += wr * qr - wi * qi;
+                    ai += wr * qi + wi * qr;
+                }
+            }
+            __syncthreads();
+        }
+        for (int off = Tp; off < 64; off <<= 1) {
+            ar += __shfl_xor(ar, off);
+            ai += __shfl_xor(ai, off);
+        }
+        if (g == 0 && has_t) {
+            double2 f = far[it];
+            f.x += (double)ar;
+            f.y += (double)ai;
+            if (out_mode & 1) f.y = -f.y;   // k < 0: conjugate of the |k| result
+            if (out_mode & 2) f.y = 0.0;    // k == 0 with real charges: the result is real
+            out[perm[it]] = f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// FP64 direct sum at sampled targets: grid (target tiles of 64, source chunks);
+// partial sums combined with FP64 atomics into a zeroed output.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void direct_kernel(const double* X, const double* Y, const double* Z,
+                                                     const double2* q, int64_t n, double k, const double* tx,
+                                                     const double* ty, const double* tz, int64_t nt,
+                                                     int64_t chunk, double2* out) {
+    __shared__ double sx[256], sy[256], sz[256], sqr[256], sqi[256];
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool valid = t < nt;
+    const double x = valid ? tx[t] : 0.0, y = valid ? ty[t] : 0.0, z = valid ? tz[t] : 0.0;
+    const int64_t s0 = (int64_t)blockIdx.y * chunk, s1 = min(n, s0 + chunk);
+    double ar = 0.0, ai = 0.0;
+    for (int64_t base = s0; base < s1; base += 256) {
+        const int64_t v = base + threadIdx.x;
+        if (v < s1) {
+            sx[threadIdx.x] = X[v];
+            sy[threadIdx.x] = Y[v];
+            sz[threadIdx.x] = Z[v];
+            sqr[threadIdx.x] = q[v].x;
+            sqi[threadIdx.x] = q[v].y;
+        }
+        __syncthreads();
+        const int nb = (int)min((int64_t)256, s1 - base);
+        for (int j = 0; j < nb; ++j) {
+            const double dx = x - sx[j], dy = y - sy[j], dz = z - sz[j];
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            if (r > 0.0) {
+                double s, c;
+                sincos(k * r, &s, &c);
+                const double wr = c / r, wi = s / r;
+                ar += wr * sqr[j] - wi * sqi[j];
+                ai += wr * sqi[j] + wi * sqr[j];
+            }
+        }
+        __syncthreads();
+    }
+    if (valid) {
+        atomicAdd(&out[t].x, ar);
+        atomicAdd(&out[t].y, ai);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------
+void upload_legendre_coefficients() {
+    static bool done = false;
+    if (done) return;
+    LegCoef h{};
+    for (int m = 0; m <= kMaxP; ++m) {
+        h.d[m] = m > 0 ? std::sqrt((2.0 * m + 1.0) / (2.0 * m)) : 0.0;
+        h.e[m] = std::sqrt(2.0 * m + 3.0);
+        for (int n = 0; n <= kMaxP; ++n) {
+            if (n >= m + 2) {
+                h.a[n][m] = std::sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m));
+                h.b[n][m] = std::sqrt(((n - 1.0) * (n - 1.0) - (double)m * m) / (4.0 * (n - 1.0) * (n - 1.0) - 1.0));
+            }
+        }
+    }
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_leg), &h, sizeof(h)) != hipSuccess)
+        fail(-3, "hipMemcpyToSymbol(Legendre coefficients) failed");
+    done = true;
+}
+
+static void check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) fail(-3, std::string("kernel launch failed (") + what + "): " + hipGetErrorString(e));
+}
+
+void launch_fill_zero(double2* buf, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, s, buf, n);
+    check_launch("fill_zero");
+}
+
+void launch_gather_charges(const void* q, int q_type, const int* perm, int64_t n, int conj, double2* q_sorted,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(gather_charges_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, q, q_type, perm,
+                       n, conj, q_sorted);
+    check_launch("gather_charges");
+}
+
+void launch_p2m(const int* leaves, int n_items, int chunk, DevBoxes bx, DevLevels lv, const double* X,
+                const double* Y, const double* Z, const double2* q, double k, int pmax, double2* M, hipStream_t s) {
+    if (n_items <= 0) return;
+    upload_legendre_coefficients();
+    const int P1 = pmax + 1, tri = P1 * (P1 + 1) / 2;
+    const size_t shm = (size_t)chunk * (2 * P1 * sizeof(double2) + tri * sizeof(double));
+    hipLaunchKernelGGL(p2m_kernel, dim3(n_items), dim3(256), shm, s, leaves, chunk, bx, lv, X, Y, Z, q, k, M);
+    check_launch("p2m");
+}
+
+void launch_l2p(const int* leaves, int n_items, DevBoxes bx, DevLevels lv, const double* X, const double* Y,
+                const double* Z, const double2* L, double k, int pmax, double2* far, hipStream_t s) {
+    if (n_items <= 0) return;
+    upload_legendre_coefficients();
+    const int P1 = pmax + 1;
+    const size_t shm = (size_t)P1 * P1 * sizeof(double2) + 64 * P1 * sizeof(double);
+    hipLaunchKernelGGL(l2p_kernel, dim3(n_items), dim3(64), shm, s, leaves, bx, lv, X, Y, Z, L, k, far);
+    check_launch("l2p");
+}
+
+void launch_xlate(const XlateArgs& a, int n_targets, hipStream_t s) {
+    if (n_targets <= 0) return;
+    const int nmx = std::max((a.ps + 1) * (a.ps + 1), (a.pt + 1) * (a.pt + 1));
+    const size_t shm = 2 * (size_t)nmx * sizeof(double2) + (kMaxP + 1) * sizeof(int);
+    hipLaunchKernelGGL(xlate_kernel, dim3(n_targets), dim3(kXlateThreads), shm, s, a);
+    check_launch("xlate");
+}
+
+void launch_p2p(int precision, int n_leaves, const int* leaf_box, DevBoxes bx, const int* rng_ptr,
+                const int* rng_start, const int* rng_pref, const int* leaf_src_total, const double* X,
+                const double* Y, const double* Z, const double2* q, double k, const double2* far, const int* perm,
+                int out_mode, double2* out, hipStream_t s) {
+    if (n_leaves <= 0) return;
+    if (precision == 0)
+        hipLaunchKernelGGL(p2p_kernel<double>, dim3(n_leaves), dim3(64), 0, s, leaf_box, bx, rng_ptr, rng_start,
+                           rng_pref, leaf_src_total, X, Y, Z, q, k, far, perm, out_mode, out);
+    else
+        hipLaunchKernelGGL(p2p_kernel<float>, dim3(n_leaves), dim3(64), 0, s, leaf_box, bx, rng_ptr, rng_start,
+                           rng_pref, leaf_src_total, X, Y, Z, q, k, far, perm, out_mode, out);
+    check_launch("p2p");
+}
+
+void launch_direct(const double* X, const double* Y, const double* Z, const double2* q, int64_t n, double k,
+                   const double* tx, const double* ty, const double* tz, int64_t nt, double2* out, hipStream_t s) {
+    if (nt <= 0) return;
+    const int64_t tiles = (nt + 255) / 256;
+    int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(4096 / tiles, (n + 4095) / 4096));
+    const int64_t chunk = (n + nchunk - 1) / nchunk;
+    nchunk = (n + chunk - 1) / chunk;
+    launch_fill_zero(out, nt, s);
+    hipLaunchKernelGGL(direct_kernel, dim3((unsigned)tiles, (unsigned)nchunk), dim3(256), 0, s, X, Y, Z, q, n, k, tx,
+                       ty, tz, nt, chunk, out);
+    check_launch("direct");
+}
+
+}  // namespace hfmm

@@ -1,0 +1,803 @@
+// Plan creation (tree, lists, operator tables, device buffers), the evaluate sequence
+// and the C ABI declared in include/hfmm.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/hfmm.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace hfmm {
+
+void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+static thread_local std::string g_last_error;
+
+#define HIPCHK(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) fail(HFMM_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kDirV = 0;         // 343 same-level displacements
+constexpr int kDirM2M = 343;     // 8 child octants, t = c_parent - c_child
+constexpr int kDirL2L = 351;     // 8 child octants, t = c_child - c_parent
+constexpr int kNumDirs = 359;
+
+struct XlateLaunch {
+    int n_targets = 0;
+    int* targets = nullptr;
+    int* ptr = nullptr;
+    int* src = nullptr;
+    unsigned short* dir = nullptr;
+    double2* coax = nullptr;
+    int64_t coax_stride = 0;
+    int ps = 0, pt = 0;
+};
+
+}  // namespace hfmm
+
+using namespace hfmm;
+
+struct hfmm_plan {
+    Tree tree;
+    Lists lists;
+    int64_t n = 0;
+    double k = 0.0;                 // |k|
+    int conj = 0;                   // k < 0: conjugate charges in and results out
+    int precision = HFMM_PREC_FP64;
+    int use_graph = 1;
+    int depth = 0;
+    int pmax = 0;
+    int p_level[kMaxLevels] = {0};
+    double h[kMaxLevels] = {0}, R[kMaxLevels] = {0};
+    std::vector<int64_t> exp_off;
+    int64_t n_coef = 0;
+    int64_t n_p2p_pairs = 0;
+    int n_leaves = 0, n_far_leaves = 0, p2m_chunk = 8;
+
+    std::vector<void*> allocs;
+    double *dX = nullptr, *dY = nullptr, *dZ = nullptr;
+    int* d_perm = nullptr;
+    double* d_centre = nullptr;
+    int *d_level = nullptr, *d_pstart = nullptr, *d_count = nullptr;
+    int64_t* d_expoff = nullptr;
+    int* d_plevel = nullptr;
+    double* d_R = nullptr;
+    int* d_far_leaves = nullptr;
+    int *d_leaf_box = nullptr, *d_rng_ptr = nullptr, *d_rng_start = nullptr, *d_rng_pref = nullptr,
+        *d_leaf_src_total = nullptr;
+    double* d_rot = nullptr;
+    double2* d_phase = nullptr;
+    DispInfo* d_dinfo = nullptr;
+    XlateLaunch m2l[kMaxLevels], m2m[kMaxLevels], l2l[kMaxLevels];
+    double2 *dM = nullptr, *dL = nullptr, *d_q = nullptr, *d_far = nullptr;
+    void* d_qin = nullptr;          // staging for host input
+    double2* d_out = nullptr;       // staging for host output
+
+    hipStream_t stream = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    const void* g_q = nullptr;
+    void* g_out = nullptr;
+    int g_qtype = -1;
+
+    template <typename T>
+    T* alloc(size_t count) {
+        if (count == 0) count = 1;
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) fail(HFMM_ERR_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        allocs.push_back(p);
+        return (T*)p;
+    }
+    template <typename T>
+    T* upload(const std::vector<T>& v) {
+        T* p = alloc<T>(v.size());
+        if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return p;
+    }
+    ~hfmm_plan() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void* p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    DevBoxes boxes() const { return DevBoxes{d_centre, d_level, d_pstart, d_count, d_expoff}; }
+    DevLevels levels() const { return DevLevels{d_plevel, d_R}; }
+};
+
+namespace {
+
+void check_device() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) fail(HFMM_ERR_DEVICE, "no HIP device available");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
+        fail(HFMM_ERR_DEVICE, std::string("libhfmm is built for gfx950, found ") + prop.gcnArchName);
+}
+
+int octant(uint64_t code) { return (int)(code & 7); }
+
+void build_tables(hfmm_plan& P) {
+    const Tree& t = P.tree;
+    // directions
+    std::map<std::pair<int, int>, int> theta_idx;
+    std::vector<double> thetas;
+    std::vector<DispInfo> dinfo(kNumDirs, DispInfo{0, 0});
+    std::vector<double> phis(kNumDirs, 0.0);
+    std::map<int, int> dist_idx;       // |d|^2 -> index
+    std::vector<int> dist_d2;
+    auto theta_of = [&](int dz, int d2) {
+        auto key = std::make_pair(dz, d2);
+        auto it = theta_idx.find(key);
+        if (it != theta_idx.end()) return it->second;
+        int id = (int)thetas.size();
+        thetas.push_back(std::acos(std::max(-1.0, std::min(1.0, dz / std::sqrt((double)d2)))));
+        theta_idx[key] = id;
+        return id;
+    };
+    for (int dx = -3; dx <= 3; ++dx)
+        for (int dy = -3; dy <= 3; ++dy)
+            for (int dz = -3; dz <= 3; ++dz) {
+                const int d = (dx + 3) * 49 + (dy + 3) * 7 + (dz + 3);
+                if (std::max(std::abs(dx), std::max(std::abs(dy), std::abs(dz))) < 2) continue;
+                const int d2 = dx * dx + dy * dy + dz * dz;
+                if (!dist_idx.count(d2)) {
+                    dist_idx[d2] = (int)dist_d2.size();
+                    dist_d2.push_back(d2);
+                }
+                dinfo[kDirV + d] = DispInfo{theta_of(dz, d2), dist_idx[d2]};
+                phis[kDirV + d] = std::atan2((double)dy, (double)dx);
+            }
+    for (int o = 0; o < 8; ++o) {
+        const int sx = 2 * (o & 1) - 1, sy = 2 * ((o >> 1) & 1) - 1, sz = 2 * ((o >> 2) & 1) - 1;
+        dinfo[kDirM2M + o] = DispInfo{theta_of(-sz, 3), 0};
+        phis[kDirM2M + o] = std::atan2((double)-sy, (double)-sx);
+        dinfo[kDirL2L + o] = DispInfo{theta_of(sz, 3), 0};
+        phis[kDirL2L + o] = std::atan2((double)sy, (double)sx);
+    }
+    const int pmax = P.pmax;
+    const int rsz = rot_size(pmax);
+    std::vector<double> rot((size_t)thetas.size() * rsz);
+    for (size_t i = 0; i < thetas.size(); ++i) rotation_table(thetas[i], pmax, &rot[i * rsz]);
+    P.d_rot = P.upload(rot);
+    std::vector<double2> phase((size_t)kNumDirs * (pmax + 1));
+    for (int d = 0; d < kNumDirs; ++d)
+        for (int m = 0; m <= pmax; ++m)
+            phase[(size_t)d * (pmax + 1) + m] = make_double2(std::cos(m * phis[d]), std::sin(m * phis[d]));
+    P.d_phase = P.upload(phase);
+    P.d_dinfo = P.upload(dinfo);
+
+    const Gaunt G = make_gaunt(pmax);
+    const int D = P.depth;
+    // V lists grouped by level / target
+    {
+        const auto& vt = P.lists.v_tgt;
+        const auto& vs = P.lists.v_src;
+        for (int L = 2; L <= D; ++L) {
+            std::vector<int> targets, ptr{0}, src;
+            std::vector<unsigned short> dir;
+            for (size_t e = 0; e < vt.size(); ++e) {
+                const int tb = vt[e];
+                if (t.level[tb] != L) continue;
+                if (targets.empty() || targets.back() != tb) {
+                    if (!targets.empty()) ptr.push_back((int)src.size());
+                    targets.push_back(tb);
+                }
+                const int sb = vs[e];
+                int tx, ty, tz, sx, sy, sz;
+                morton_grid(t.code[tb], tx, ty, tz);
+                morton_grid(t.code[sb], sx, sy, sz);
+                const int dx = tx - sx, dy = ty - sy, dz = tz - sz;
+                src.push_back(sb);
+                dir.push_back((unsigned short)(kDirV + (dx + 3) * 49 + (dy + 3) * 7 + (dz + 3)));
+            }
+            if (targets.empty()) continue;
+            ptr.push_back((int)src.size());
+            XlateLaunch& X = P.m2l[L];
+            X.n_targets = (int)targets.size();
+            X.targets = P.upload(targets);
+            X.ptr = P.upload(ptr);
+            X.src = P.upload(src);
+            X.dir = P.upload(dir);
+            X.ps = X.pt = P.p_level[L];
+            X.coax_stride = coax_size(X.pt, X.ps);
+            std::vector<double> cx((size_t)dist_d2.size() * X.coax_stride * 2);
+            for (size_t i = 0; i < dist_d2.size(); ++i)
+                coax_table(0, G, P.k, 2.0 * P.h[L] * std::sqrt((double)dist_d2[i]), P.R[L], P.R[L], X.ps, X.pt,
+                           &cx[i * X.coax_stride * 2]);
+            X.coax = (double2*)P.upload(cx);
+        }
+    }
+    // M2M (parent level L <- children at L+1) and L2L (parent level L -> children at L+1)
+    for (int L = 2; L < D; ++L) {
+        std::vector<int> targets, ptr{0}, src;
+        std::vector<unsigned short> dir;
+        std::vector<int> ctargets, cptr{0}, csrc;
+        std::vector<unsigned short> cdir;
+        for (int c = t.level_start[L + 1]; c < t.level_start[L + 2]; ++c) {
+            const int par = t.parent[c];
+            if (targets.empty() || targets.back() != par) {
+                if (!targets.empty()) ptr.push_back((int)src.size());
+                targets.push_back(par);
+            }
+            src.push_back(c);
+            dir.push_back((unsigned short)(kDirM2M + octant(t.code[c])));
+            ctargets.push_back(c);
+            csrc.push_back(par);
+            cdir.push_back((unsigned short)(kDirL2L + octant(t.code[c])));
+            cptr.push_back((int)csrc.size());
+        }
+        if (targets.empty()) continue;
+        ptr.push_back((int)src.size());
+        XlateLaunch& A = P.m2m[L];
+        A.n_targets = (int)targets.size();
+        A.targets = P.upload(targets);
+        A.ptr = P.upload(ptr);
+        A.src = P.upload(src);
+        A.dir = P.upload(dir);
+        A.ps = P.p_level[L + 1];
+        A.pt = P.p_level[L];
+        A.coax_stride = coax_size(A.pt, A.ps);
+        std::vector<double> cx((size_t)A.coax_stride * 2);
+        coax_table(1, G, P.k, P.R[L + 1], P.R[L + 1], P.R[L], A.ps, A.pt, cx.data());
+        A.coax = (double2*)P.upload(cx);
+        XlateLaunch& B = P.l2l[L];
+        B.n_targets = (int)ctargets.size();
+        B.targets = P.upload(ctargets);
+        B.ptr = P.upload(cptr);
+        B.src = P.upload(csrc);
+        B.dir = P.upload(cdir);
+        B.ps = P.p_level[L];
+        B.pt = P.p_level[L + 1];
+        B.coax_stride = coax_size(B.pt, B.ps);
+        std::vector<double> cy((size_t)B.coax_stride * 2);
+        coax_table(2, G, P.k, P.R[L + 1], P.R[L], P.R[L + 1], B.ps, B.pt, cy.data());
+        B.coax = (double2*)P.upload(cy);
+    }
+}
+
+void build_p2p(hfmm_plan& P) {
+    const Tree& t = P.tree;
+    const Lists& ls = P.lists;
+    const int nb = t.n_boxes();
+    auto csr = [nb](const std::vector<int>& tg, const std::vector<int>& sr, std::vector<int>& ptr, std::vector<int>& s) {
+        ptr.assign(nb + 1, 0);
+        for (int x : tg) ptr[x + 1]++;
+        for (int i = 0; i < nb; ++i) ptr[i + 1] += ptr[i];
+        s.assign(sr.size(), 0);
+        std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+        for (size_t e = 0; e < tg.size(); ++e) s[pos[tg[e]]++] = sr[e];
+    };
+    std::vector<int> up, us, wp, ws, xp, xs;
+    csr(ls.u_tgt, ls.u_src, up, us);
+    csr(ls.w_tgt, ls.w_src, wp, ws);
+    csr(ls.x_tgt, ls.x_src, xp, xs);
+    std::vector<int> leaf_box, rng_ptr{0}, rng_start, rng_pref, total;
+    std::vector<int> far_leaves;
+    int64_t pairs = 0;
+    for (int b = 0; b < nb; ++b) {
+        if (!t.is_leaf[b]) continue;
+        leaf_box.push_back(b);
+        if (t.level[b] >= 2) far_leaves.push_back(b);
+        int acc = 0;
+        auto add = [&](int box) {
+            rng_start.push_back(t.pstart[box]);
+            rng_pref.push_back(acc);
+            acc += t.count[box];
+        };
+        add(b);
+        for (int e = up[b]; e < up[b + 1]; ++e) add(us[e]);
+        for (int e = wp[b]; e < wp[b + 1]; ++e) add(ws[e]);
+        for (int a = b; a >= 0; a = t.parent[a])
+            for (int e = xp[a]; e < xp[a + 1]; ++e) add(xs[e]);
+        rng_ptr.push_back((int)rng_start.size());
+        total.push_back(acc);
+        pairs += (int64_t)acc * t.count[b];
+    }
+    P.n_leaves = (int)leaf_box.size();
+    P.n_far_leaves = (int)far_leaves.size();
+    P.n_p2p_pairs = pairs;
+    P.d_leaf_box = P.upload(leaf_box);
+    P.d_far_leaves = P.upload(far_leaves);
+    P.d_rng_ptr = P.upload(rng_ptr);
+    P.d_rng_start = P.upload(rng_start);
+    P.d_rng_pref = P.upload(rng_pref);
+    P.d_leaf_src_total = P.upload(total);
+}
+
+void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm_opts& o) {
+    if (!std::isfinite(k)) fail(HFMM_ERR_ARG, "k must be a finite real number");
+    if (n <= 0 || n > (int64_t)1 << 30) fail(HFMM_ERR_ARG, "positions must be a nonempty finite (N,3) array");
+    for (int64_t i = 0; i < 3 * n; ++i)
+        if (!std::isfinite(xyz[i])) fail(HFMM_ERR_ARG, "positions must be a nonempty finite (N,3) array");
+    if (o.p < 2 || o.n_max < 1) fail(HFMM_ERR_ARG, "p must be an integer >=2; N_max must be a positive integer");
+    if (o.p > kMaxP) fail(HFMM_ERR_ARG, "p exceeds the supported maximum of 40");
+    for (int L = 0; L < HFMM_MAX_LEVELS; ++L)
+        if (o.p_level[L] != 0 && (o.p_level[L] < 2 || o.p_level[L] > kMaxP))
+            fail(HFMM_ERR_ARG, "p_level entries must be 0 or in [2, 40]");
+    if (o.precision != HFMM_PREC_FP64 && o.precision != HFMM_PREC_MIXED) fail(HFMM_ERR_ARG, "unknown precision");
+    check_device();
+    upload_legendre_coefficients();
+    P.n = n;
+    P.k = std::fabs(k);
+    P.conj = k < 0;
+    P.precision = o.precision;
+    P.use_graph = o.use_graph;
+    P.tree = build_tree(xyz, n, o.n_max, o.max_depth);
+    P.lists = build_lists(P.tree);
+    const Tree& t = P.tree;
+    P.depth = t.depth;
+    P.pmax = 0;
+    for (int L = 0; L < kMaxLevels; ++L) {
+        P.p_level[L] = (L < HFMM_MAX_LEVELS && o.p_level[L] > 0) ? o.p_level[L] : o.p;
+        P.h[L] = t.extent * std::ldexp(1.0, -(L + 1));
+        P.R[L] = std::sqrt(3.0) * P.h[L];
+        if (L >= 1 && L <= t.depth) P.pmax = std::max(P.pmax, P.p_level[L]);
+    }
+    HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+    // sorted positions (SoA, FP64) and permutation
+    std::vector<double> X(n), Y(n), Z(n);
+    std::vector<int> perm(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t o2 = t.perm[i];
+        X[i] = xyz[3 * o2];
+        Y[i] = xyz[3 * o2 + 1];
+        Z[i] = xyz[3 * o2 + 2];
+        perm[i] = (int)o2;
+    }
+    P.dX = P.upload(X);
+    P.dY = P.upload(Y);
+    P.dZ = P.upload(Z);
+    P.d_perm = P.upload(perm);
+    // boxes
+    const int nb = t.n_boxes();
+    std::vector<double> centre(3 * (size_t)nb);
+    P.exp_off.assign(nb, 0);
+    int64_t off = 0;
+    for (int b = 0; b < nb; ++b) {
+        int gx, gy, gz;
+        morton_grid(t.code[b], gx, gy, gz);
+        const double s = t.extent * std::ldexp(1.0, -t.level[b]);
+        const double half = t.extent * std::ldexp(1.0, -(t.level[b] + 1));
+        centre[3 * b] = gx * s + half + t.pmin[0];
+        centre[3 * b + 1] = gy * s + half + t.pmin[1];
+        centre[3 * b + 2] = gz * s + half + t.pmin[2];
+        P.exp_off[b] = off;
+        off += ncoef(P.p_level[t.level[b]]);
+    }
+    P.n_coef = off;
+    P.d_centre = P.upload(centre);
+    P.d_level = P.upload(t.level);
+    P.d_pstart = P.upload(t.pstart);
+    P.d_count = P.upload(t.count);
+    P.d_expoff = P.upload(P.exp_off);
+    std::vector<int> pl(P.p_level, P.p_level + kMaxLevels);
+    std::vector<double> rl(P.R, P.R + kMaxLevels);
+    P.d_plevel = P.upload(pl);
+    P.d_R = P.upload(rl);
+    build_p2p(P);
+    build_tables(P);
+    // P2M chunk: points staged per pass, so that LDS stays within 60 KB
+    {
+        const int P1 = P.pmax + 1, tri = P1 * (P1 + 1) / 2;
+        const int per_point = 2 * P1 * (int)sizeof(double2) + tri * (int)sizeof(double);
+        P.p2m_chunk = std::max(1, std::min(32, 60 * 1024 / per_point));
+    }
+    P.dM = P.alloc<double2>(P.n_coef);
+    P.dL = P.alloc<double2>(P.n_coef);
+    P.d_q = P.alloc<double2>(n);
+    P.d_far = P.alloc<double2>(n);
+    P.d_qin = P.alloc<double2>(n);
+    P.d_out = P.alloc<double2>(n);
+    HIPCHK(hipDeviceSynchronize());
+}
+
+XlateArgs xargs(const hfmm_plan& P, const XlateLaunch& X, const double2* srcbuf, double2* dstbuf, int accumulate) {
+    XlateArgs a;
+    a.targets = X.targets;
+    a.ptr = X.ptr;
+    a.src = X.src;
+    a.dir = X.dir;
+    a.dinfo = P.d_dinfo;
+    a.phase = P.d_phase;
+    a.rot = P.d_rot;
+    a.coax = X.coax;
+    a.coax_stride = X.coax_stride;
+    a.ps = X.ps;
+    a.pt = X.pt;
+    a.pmax = P.pmax;
+    a.accumulate = accumulate;
+    a.srcbuf = srcbuf;
+    a.dstbuf = dstbuf;
+    a.exp_off = P.d_expoff;
+    return a;
+}
+
+// The device sequence of one evaluate.  events (optional, 8 entries) bracket phases.
+void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStream_t s, hipEvent_t* ev) {
+    auto mark = [&](int i) {
+        if (ev) HIPCHK(hipEventRecord(ev[i], s));
+    };
+    mark(0);
+    launch_gather_charges(q, q_type, P.d_perm, P.n, P.conj, P.d_q, s);
+    launch_fill_zero(P.dL, P.n_coef, s);
+    launch_fill_zero(P.d_far, P.n, s);
+    mark(1);
+    const DevBoxes bx = P.boxes();
+    const DevLevels lv = P.levels();
+    launch_p2m(P.d_far_leaves, P.n_far_leaves, P.p2m_chunk, bx, lv, P.dX, P.dY, P.dZ, P.d_q, P.k, P.pmax, P.dM, s);
+    mark(2);
+    for (int L = P.depth - 1; L >= 2; --L)
+        if (P.m2m[L].n_targets) launch_xlate(xargs(P, P.m2m[L], P.dM, P.dM, 0), P.m2m[L].n_targets, s);
+    mark(3);
+    for (int L = 2; L <= P.depth; ++L)
+        if (P.m2l[L].n_targets) launch_xlate(xargs(P, P.m2l[L], P.dM, P.dL, 1), P.m2l[L].n_targets, s);
+    mark(4);
+    for (int L = 2; L < P.depth; ++L)
+        if (P.l2l[L].n_targets) launch_xlate(xargs(P, P.l2l[L], P.dL, P.dL, 1), P.l2l[L].n_targets, s);
+    mark(5);
+    launch_l2p(P.d_far_leaves, P.n_far_leaves, bx, lv, P.dX, P.dY, P.dZ, P.dL, P.k, P.pmax, P.d_far, s);
+    mark(6);
+    launch_p2p(P.precision, P.n_leaves, P.d_leaf_box, bx, P.d_rng_ptr, P.d_rng_start, P.d_rng_pref,
+               P.d_leaf_src_total, P.dX, P.dY, P.dZ, P.d_q, P.k, P.d_far, P.d_perm,
+               (P.conj ? 1 : 0) | ((P.k == 0.0 && q_type == HFMM_F64) ? 2 : 0), out, s);
+    mark(7);
+}
+
+void evaluate_device(hfmm_plan& P, const void* q, int q_type, double2* out, hipStream_t s) {
+    if (!P.use_graph) {
+        run_sequence(P, q, q_type, out, s, nullptr);
+        return;
+    }
+    if (!P.gexec || P.g_q != q || P.g_out != out || P.g_qtype != q_type) {
+        if (P.gexec) {
+            HIPCHK(hipGraphExecDestroy(P.gexec));
+            P.gexec = nullptr;
+        }
+        if (P.graph) {
+            HIPCHK(hipGraphDestroy(P.graph));
+            P.graph = nullptr;
+        }
+        HIPCHK(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            run_sequence(P, q, q_type, out, P.stream, nullptr);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(P.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(P.stream, &P.graph));
+        HIPCHK(hipGraphInstantiate(&P.gexec, P.graph, nullptr, nullptr, 0));
+        P.g_q = q;
+        P.g_out = out;
+        P.g_qtype = q_type;
+    }
+    HIPCHK(hipGraphLaunch(P.gexec, s));
+}
+
+}  // namespace
+
+namespace {
+int guard_impl(void (*fn)(void*), void* ctx) {
+    try {
+        fn(ctx);
+        return HFMM_OK;
+    } catch (const Error& e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return HFMM_ERR_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return HFMM_ERR_INTERNAL;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return HFMM_ERR_INTERNAL;
+    }
+}
+template <typename F>
+int guarded(F&& f) {
+    return guard_impl([](void* c) { (*(F*)c)(); }, (void*)&f);
+}
+}  // namespace
+
+static void export_tree(const Tree& t, int* level, uint64_t* code, int* count, int* pstart, int* parent,
+                        uint8_t* is_leaf, int64_t* perm);
+static void export_list(const Lists& ls, char which, int* pairs);
+
+extern "C" {
+
+void hfmm_opts_default(hfmm_opts* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->p = 4;
+    o->n_max = 16;
+    o->max_depth = 0;
+    o->precision = HFMM_PREC_FP64;
+    o->use_graph = 1;
+}
+
+int hfmm_plan_create(const double* xyz, int64_t n, double k, const hfmm_opts* o, hfmm_plan** out) {
+    return guarded([&] {
+        if (!out || !o || !xyz) fail(HFMM_ERR_ARG, "null argument");
+        *out = nullptr;
+        hfmm_plan* P = new hfmm_plan();
+        try {
+            build_plan(*P, xyz, n, k, *o);
+        } catch (...) {
+            delete P;
+            throw;
+        }
+        *out = P;
+    });
+}
+
+int hfmm_plan_info_get(const hfmm_plan* P, hfmm_plan_info* info) {
+    return guarded([&] {
+        if (!P || !info) fail(HFMM_ERR_ARG, "null argument");
+        std::memset(info, 0, sizeof(*info));
+        info->n_points = P->n;
+        info->depth = P->depth;
+        info->n_boxes = P->tree.n_boxes();
+        info->n_leaves = P->n_leaves;
+        info->maxleaf = P->tree.maxleaf;
+        info->n_v = (int64_t)P->lists.v_tgt.size();
+        info->n_u = (int64_t)P->lists.u_tgt.size();
+        info->n_w = (int64_t)P->lists.w_tgt.size();
+        info->n_x = (int64_t)P->lists.x_tgt.size();
+        info->n_p2p_pairs = P->n_p2p_pairs;
+        for (int L = 0; L < HFMM_MAX_LEVELS; ++L) info->p_level[L] = (L >= 1 && L <= P->depth) ? P->p_level[L] : 0;
+        info->extent = P->tree.extent;
+        for (int a = 0; a < 3; ++a) info->pmin[a] = P->tree.pmin[a];
+    });
+}
+
+int hfmm_plan_export_tree(const hfmm_plan* P, int* level, uint64_t* code, int* count, int* pstart, int* parent,
+                          uint8_t* is_leaf, int64_t* perm) {
+    return guarded([&] {
+        if (!P) fail(HFMM_ERR_ARG, "null plan");
+        export_tree(P->tree, level, code, count, pstart, parent, is_leaf, perm);
+    });
+}
+
+int hfmm_plan_export_list(const hfmm_plan* P, char which, int* pairs) {
+    return guarded([&] {
+        if (!P || !pairs) fail(HFMM_ERR_ARG, "null argument");
+        export_list(P->lists, which, pairs);
+    });
+}
+
+int hfmm_evaluate(hfmm_plan* P, const void* q, int q_type, void* out, int ptr_kind, void* stream) {
+    return guarded([&] {
+        if (!P || !q || !out) fail(HFMM_ERR_ARG, "null argument");
+        if (q_type != HFMM_F64 && q_type != HFMM_C128) fail(HFMM_ERR_ARG, "unknown charge type");
+        hipStream_t s = stream ? (hipStream_t)stream : P->stream;
+        if (ptr_kind == HFMM_HOST) {
+            const size_t qbytes = (size_t)P->n * (q_type == HFMM_F64 ? 8 : 16);
+            HIPCHK(hipMemcpyAsync(P->d_qin, q, qbytes, hipMemcpyHostToDevice, s));
+            evaluate_device(*P, P->d_qin, q_type, P->d_out, s);
+            HIPCHK(hipMemcpyAsync(out, P->d_out, (size_t)P->n * 16, hipMemcpyDeviceToHost, s));
+        } else if (ptr_kind == HFMM_DEVICE) {
+            evaluate_device(*P, q, q_type, (double2*)out, s);
+        } else {
+            fail(HFMM_ERR_ARG, "unknown pointer kind");
+        }
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+int hfmm_benchmark(hfmm_plan* P, const void* q_dev, int q_type, void* out_dev, int warmup, int reps,
+                   double* median_ms, double* phase_ms) {
+    return guarded([&] {
+        if (!P || !q_dev || !out_dev || reps < 1) fail(HFMM_ERR_ARG, "bad argument");
+        hipStream_t s = P->stream;
+        for (int i = 0; i < warmup; ++i) evaluate_device(*P, q_dev, q_type, (double2*)out_dev, s);
+        HIPCHK(hipStreamSynchronize(s));
+        std::vector<hipEvent_t> ev(2 * reps);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        for (int i = 0; i < reps; ++i) {
+            HIPCHK(hipEventRecord(ev[2 * i], s));
+            evaluate_device(*P, q_dev, q_type, (double2*)out_dev, s);
+            HIPCHK(hipEventRecord(ev[2 * i + 1], s));
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        std::vector<double> ms(reps);
+        for (int i = 0; i < reps; ++i) {
+            float f = 0;
+            HIPCHK(hipEventElapsedTime(&f, ev[2 * i], ev[2 * i + 1]));
+            ms[i] = f;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        std::sort(ms.begin(), ms.end());
+        if (median_ms) *median_ms = ms[reps / 2];
+        if (phase_ms) {
+            hipEvent_t pe[8];
+            for (auto& e : pe) HIPCHK(hipEventCreate(&e));
+            run_sequence(*P, q_dev, q_type, (double2*)out_dev, s, pe);
+            HIPCHK(hipStreamSynchronize(s));
+            for (int i = 0; i < 7; ++i) {
+                float f = 0;
+                HIPCHK(hipEventElapsedTime(&f, pe[i], pe[i + 1]));
+                phase_ms[i] = f;
+            }
+            for (auto& e : pe) (void)hipEventDestroy(e);
+        }
+    });
+}
+
+int hfmm_plan_destroy(hfmm_plan* P) {
+    return guarded([&] { delete P; });
+}
+
+int hfmm_direct(const double* xyz, int64_t n, const double* q, double k, const int64_t* targets, int64_t nt,
+                double* out) {
+    return guarded([&] {
+        if (!xyz || !q || !out || (nt > 0 && !targets) || n <= 0 || nt < 0) fail(HFMM_ERR_ARG, "bad argument");
+        if (!std::isfinite(k)) fail(HFMM_ERR_ARG, "k must be a finite real number");
+        check_device();
+        std::vector<double> X(n), Y(n), Z(n), tx(nt), ty(nt), tz(nt);
+        for (int64_t i = 0; i < n; ++i) {
+            X[i] = xyz[3 * i];
+            Y[i] = xyz[3 * i + 1];
+            Z[i] = xyz[3 * i + 2];
+        }
+        for (int64_t i = 0; i < nt; ++i) {
+            if (targets[i] < 0 || targets[i] >= n) fail(HFMM_ERR_ARG, "target index out of range");
+            tx[i] = X[targets[i]];
+            ty[i] = Y[targets[i]];
+            tz[i] = Z[targets[i]];
+        }
+        hfmm_plan tmp;   // owns the temporaries
+        double* dX = tmp.upload(X);
+        double* dY = tmp.upload(Y);
+        double* dZ = tmp.upload(Z);
+        double* dtx = tmp.upload(tx);
+        double* dty = tmp.upload(ty);
+        double* dtz = tmp.upload(tz);
+        double2* dq = tmp.alloc<double2>(n);
+        HIPCHK(hipMemcpy(dq, q, (size_t)n * 16, hipMemcpyHostToDevice));
+        double2* dout = tmp.alloc<double2>(nt);
+        launch_direct(dX, dY, dZ, dq, n, k, dtx, dty, dtz, nt, dout, nullptr);
+        HIPCHK(hipDeviceSynchronize());
+        if (nt) HIPCHK(hipMemcpy(out, dout, (size_t)nt * 16, hipMemcpyDeviceToHost));
+    });
+}
+
+struct hfmm_tree {
+    Tree tree;
+    Lists lists;
+};
+
+int hfmm_tree_create(const double* xyz, int64_t n, int n_max, int max_depth, hfmm_tree** out) {
+    return guarded([&] {
+        if (!out || !xyz) fail(HFMM_ERR_ARG, "null argument");
+        *out = nullptr;
+        if (n <= 0 || n > (int64_t)1 << 30) fail(HFMM_ERR_ARG, "positions must be a nonempty finite (N,3) array");
+        for (int64_t i = 0; i < 3 * n; ++i)
+            if (!std::isfinite(xyz[i])) fail(HFMM_ERR_ARG, "positions must be a nonempty finite (N,3) array");
+        if (n_max < 1) fail(HFMM_ERR_ARG, "p must be an integer >=2; N_max must be a positive integer");
+        hfmm_tree* T = new hfmm_tree();
+        try {
+            T->tree = build_tree(xyz, n, n_max, max_depth);
+            T->lists = build_lists(T->tree);
+        } catch (...) {
+            delete T;
+            throw;
+        }
+        *out = T;
+    });
+}
+
+int hfmm_tree_info_get(const hfmm_tree* T, hfmm_plan_info* info) {
+    return guarded([&] {
+        if (!T || !info) fail(HFMM_ERR_ARG, "null argument");
+        std::memset(info, 0, sizeof(*info));
+        info->n_points = T->tree.n_points;
+        info->depth = T->tree.depth;
+        info->n_boxes = T->tree.n_boxes();
+        int nl = 0;
+        for (auto f : T->tree.is_leaf) nl += f;
+        info->n_leaves = nl;
+        info->maxleaf = T->tree.maxleaf;
+        info->n_v = (int64_t)T->lists.v_tgt.size();
+        info->n_u = (int64_t)T->lists.u_tgt.size();
+        info->n_w = (int64_t)T->lists.w_tgt.size();
+        info->n_x = (int64_t)T->lists.x_tgt.size();
+        info->extent = T->tree.extent;
+        for (int a = 0; a < 3; ++a) info->pmin[a] = T->tree.pmin[a];
+    });
+}
+
+static void export_tree(const Tree& t, int* level, uint64_t* code, int* count, int* pstart, int* parent,
+                        uint8_t* is_leaf, int64_t* perm) {
+    if (level) std::copy(t.level.begin(), t.level.end(), level);
+    if (code) std::copy(t.code.begin(), t.code.end(), code);
+    if (count) std::copy(t.count.begin(), t.count.end(), count);
+    if (pstart) std::copy(t.pstart.begin(), t.pstart.end(), pstart);
+    if (parent) std::copy(t.parent.begin(), t.parent.end(), parent);
+    if (is_leaf) std::copy(t.is_leaf.begin(), t.is_leaf.end(), is_leaf);
+    if (perm) std::copy(t.perm.begin(), t.perm.end(), perm);
+}
+
+static void export_list(const Lists& ls, char which, int* pairs) {
+    const std::vector<int>*a, *b;
+    switch (which) {
+        case 'V': a = &ls.v_tgt; b = &ls.v_src; break;
+        case 'U': a = &ls.u_tgt; b = &ls.u_src; break;
+        case 'W': a = &ls.w_tgt; b = &ls.w_src; break;
+        case 'X': a = &ls.x_tgt; b = &ls.x_src; break;
+        default: fail(HFMM_ERR_ARG, "list must be one of V, U, W, X");
+    }
+    for (size_t i = 0; i < a->size(); ++i) {
+        pairs[2 * i] = (*a)[i];
+        pairs[2 * i + 1] = (*b)[i];
+    }
+}
+
+int hfmm_tree_export(const hfmm_tree* T, int* level, uint64_t* code, int* count, int* pstart, int* parent,
+                     uint8_t* is_leaf, int64_t* perm) {
+    return guarded([&] {
+        if (!T) fail(HFMM_ERR_ARG, "null tree");
+        export_tree(T->tree, level, code, count, pstart, parent, is_leaf, perm);
+    });
+}
+
+int hfmm_tree_export_list(const hfmm_tree* T, char which, int* pairs) {
+    return guarded([&] {
+        if (!T || !pairs) fail(HFMM_ERR_ARG, "null argument");
+        export_list(T->lists, which, pairs);
+    });
+}
+
+int hfmm_tree_destroy(hfmm_tree* T) {
+    return guarded([&] { delete T; });
+}
+
+int hfmm_host_jhat(double x, int p, double* out) {
+    return guarded([&] {
+        if (!out || p < 0 || p > 4 * kMaxP) fail(HFMM_ERR_ARG, "bad argument");
+        jhat_host(x, p, out);
+    });
+}
+
+int hfmm_host_hhat(double x, int p, double* re, double* im) {
+    return guarded([&] {
+        if (!re || !im || p < 1 || p > 4 * kMaxP) fail(HFMM_ERR_ARG, "bad argument");
+        hhat_host(x, p, re, im);
+    });
+}
+
+int hfmm_host_rotation(double theta, int p, double* out) {
+    return guarded([&] {
+        if (!out || p < 0 || p > kMaxP) fail(HFMM_ERR_ARG, "bad argument");
+        rotation_table(theta, p, out);
+    });
+}
+
+int hfmm_host_coax(int kind, double k, double t, double rs, double rt, int ps, int pt, double* out) {
+    return guarded([&] {
+        if (!out || kind < 0 || kind > 2 || ps < 0 || pt < 0 || ps > kMaxP || pt > kMaxP || !(t > 0) ||
+            !(rs > 0) || !(rt > 0) || !(k >= 0))
+            fail(HFMM_ERR_ARG, "bad argument");
+        const Gaunt G = make_gaunt(std::max(ps, pt));
+        coax_table(kind, G, k, t, rs, rt, ps, pt, out);
+    });
+}
+
+const char* hfmm_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+/* libhfmm: MI355X-native 3-D Helmholtz FMM evaluator, C ABI.
+ *
+ * Kernel: phi(x_t) = sum_{s, x_s != x_t} q_s exp(i k |x_t - x_s|) / |x_t - x_s|
+ * (no 1/(4 pi) factor, self term excluded), as in the reference's HFMM.py:12-52
+ * and test.py:11-18.
+ *
+ * The reference exposes this path as Python functions, not as an FFI; the
+ * entry points below are what its `setup` / `evaluate` pair would bind to:
+ *   hfmm_plan_create   replaces HFMM.setup            (HFMM.py:55-80)
+ *   hfmm_evaluate      replaces HFMM.evaluate         (HFMM.py:83-91)
+ *   hfmm_plan_destroy  releases what setup's device_put held (jaxfmm/kifmm/setup.py:191)
+ *   hfmm_direct        replaces the FP64 direct sum   (test.py:11-18), on the GPU
+ * All calls return 0 on success or a negative status; no exception crosses the ABI.
+ * hfmm_last_error() returns the message of the calling thread's last failure.
+ * Plans are not thread-safe; distinct plans may be used from distinct threads.
+ */
+#ifndef HFMM_H
+#define HFMM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define HFMM_OK 0
+#define HFMM_ERR_ARG (-2)      /* invalid argument: the Python layer raises ValueError */
+#define HFMM_ERR_DEVICE (-3)   /* HIP runtime failure or no gfx950 device */
+#define HFMM_ERR_MEMORY (-4)   /* device allocation failed */
+#define HFMM_ERR_INTERNAL (-9)
+
+/* precision of the evaluate path */
+#define HFMM_PREC_FP64 0       /* FP64 everywhere (the plain FMM) */
+#define HFMM_PREC_MIXED 1      /* FP32 near field on leaf-centred coordinates; far field FP64 */
+
+/* pointer kinds for q / out */
+#define HFMM_HOST 0
+#define HFMM_DEVICE 1
+
+/* charge / output element types */
+#define HFMM_F64 0             /* real float64 charges */
+#define HFMM_C128 1            /* complex128, interleaved (re, im) */
+
+#define HFMM_MAX_LEVELS 24
+
+typedef struct hfmm_opts {
+    int p;                          /* truncation degree at every level (>= 2) */
+    int p_level[HFMM_MAX_LEVELS];   /* per-level degree; 0 = use p */
+    int n_max;                      /* leaf threshold N_max (>= 1) */
+    int max_depth;                  /* <= 0: automatic depth (reference rule) */
+    int precision;                  /* HFMM_PREC_* */
+    int use_graph;                  /* 1: replay evaluate as a captured hipGraph */
+} hfmm_opts;
+
+typedef struct hfmm_plan hfmm_plan;
+
+typedef struct hfmm_plan_info {
+    int64_t n_points;
+    int depth;
+    int n_boxes;
+    int n_leaves;
+    int maxleaf;
+    int64_t n_v, n_u, n_w, n_x;     /* list sizes (U without self pairs) */
+    int64_t n_p2p_pairs;            /* point pairs evaluated directly (incl. r = 0 pairs) */
+    int p_level[HFMM_MAX_LEVELS];   /* degree used at each level */
+    double extent;
+    double pmin[3];
+} hfmm_plan_info;
+
+void hfmm_opts_default(hfmm_opts* o);
+
+int hfmm_plan_create(const double* xyz /* N x 3, host */, int64_t n, double k, const hfmm_opts* o,
+                     hfmm_plan** out);
+int hfmm_plan_info_get(const hfmm_plan* plan, hfmm_plan_info* info);
+
+/* Tree export (canonical order: level ascending, Morton code ascending).  Each array
+ * pointer may be NULL; sizes are n_boxes (per-box arrays) or n_points (perm). */
+int hfmm_plan_export_tree(const hfmm_plan* plan, int* level, uint64_t* code, int* count, int* pstart,
+                          int* parent, uint8_t* is_leaf, int64_t* perm);
+/* List export: which = 'V', 'U', 'W', 'X'; pairs (target, source) as int32 [n][2]. */
+int hfmm_plan_export_list(const hfmm_plan* plan, char which, int* pairs);
+
+/* phi for all N points in input order.  q: N charges of q_type (HFMM_F64 or HFMM_C128);
+ * out: N complex128.  q and out are both host or both device pointers (ptr_kind).
+ * stream: a hipStream_t or NULL for the plan's own stream; the call returns after the
+ * result is complete. */
+int hfmm_evaluate(hfmm_plan* plan, const void* q, int q_type, void* out, int ptr_kind, void* stream);
+
+/* Time the device part of evaluate (inputs already resident): runs `reps` evaluates
+ * after `warmup`, returns per-evaluate milliseconds from hipEvents (median) and, if
+ * phase_ms is non-NULL, the per-phase milliseconds of one extra instrumented run:
+ * [0] gather, [1] P2M, [2] M2M, [3] M2L, [4] L2L, [5] L2P, [6] P2P. */
+int hfmm_benchmark(hfmm_plan* plan, const void* q_dev, int q_type, void* out_dev, int warmup, int reps,
+                   double* median_ms, double* phase_ms);
+
+int hfmm_plan_destroy(hfmm_plan* plan);
+
+/* FP64 direct sum on the GPU at selected targets: out[i] = phi(xyz[targets[i]]) from all
+ * n sources.  Host pointers; q complex128 interleaved. */
+int hfmm_direct(const double* xyz, int64_t n, const double* q, double k, const int64_t* targets,
+                int64_t n_targets, double* out);
+
+/* Host-only octree and lists (no device use).  Replaces the reference's tree build
+ * and list builders: build_fmm_tree_auto + assign_particles (jaxfmm/trees/cubic.py:530-695),
+ * build_v_eager / build_ulist_eager / build_wx_eager (jaxfmm/trees/lists.py:818-931).
+ * Export functions take the same arguments as the plan variants above. */
+typedef struct hfmm_tree hfmm_tree;
+int hfmm_tree_create(const double* xyz, int64_t n, int n_max, int max_depth, hfmm_tree** out);
+int hfmm_tree_info_get(const hfmm_tree* tree, hfmm_plan_info* info);
+int hfmm_tree_export(const hfmm_tree* tree, int* level, uint64_t* code, int* count, int* pstart, int* parent,
+                     uint8_t* is_leaf, int64_t* perm);
+int hfmm_tree_export_list(const hfmm_tree* tree, char which, int* pairs);
+int hfmm_tree_destroy(hfmm_tree* tree);
+
+/* Host FP64 operator-table generators, exposed so the tables can be checked without a
+ * GPU (no counterpart in the reference, whose Helmholtz path has no multipole code).
+ * jhat_n(x) = j_n(x)(2n+1)!!/x^n and hhat_n(x) = h_n(x)x^{n+1}/(2n-1)!!, n = 0..p;
+ * rotation: packed E^n(theta), Y_n^m(R_y(theta)u) = sum_m' E[m][m'] Y_n^m'(u);
+ * coax: kind 0 M2L, 1 M2M, 2 L2L, scaled coaxial operator along +z (complex, packed by m). */
+int hfmm_host_jhat(double x, int p, double* out);
+int hfmm_host_hhat(double x, int p, double* out_re, double* out_im);
+int hfmm_host_rotation(double theta, int p, double* out);
+int hfmm_host_coax(int kind, double k, double t, double rs, double rt, int ps, int pt, double* out);
+
+const char* hfmm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HFMM_H */

@@ -1,0 +1,168 @@
+"""GPU parity tests: the HIP FMM (through the C ABI) against the CPU oracle.
+
+Every case prints its name, sizes and seed before touching the GPU.  Bounds:
+* GPU FP64 direct sum vs the C oracle: relative error < 1e-13 (same FP64 sum);
+* FMM vs the FP64 direct oracle on sampled targets: the truncation tolerance stated
+  per case (degree-p solid-harmonic expansions);
+* invariants of the reference harness (test.py:63-72): complex128 and finite output,
+  linearity evaluate(-0.5 q) == -0.5 evaluate(q) to 1e-10, k = 0 gives an exactly real
+  result, errors fall as p grows.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import direct as D
+from oracle import tree as T
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def hf():
+    import hfmm0_amd
+    return hfmm0_amd
+
+
+def test_gpu_direct_matches_oracle(hf):
+    say("case: gpu direct N=3000 k=7")
+    x, q = D.uniform_problem(3000)
+    idx = D.sample_targets(3000, 300)
+    got = hf.direct(x, q, 7.0, idx)
+    ref = D.direct_c(x, q, 7.0, idx)
+    assert D.relative_error(got, ref) < 1e-13
+
+
+def _err(hf, x, q, k, p, nmax, **kw):
+    st = hf.setup(x, k, p=p, N_max=nmax, **kw)
+    out = hf.evaluate(st, q)
+    idx = D.sample_targets(len(x), 400)
+    ref = D.direct_c(x, q, k, idx)
+    return D.relative_error(out[idx], ref), out, st
+
+
+@pytest.mark.parametrize("k,p,tol", [(0.0, 8, 1e-4), (1.0, 8, 1e-4), (10.0, 10, 1e-4), (10.0, 16, 2e-7)])
+def test_fmm_uniform(hf, k, p, tol):
+    say(f"case: fmm uniform N=6000 k={k} p={p} N_max=32")
+    x, q = D.uniform_problem(6000)
+    e, out, st = _err(hf, x, q, k, p, 32)
+    say(f"  rel err {e:.3e} depth {st.depth}")
+    assert out.dtype == np.complex128 and np.isfinite(out).all()
+    assert e < tol
+
+
+def test_fmm_adaptive_clustered(hf):
+    say("case: fmm clustered N=4000 k=3 p=12 N_max=16")
+    rng = np.random.default_rng(42 + 4000)
+    x = rng.uniform(size=(4000, 3))
+    x[:2000] = rng.uniform(0.1, 0.25, (2000, 3))
+    q = rng.normal(size=4000)
+    e, out, st = _err(hf, x, q, 3.0, 12, 16)
+    info = st.info
+    say(f"  rel err {e:.3e} depth {st.depth} W {info.n_w} X {info.n_x}")
+    assert info.n_w > 0 and info.n_x > 0
+    assert e < 1e-5
+
+
+def test_tree_of_plan_matches_oracle(hf):
+    say("case: plan tree vs oracle N=5000 N_max=16 clustered")
+    rng = np.random.default_rng(9)
+    x = rng.uniform(size=(5000, 3))
+    x[:2500] = rng.uniform(0.3, 0.4, (2500, 3))
+    st = hf.setup(x, 2.0, p=4, N_max=16)
+    got = st.tree()
+    ref = T.build_tree(x, 16)
+    for key in ("level", "code", "count", "pstart", "parent", "is_leaf", "perm"):
+        np.testing.assert_array_equal(np.asarray(got[key]).astype(np.int64), np.asarray(ref[key]).astype(np.int64))
+
+
+def test_reference_harness_invariants(hf):
+    """test.py defaults: n in {256, 640}, k in {0, 1, 3}, leaf 16, real charges."""
+    for n in (256, 640):
+        rng = np.random.default_rng(42 + n)
+        x = rng.uniform(0.0, 1.0, (n, 3))
+        q = rng.normal(size=n)
+        for k in (0.0, 1.0, 3.0):
+            ref = D.direct_numpy(x, q, k)
+            base = None
+            for p in (3, 5, 8):
+                say(f"case: harness n={n} k={k} p={p}")
+                st = hf.setup(x, k, p=p, N_max=16)
+                out = hf.evaluate(st, q)
+                assert out.dtype == np.complex128 and np.isfinite(out).all()
+                errs = np.array([D.relative_error(a, b) for a, b in
+                                 ((out, ref), (out.real, ref.real), (out.imag, ref.imag))])
+                if k == 0:
+                    errs = errs[:2]
+                    assert np.count_nonzero(out.imag) == 0
+                assert max(errs) < (5e-2 if p <= 3 else 5e-3), errs
+                np.testing.assert_allclose(hf.evaluate(st, -0.5 * q), -0.5 * out, rtol=1e-10, atol=1e-10)
+                if base is not None:
+                    assert np.all(errs < np.maximum(0.8 * base, 2e-6)), (errs, base)
+                base = errs if base is None else base
+
+
+def test_complex_charges_are_linear_combination(hf):
+    say("case: complex charges N=3000 k=5 p=10")
+    x, q = D.uniform_problem(3000)
+    st = hf.setup(x, 5.0, p=10, N_max=32)
+    a = hf.evaluate(st, q)
+    b = hf.evaluate(st, q.real) + 1j * hf.evaluate(st, q.imag)
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(a).max())
+
+
+def test_negative_k_is_conjugate(hf):
+    say("case: negative k N=2000 k=-4")
+    x, q = D.uniform_problem(2000)
+    a = hf.evaluate(hf.setup(x, 4.0, p=10, N_max=32), q)
+    b = hf.evaluate(hf.setup(x, -4.0, p=10, N_max=32), np.conj(q))
+    np.testing.assert_allclose(b, np.conj(a), rtol=1e-12, atol=1e-12 * np.abs(a).max())
+
+
+def test_mixed_precision_near_field(hf):
+    say("case: mixed precision N=8000 k=20 p=14")
+    x, q = D.uniform_problem(8000)
+    e64, out64, _ = _err(hf, x, q, 20.0, 14, 64)
+    e32, out32, _ = _err(hf, x, q, 20.0, 14, 64, precision="mixed")
+    say(f"  fp64 {e64:.3e} mixed {e32:.3e}")
+    assert e32 < 2 * e64 + 1e-6
+    assert D.relative_error(out32, out64) < 1e-6
+
+
+def test_graph_replay_matches_stream_run(hf):
+    say("case: graph vs eager N=4000 k=6 p=9")
+    x, q = D.uniform_problem(4000)
+    a = hf.setup(x, 6.0, p=9, N_max=32, use_graph=True)
+    b = hf.setup(x, 6.0, p=9, N_max=32, use_graph=False)
+    ra = [hf.evaluate(a, q) for _ in range(3)]
+    rb = hf.evaluate(b, q)
+    for r in ra:
+        np.testing.assert_array_equal(r, rb)
+
+
+def test_device_pointer_path(hf):
+    import torch
+    say("case: device tensors N=5000 k=8 p=10")
+    x, q = D.uniform_problem(5000)
+    st = hf.setup(x, 8.0, p=10, N_max=32)
+    host = hf.evaluate(st, q)
+    qd = torch.from_numpy(q).to("cuda")
+    od = torch.empty(5000, dtype=torch.complex128, device="cuda")
+    hf.evaluate_device(st, qd, od)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(od.cpu().numpy(), host)
+
+
+def test_per_level_degrees(hf):
+    say("case: per-level p N=20000 k=10 p_level=[10,12,10,9]")
+    x, q = D.uniform_problem(20000)
+    e, _, st = _err(hf, x, q, 10.0, 10, 64, p_level=[10, 12, 10, 9])
+    say(f"  rel err {e:.3e} levels {st.p_level}")
+    assert st.p_level == [10, 12, 10]
+    assert e < 1e-4

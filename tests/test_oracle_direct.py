@@ -1,0 +1,77 @@
+"""Pins for the direct-sum oracle: closed forms, identities, and C/NumPy agreement."""
+import numpy as np
+import pytest
+
+from oracle import direct
+
+
+def test_two_points_closed_form():
+    x = np.array([[0.0, 0.0, 0.0], [0.3, 0.4, 0.0]])          # r = 0.5
+    q = np.array([2.0, -1.0 + 0.5j])
+    k = 3.0
+    g = np.exp(1j * k * 0.5) / 0.5
+    want = np.array([g * q[1], g * q[0]])
+    for f in (direct.direct_numpy, direct.direct_c):
+        got = f(x, q, k)
+        np.testing.assert_allclose(got, want, rtol=1e-15, atol=0)
+
+
+def test_coincident_points_excluded():
+    x = np.array([[0.1, 0.2, 0.3], [0.1, 0.2, 0.3], [1.1, 0.2, 0.3]])
+    q = np.array([1.0, 5.0, 1.0])
+    for f in (direct.direct_numpy, direct.direct_c):
+        got = f(x, q, 2.0)
+        g = np.exp(2j) / 1.0
+        np.testing.assert_allclose(got, [g, g, 6 * g], rtol=1e-15)
+
+
+def test_polygon_coulomb_k0():
+    # n unit charges on a unit circle: every point sees sum_{j=1}^{n-1} 1/(2 sin(pi j/n))
+    n = 12
+    t = 2 * np.pi * np.arange(n) / n
+    x = np.stack([np.cos(t), np.sin(t), np.zeros(n)], 1)
+    want = sum(1.0 / (2 * np.sin(np.pi * j / n)) for j in range(1, n))
+    for f in (direct.direct_numpy, direct.direct_c):
+        got = f(x, np.ones(n), 0.0)
+        np.testing.assert_allclose(got.real, want, rtol=1e-14)
+        assert np.all(got.imag == 0)
+
+
+def test_polygon_helmholtz():
+    n = 9
+    k = 4.5
+    t = 2 * np.pi * np.arange(n) / n
+    x = np.stack([np.cos(t), np.sin(t), np.full(n, 0.7)], 1)
+    want = sum(np.exp(1j * k * 2 * np.sin(np.pi * j / n)) / (2 * np.sin(np.pi * j / n)) for j in range(1, n))
+    got = direct.direct_c(x, np.ones(n), k)
+    np.testing.assert_allclose(got, want, rtol=1e-13)
+
+
+def test_linearity_and_reciprocity():
+    rng = np.random.default_rng(7)
+    x = rng.uniform(size=(300, 3))
+    q1 = rng.normal(size=300) + 1j * rng.normal(size=300)
+    q2 = rng.normal(size=300)
+    k = 7.0
+    a = direct.direct_c(x, q1, k)
+    b = direct.direct_c(x, q2, k)
+    c = direct.direct_c(x, 2 * q1 - 3 * q2, k)
+    np.testing.assert_allclose(c, 2 * a - 3 * b, rtol=1e-12, atol=1e-12)
+    # reciprocity: sum_t q2_t phi_1(t) == sum_s q1_s phi_2(s) (symmetric kernel)
+    np.testing.assert_allclose(np.dot(q2, a), np.dot(q1, b), rtol=1e-12)
+
+
+def test_c_matches_numpy_and_samples():
+    x, q = direct.uniform_problem(2000)
+    idx = direct.sample_targets(2000, 100)
+    a = direct.direct_numpy(x, q, 10.0, idx)
+    b = direct.direct_c(x, q, 10.0, idx)
+    assert direct.relative_error(b, a) < 1e-14
+    assert len(np.unique(idx)) == 100
+
+
+def test_negative_k_is_conjugate():
+    x, q = direct.uniform_problem(400)
+    a = direct.direct_c(x, q, 5.0)
+    b = direct.direct_c(x, np.conj(q), -5.0)
+    np.testing.assert_allclose(b, np.conj(a), rtol=1e-13, atol=1e-13)
