@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Headline benchmark: N=1M, k=40 Helmholtz FMM evaluations/s at rel err <= 1e-6,
+tuned form (mixed precision, per-level degree) vs the plain FP64 fixed-degree FMM.
+
+Workload (uniform cube, the GPU-at-scale configuration): N = 1,000,000 points uniform
+in [0,1]^3 (seed 42+N), complex charges N(0,1) + i N(0,1), k = 40, at most 128 points
+per leaf.  One step = one evaluate (all N targets, one charge vector) with inputs
+resident in HBM, replayed as a hipGraph.  Relative error is measured against the FP64
+direct sum (GPU) on 4096 sampled targets.
+
+Prints ONE JSON line (rank 0, N=1 only).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PLAIN_P = 20                         # fixed degree of the plain FP64 FMM
+TUNED_P = "20,16,14,14"              # degrees at levels 2.. of the tuned form
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
+
+
+def parse_levels(s, depth):
+    v = [int(a) for a in s.split(",")]
+    lv = [v[0]] + v                  # level 1 has no far field; mirror level 2
+    while len(lv) < depth:
+        lv.append(lv[-1])
+    return lv[:depth]
+
+
+def v_pairs_per_level(st):
+    import numpy as np
+    t = st.tree()
+    lvl = t["level"][t["V"][:, 0]] if len(t["V"]) else np.zeros(0, int)
+    return {int(l): int((lvl == l).sum()) for l in range(1, st.depth + 1)}, t
+
+
+def run_form(hf, x, q_dev, out_dev, k, leaf, p, p_level, precision, steps, warmup, idx, ref):
+    import torch
+    import numpy as np
+    from hfmm0_amd.workloads import f_m2l
+    t0 = time.perf_counter()
+    st = hf.setup(x, k, p=p, N_max=leaf, p_level=p_level, precision=precision)
+    setup_s = time.perf_counter() - t0
+    for _ in range(warmup):
+        hf.evaluate_device(st, q_dev, out_dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hf.evaluate_device(st, q_dev, out_dev)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    res = out_dev.cpu().numpy()
+    err = float(np.linalg.norm(res[idx] - ref) / np.linalg.norm(ref))
+    med, phases = hf.benchmark(st, q_dev, out_dev, warmup=1, reps=max(3, steps), phases=True)
+    vpl, tree = v_pairs_per_level(st)
+    m2l_flops = sum(f_m2l(st.p_level[l - 1]) * n for l, n in vpl.items())
+    info = st.info
+    return dict(st=st, ms=ms, event_ms=med, err=err, phases=phases, m2l_flops=m2l_flops, setup_s=setup_s,
+                n_p2p_pairs=int(info.n_p2p_pairs), depth=st.depth, p_level=st.p_level, v_per_level=vpl,
+                n_v=int(info.n_v))
+
+
+def cpu_baseline(x, q, k, budget_s=12.0):
+    """Oracle FP64 direct sum (C, OpenMP) on the host cores, timed on a bounded sample of
+    targets of the same workload, scaled to evaluations/s."""
+    import numpy as np
+    from oracle import direct as D
+    D.build()
+    cores = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        cores = min(cores, int(omp))
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    n = len(x)
+    nt = 64
+    t0 = time.perf_counter()
+    D.direct_c(x, q, k, np.arange(nt))
+    dt = time.perf_counter() - t0
+    nt2 = int(max(64, min(n, nt * budget_s / max(dt, 1e-3))))
+    idx = np.random.default_rng(1).choice(n, nt2, replace=False)
+    t0 = time.perf_counter()
+    D.direct_c(x, q, k, idx)
+    dt2 = time.perf_counter() - t0
+    return {"value": nt2 / dt2, "unit": "evals/s", "cores": cores, "kind": "port",
+            "sample": f"oracle FP64 direct sum (oracle/direct.c, OpenMP) at {nt2} sampled targets x {n} sources, "
+                      f"{dt2:.1f} s; value = targets/s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--k", type=float, default=40.0)
+    ap.add_argument("--leaf", type=int, default=128)
+    ap.add_argument("--plain-p", type=int, default=PLAIN_P)
+    ap.add_argument("--tuned-p", type=str, default=TUNED_P)
+    ap.add_argument("--tuned-precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--samples", type=int, default=4096)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import hfmm0_amd as hf
+    from hfmm0_amd.workloads import P2P_FLOPS_PER_PAIR, sample_targets, uniform_cube
+
+    x, q = uniform_cube(args.n)
+    idx = sample_targets(args.n, args.samples)
+    t0 = time.perf_counter()
+    ref = hf.direct(x, q, args.k, idx)
+    direct_s = time.perf_counter() - t0
+    q_dev = torch.from_numpy(q).to("cuda")
+    out_dev = torch.empty(args.n, dtype=torch.complex128, device="cuda")
+
+    # depth of the tree (auto rule) to expand the per-level degree list
+    from hfmm0_amd.tree import build_tree
+    depth = build_tree(x, args.leaf)["depth"]
+    tuned_levels = parse_levels(args.tuned_p, depth)
+    tuned = run_form(hf, x, q_dev, out_dev, args.k, args.leaf, max(tuned_levels), tuned_levels,
+                     args.tuned_precision, args.steps, args.warmup, idx, ref)
+    plain = None
+    if not args.no_plain:
+        plain = run_form(hf, x, q_dev, out_dev, args.k, args.leaf, args.plain_p, None, "fp64",
+                         max(2, args.steps // 2), 1, idx, ref)
+
+    value = args.n / (tuned["ms"] * 1e-3)
+    m2l_ms = tuned["phases"]["m2l"]
+    achieved = tuned["m2l_flops"] / (m2l_ms * 1e-3)
+    p2p_ms = tuned["phases"]["p2p"]
+    line = {
+        "metric": "N=1M k=40 FMM evals/s at rel err 1e-6: tuned form vs plain FP64 GPU",
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tuned["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 near field + f64 far field" if args.tuned_precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"uniform cube N={args.n}, k={args.k:g}, leaf {args.leaf}, complex charges",
+                   "n": args.n, "k": args.k, "leaf": args.leaf, "depth": tuned["depth"],
+                   "tuned_p_level": tuned["p_level"], "plain_p": args.plain_p,
+                   "samples": len(idx)},
+        "rel_err": tuned["err"],
+        "roofline": {"bound": "mfma", "kernel": "m2l (xlate_kernel over V pairs, all levels)",
+                     "achieved": achieved / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK["fp64"], "traffic": None},
+        "phases_ms": tuned["phases"],
+        "p2p": {"pairs": tuned["n_p2p_pairs"], "ms": p2p_ms,
+                "algorithmic_tflops": P2P_FLOPS_PER_PAIR * tuned["n_p2p_pairs"] / (p2p_ms * 1e-3) / 1e12},
+        "graph_event_ms": tuned["event_ms"],
+        "setup_s": tuned["setup_s"],
+        "direct_ref_s": direct_s,
+    }
+    if plain is not None:
+        line["plain_fp64"] = {"evals_per_s": args.n / (plain["ms"] * 1e-3), "ms_per_step": plain["ms"],
+                              "rel_err": plain["err"], "p": args.plain_p, "phases_ms": plain["phases"]}
+        line["speedup_vs_plain"] = plain["ms"] / tuned["ms"]
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(x, q, args.k)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

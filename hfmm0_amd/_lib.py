@@ -91,6 +91,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise HFMMError(f"{LIB_PATH} is missing: build it with `python -m hfmm0_amd.build` "
                             "(there is no non-HIP implementation)")
+        # torch (ROCm build) ships its own libamdhip64.so with the soname libamdhip64.so.7.
+        # Loading torch first makes libhfmm bind to that already-loaded runtime, so a
+        # process that also uses torch holds a single HIP runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         l = ctypes.CDLL(LIB_PATH)
         _declare(l)
         _lib = l

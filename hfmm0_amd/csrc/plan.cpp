@@ -627,12 +627,15 @@ int hfmm_benchmark(hfmm_plan* P, const void* q_dev, int q_type, void* out_dev, i
         if (phase_ms) {
             hipEvent_t pe[8];
             for (auto& e : pe) HIPCHK(hipEventCreate(&e));
-            run_sequence(*P, q_dev, q_type, (double2*)out_dev, s, pe);
-            HIPCHK(hipStreamSynchronize(s));
-            for (int i = 0; i < 7; ++i) {
-                float f = 0;
-                HIPCHK(hipEventElapsedTime(&f, pe[i], pe[i + 1]));
-                phase_ms[i] = f;
+            for (int i = 0; i < 7; ++i) phase_ms[i] = 0.0;
+            for (int r = 0; r < reps; ++r) {
+                run_sequence(*P, q_dev, q_type, (double2*)out_dev, s, pe);
+                HIPCHK(hipStreamSynchronize(s));
+                for (int i = 0; i < 7; ++i) {
+                    float f = 0;
+                    HIPCHK(hipEventElapsedTime(&f, pe[i], pe[i + 1]));
+                    phase_ms[i] += f / reps;
+                }
             }
             for (auto& e : pe) (void)hipEventDestroy(e);
         }

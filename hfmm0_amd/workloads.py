@@ -1,0 +1,30 @@
+"""Synthetic inputs of the benchmark workloads (seeded, as in the reference harness:
+``default_rng(seed + N)``, positions drawn first, then charges; test.py:40-44)."""
+import numpy as np
+
+
+def uniform_cube(n, complex_charges=True, seed=42):
+    """N points uniform in [0,1]^3; charges N(0,1) (+ i N(0,1))."""
+    rng = np.random.default_rng(seed + n)
+    x = rng.uniform(0.0, 1.0, (n, 3))
+    q = rng.normal(size=n)
+    if complex_charges:
+        q = q + 1j * rng.normal(size=n)
+    return x, q
+
+
+def sample_targets(n, n_samp):
+    """Sampled-target set for error figures: default_rng(0).choice(N, n, replace=False)."""
+    return np.sort(np.random.default_rng(0).choice(n, min(n_samp, n), replace=False))
+
+
+def f_m2l(p):
+    """Algorithmic flops of one rotate / coaxial / rotate-back M2L at degree p:
+    8 S_rot + 8 S_z + 12 (p+1)^2 with S_rot = sum_n (2n+1)^2 (two real rotations of a
+    complex vector) and S_z = sum_m (p+1-|m|)^2 (complex coaxial translation)."""
+    s_rot = (p + 1) * (2 * p + 1) * (2 * p + 3) // 3
+    s_z = sum((p + 1 - abs(m)) ** 2 for m in range(-p, p + 1))
+    return 8 * s_rot + 8 * s_z + 12 * (p + 1) ** 2
+
+
+P2P_FLOPS_PER_PAIR = 23      # 3 sub, 5 r^2, rsqrt, r, kr, sin, cos, 2 scale, 8 complex MAC

@@ -88,8 +88,9 @@ int hfmm_evaluate(hfmm_plan* plan, const void* q, int q_type, void* out, int ptr
 
 /* Time the device part of evaluate (inputs already resident): runs `reps` evaluates
  * after `warmup`, returns per-evaluate milliseconds from hipEvents (median) and, if
- * phase_ms is non-NULL, the per-phase milliseconds of one extra instrumented run:
- * [0] gather, [1] P2M, [2] M2M, [3] M2L, [4] L2L, [5] L2P, [6] P2P. */
+ * phase_ms is non-NULL, per-phase milliseconds averaged over `reps` further
+ * instrumented (non-graph) runs with hipEvents between phases on the plan's stream:
+ * [0] gather+clear, [1] P2M, [2] M2M, [3] M2L, [4] L2L, [5] L2P, [6] P2P. */
 int hfmm_benchmark(hfmm_plan* plan, const void* q_dev, int q_type, void* out_dev, int warmup, int reps,
                    double* median_ms, double* phase_ms);
 
