@@ -19,14 +19,16 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PLAIN_P = 20                         # fixed degree of the plain FP64 FMM
-TUNED_P = "20,16,14,14"              # degrees at levels 2.. of the tuned form
+PLAIN_P = 17                         # smallest fixed degree of the plain FP64 FMM with rel err <= 1e-6
+TUNED_P = "19,15,12,11"              # degrees at levels 2, 3, ... of the tuned form (last one repeats)
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
 
 
 def parse_levels(s, depth):
+    """Degrees for levels 1..depth from a list for levels 2, 3, ...; level 1 carries no
+    far field and takes the level-2 degree."""
     v = [int(a) for a in s.split(",")]
-    lv = [v[0]] + v                  # level 1 has no far field; mirror level 2
+    lv = [v[0]] + v
     while len(lv) < depth:
         lv.append(lv[-1])
     return lv[:depth]

@@ -64,12 +64,15 @@ class HFMMState:
         return _export(self.handle, self.info, lib.hfmm_plan_export_tree, lib.hfmm_plan_export_list)
 
 
-def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, use_graph=True):
+def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, use_graph=True,
+          m2l_kernel="auto"):
     """Build the octree, lists and operator tables; upload everything to the GPU.
 
     Validation order and messages follow ``HFMM.py:57-64`` and ``HFMM.py:14-16``.
     ``p_level`` optionally gives the truncation degree per tree level (index 0 is
-    level 1); ``precision`` is ``"fp64"`` (plain FMM) or ``"mixed"`` (FP32 near field).
+    level 1); ``precision`` is ``"fp64"`` (plain FMM) or ``"mixed"`` (FP32 near field and
+    FP32 M2L on matrix cores).  ``m2l_kernel="generic"`` forces the thread-per-coefficient
+    M2L kernel instead of the matrix-core block kernel.
     """
     x = np.asarray(positions, dtype=np.float64)
     if x.ndim != 2 or x.shape[1] != 3 or len(x) == 0 or not np.isfinite(x).all():
@@ -89,6 +92,9 @@ def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_le
     o.max_depth = int(max_depth) if max_depth is not None else 0
     o.precision = PRECISIONS[precision]
     o.use_graph = 1 if use_graph else 0
+    if m2l_kernel not in ("auto", "generic"):
+        raise ValueError("m2l_kernel must be 'auto' or 'generic'")
+    o.m2l_kernel = 0 if m2l_kernel == "auto" else 1
     if p_level is not None:
         for i, v in enumerate(p_level):
             if i + 1 >= L.MAX_LEVELS:

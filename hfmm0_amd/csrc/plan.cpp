@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -31,6 +32,22 @@ constexpr int kDirV = 0;         // 343 same-level displacements
 constexpr int kDirM2M = 343;     // 8 child octants, t = c_parent - c_child
 constexpr int kDirL2L = 351;     // 8 child octants, t = c_child - c_parent
 constexpr int kNumDirs = 359;
+
+struct M2LBlockLevel {
+    bool use = false;
+    int tb = 0;
+    int n_blocks = 0;
+    int* targets = nullptr;
+    int* ptr = nullptr;
+    unsigned short* dir = nullptr;
+    int* src = nullptr;
+    int* units1 = nullptr;
+    int n_units1 = 0;
+    int* units2 = nullptr;
+    int n_units2 = 0;
+    void* coax = nullptr;          // table in the kernel's precision
+    int coax_stride = 0;
+};
 
 struct XlateLaunch {
     int n_targets = 0;
@@ -79,6 +96,12 @@ struct hfmm_plan {
     double2* d_phase = nullptr;
     DispInfo* d_dinfo = nullptr;
     XlateLaunch m2l[kMaxLevels], m2m[kMaxLevels], l2l[kMaxLevels];
+    M2LBlockLevel m2lb[kMaxLevels];
+    int m2l_kernel = 0;
+    size_t max_lds = 65536;
+    float* d_rot32 = nullptr;
+    float2* d_phase32 = nullptr;
+    float2* dM32 = nullptr;
     double2 *dM = nullptr, *dL = nullptr, *d_q = nullptr, *d_far = nullptr;
     void* d_qin = nullptr;          // staging for host input
     double2* d_out = nullptr;       // staging for host output
@@ -130,6 +153,94 @@ void check_device() {
 
 int octant(uint64_t code) { return (int)(code & 7); }
 
+// Target blocks of the matrix-core M2L: boxes of level L with the same octant, TB at a
+// time in Morton order; per block the displacements present (sorted by rotation and
+// distance) with the source box of every target (-1 if that target lacks it).
+void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, const std::vector<int>& ptr,
+                      const std::vector<int>& src, const std::vector<unsigned short>& dir,
+                      const std::vector<double>& coax_d, int n_dist, const std::vector<DispInfo>& dinfo) {
+    if (P.m2l_kernel != 0) return;
+    const int p = P.p_level[L];
+    const int fp32 = P.precision == HFMM_PREC_MIXED;
+    const int eb = fp32 ? 4 : 8;
+    int tb = 0;
+    for (int c : {16, 8, 4})
+        if (!tb && m2l_block_lds_bytes(p, c, eb) <= 80 * 1024) tb = c;
+    for (int c : {16, 8, 4})
+        if (!tb && m2l_block_lds_bytes(p, c, eb) <= P.max_lds) tb = c;
+    if (!tb) return;
+    const int ncol = 2 * tb, nct = (ncol + 15) / 16;
+    std::vector<std::pair<int, int>> u1, u2;   // (cost, code)
+    for (int n = 0; n <= p; ++n)
+        for (int mt = 0; mt < (2 * n + 1 + 15) / 16; ++mt)
+            for (int ct = 0; ct < nct; ++ct) u1.push_back({(2 * n + 1 + 3) / 4, n | (mt << 8) | (ct << 16)});
+    for (int m = -p; m <= p; ++m) {
+        const int len = p + 1 - std::abs(m);
+        for (int mt = 0; mt < (len + 15) / 16; ++mt)
+            for (int ct = 0; ct < nct; ++ct) u2.push_back({2 * ((len + 3) / 4), (m + p) | (mt << 8) | (ct << 16)});
+    }
+    auto by_cost = [](const std::pair<int, int>& a, const std::pair<int, int>& b) { return a.first > b.first; };
+    std::stable_sort(u1.begin(), u1.end(), by_cost);
+    std::stable_sort(u2.begin(), u2.end(), by_cost);
+    if ((int)(u1.size() + 3) / 4 > 48) return;
+    std::vector<int> units1, units2;
+    for (auto& x : u1) units1.push_back(x.second);
+    for (auto& x : u2) units2.push_back(x.second);
+
+    const Tree& t = P.tree;
+    std::vector<int> tpos(t.n_boxes(), -1);     // target ordinal in `targets`
+    for (size_t i = 0; i < targets.size(); ++i) tpos[targets[i]] = (int)i;
+    std::vector<int> blk_targets, blk_ptr{0}, ent_src;
+    std::vector<unsigned short> ent_dir;
+    for (int o = 0; o < 8; ++o) {
+        std::vector<int> boxes;
+        for (int b = t.level_start[L]; b < t.level_start[L + 1]; ++b)
+            if ((int)(t.code[b] & 7) == o && tpos[b] >= 0) boxes.push_back(b);
+        for (size_t b0 = 0; b0 < boxes.size(); b0 += tb) {
+            std::map<std::tuple<int, int, int>, std::vector<int>> ents;   // (theta, dist, dir) -> sources
+            for (int j = 0; j < tb; ++j) {
+                const int b = b0 + j < boxes.size() ? boxes[b0 + j] : -1;
+                blk_targets.push_back(b);
+                if (b < 0) continue;
+                const int ti = tpos[b];
+                for (int e = ptr[ti]; e < ptr[ti + 1]; ++e) {
+                    const int d = dir[e];
+                    auto key = std::make_tuple(dinfo[d].theta, dinfo[d].dist, d);
+                    auto it = ents.find(key);
+                    if (it == ents.end()) it = ents.emplace(key, std::vector<int>(tb, -1)).first;
+                    it->second[j] = src[e];
+                }
+            }
+            for (auto& kv : ents) {
+                ent_dir.push_back((unsigned short)std::get<2>(kv.first));
+                for (int j = 0; j < tb; ++j) ent_src.push_back(kv.second[j]);
+            }
+            blk_ptr.push_back((int)ent_dir.size());
+        }
+    }
+    M2LBlockLevel& B = P.m2lb[L];
+    B.use = true;
+    B.tb = tb;
+    B.n_blocks = (int)blk_ptr.size() - 1;
+    B.targets = P.upload(blk_targets);
+    B.ptr = P.upload(blk_ptr);
+    B.dir = P.upload(ent_dir);
+    B.src = P.upload(ent_src);
+    B.units1 = P.upload(units1);
+    B.n_units1 = (int)units1.size();
+    B.units2 = P.upload(units2);
+    B.n_units2 = (int)units2.size();
+    B.coax_stride = coax_size(p, p);
+    if (fp32) {
+        std::vector<float2> c32((size_t)n_dist * B.coax_stride);
+        for (size_t i = 0; i < c32.size(); ++i) c32[i] = make_float2((float)coax_d[2 * i], (float)coax_d[2 * i + 1]);
+        B.coax = P.upload(c32);
+    } else {
+        B.coax = P.upload(coax_d);
+    }
+    prepare_m2l_block(p, B.n_units1, tb, fp32);
+}
+
 void build_tables(hfmm_plan& P) {
     const Tree& t = P.tree;
     // directions
@@ -173,11 +284,20 @@ void build_tables(hfmm_plan& P) {
     std::vector<double> rot((size_t)thetas.size() * rsz);
     for (size_t i = 0; i < thetas.size(); ++i) rotation_table(thetas[i], pmax, &rot[i * rsz]);
     P.d_rot = P.upload(rot);
+    if (P.precision == HFMM_PREC_MIXED) {
+        std::vector<float> r32(rot.begin(), rot.end());
+        P.d_rot32 = P.upload(r32);
+    }
     std::vector<double2> phase((size_t)kNumDirs * (pmax + 1));
     for (int d = 0; d < kNumDirs; ++d)
         for (int m = 0; m <= pmax; ++m)
             phase[(size_t)d * (pmax + 1) + m] = make_double2(std::cos(m * phis[d]), std::sin(m * phis[d]));
     P.d_phase = P.upload(phase);
+    if (P.precision == HFMM_PREC_MIXED) {
+        std::vector<float2> ph32(phase.size());
+        for (size_t i = 0; i < phase.size(); ++i) ph32[i] = make_float2((float)phase[i].x, (float)phase[i].y);
+        P.d_phase32 = P.upload(ph32);
+    }
     P.d_dinfo = P.upload(dinfo);
 
     const Gaunt G = make_gaunt(pmax);
@@ -219,6 +339,7 @@ void build_tables(hfmm_plan& P) {
                 coax_table(0, G, P.k, 2.0 * P.h[L] * std::sqrt((double)dist_d2[i]), P.R[L], P.R[L], X.ps, X.pt,
                            &cx[i * X.coax_stride * 2]);
             X.coax = (double2*)P.upload(cx);
+            build_m2l_blocks(P, L, targets, ptr, src, dir, cx, (int)dist_d2.size(), dinfo);
         }
     }
     // M2M (parent level L <- children at L+1) and L2L (parent level L -> children at L+1)
@@ -336,6 +457,13 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
     P.conj = k < 0;
     P.precision = o.precision;
     P.use_graph = o.use_graph;
+    P.m2l_kernel = o.m2l_kernel;
+    {
+        int dev = 0, lds = 65536;
+        HIPCHK(hipGetDevice(&dev));
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+            P.max_lds = (size_t)lds;
+    }
     P.tree = build_tree(xyz, n, o.n_max, o.max_depth);
     P.lists = build_lists(P.tree);
     const Tree& t = P.tree;
@@ -397,6 +525,7 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
         P.p2m_chunk = std::max(1, std::min(32, 60 * 1024 / per_point));
     }
     P.dM = P.alloc<double2>(P.n_coef);
+    if (P.precision == HFMM_PREC_MIXED) P.dM32 = P.alloc<float2>(P.n_coef);
     P.dL = P.alloc<double2>(P.n_coef);
     P.d_q = P.alloc<double2>(n);
     P.d_far = P.alloc<double2>(n);
@@ -443,8 +572,42 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
     for (int L = P.depth - 1; L >= 2; --L)
         if (P.m2m[L].n_targets) launch_xlate(xargs(P, P.m2m[L], P.dM, P.dM, 0), P.m2m[L].n_targets, s);
     mark(3);
-    for (int L = 2; L <= P.depth; ++L)
-        if (P.m2l[L].n_targets) launch_xlate(xargs(P, P.m2l[L], P.dM, P.dL, 1), P.m2l[L].n_targets, s);
+    const bool fp32 = P.precision == HFMM_PREC_MIXED;
+    bool converted = false;
+    for (int L = 2; L <= P.depth; ++L) {
+        if (!P.m2l[L].n_targets) continue;
+        const M2LBlockLevel& B = P.m2lb[L];
+        if (!B.use) {
+            launch_xlate(xargs(P, P.m2l[L], P.dM, P.dL, 1), P.m2l[L].n_targets, s);
+            continue;
+        }
+        if (fp32 && !converted) {
+            launch_to_float(P.dM, P.dM32, P.n_coef, s);
+            converted = true;
+        }
+        M2LBlockArgs a;
+        a.p = P.p_level[L];
+        a.n_blocks = B.n_blocks;
+        a.blk_targets = B.targets;
+        a.blk_ptr = B.ptr;
+        a.ent_dir = B.dir;
+        a.ent_src = B.src;
+        a.units1 = B.units1;
+        a.n_units1 = B.n_units1;
+        a.units2 = B.units2;
+        a.n_units2 = B.n_units2;
+        a.rot = fp32 ? (const void*)P.d_rot32 : (const void*)P.d_rot;
+        a.rot_stride = rot_size(P.pmax);
+        a.phase = fp32 ? (const void*)P.d_phase32 : (const void*)P.d_phase;
+        a.phase_stride = P.pmax + 1;
+        a.dinfo = P.d_dinfo;
+        a.coax = B.coax;
+        a.coax_stride = B.coax_stride;
+        a.M = fp32 ? (const void*)P.dM32 : (const void*)P.dM;
+        a.L = P.dL;
+        a.exp_off = P.d_expoff;
+        launch_m2l_block(a, B.tb, fp32, s);
+    }
     mark(4);
     for (int L = 2; L < P.depth; ++L)
         if (P.l2l[L].n_targets) launch_xlate(xargs(P, P.l2l[L], P.dL, P.dL, 1), P.l2l[L].n_targets, s);
@@ -529,6 +692,7 @@ void hfmm_opts_default(hfmm_opts* o) {
     o->max_depth = 0;
     o->precision = HFMM_PREC_FP64;
     o->use_graph = 1;
+    o->m2l_kernel = 0;
 }
 
 int hfmm_plan_create(const double* xyz, int64_t n, double k, const hfmm_opts* o, hfmm_plan** out) {

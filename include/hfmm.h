@@ -31,7 +31,8 @@ extern "C" {
 
 /* precision of the evaluate path */
 #define HFMM_PREC_FP64 0       /* FP64 everywhere (the plain FMM) */
-#define HFMM_PREC_MIXED 1      /* FP32 near field on leaf-centred coordinates; far field FP64 */
+#define HFMM_PREC_MIXED 1      /* FP32 near field on leaf-centred coordinates and FP32 M2L on
+                                  matrix cores; P2M/M2M/L2L/L2P and accumulation in FP64 */
 
 /* pointer kinds for q / out */
 #define HFMM_HOST 0
@@ -50,6 +51,7 @@ typedef struct hfmm_opts {
     int max_depth;                  /* <= 0: automatic depth (reference rule) */
     int precision;                  /* HFMM_PREC_* */
     int use_graph;                  /* 1: replay evaluate as a captured hipGraph */
+    int m2l_kernel;                 /* 0: matrix-core block kernel where it fits, 1: generic kernel */
 } hfmm_opts;
 
 typedef struct hfmm_plan hfmm_plan;

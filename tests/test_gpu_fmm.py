@@ -166,3 +166,27 @@ def test_per_level_degrees(hf):
     say(f"  rel err {e:.3e} levels {st.p_level}")
     assert st.p_level == [10, 12, 10]
     assert e < 1e-4
+
+
+@pytest.mark.parametrize("k,p", [(0.0, 8), (10.0, 12), (25.0, 18)])
+def test_matrix_core_m2l_matches_generic_fp64(hf, k, p):
+    say(f"case: mfma m2l fp64 vs generic N=20000 k={k} p={p}")
+    x, q = D.uniform_problem(20000)
+    a = hf.evaluate(hf.setup(x, k, p=p, N_max=32), q)
+    b = hf.evaluate(hf.setup(x, k, p=p, N_max=32, m2l_kernel="generic"), q)
+    d = D.relative_error(a, b)
+    say(f"  rel diff {d:.3e}")
+    assert d < 1e-13
+
+
+def test_matrix_core_m2l_fp32(hf):
+    say("case: mfma m2l fp32 N=30000 k=20 p=14")
+    x, q = D.uniform_problem(30000)
+    idx = D.sample_targets(30000, 500)
+    ref = D.direct_c(x, q, 20.0, idx)
+    a = hf.evaluate(hf.setup(x, 20.0, p=14, N_max=32, precision="mixed"), q)
+    b = hf.evaluate(hf.setup(x, 20.0, p=14, N_max=32, m2l_kernel="generic"), q)
+    ea, eb = D.relative_error(a[idx], ref), D.relative_error(b[idx], ref)
+    say(f"  mixed {ea:.3e} fp64 {eb:.3e} diff {D.relative_error(a, b):.3e}")
+    assert D.relative_error(a, b) < 3e-6
+    assert ea < eb + 3e-6
