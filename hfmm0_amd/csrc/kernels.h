@@ -72,28 +72,31 @@ void launch_direct(const double* X, const double* Y, const double* Z, const doub
 // M2L on matrix cores over target blocks (see m2l_mfma.hip).
 struct M2LBlockArgs {
     int p;                           // degree of this level
-    int n_blocks;
-    const int* blk_targets;          // n_blocks x TB target boxes (-1: empty slot)
-    const int* blk_ptr;              // n_blocks + 1 offsets into the entries
+    int n_items;
+    const int4* items;               // (block, first entry, end entry, atomic add)
+    const int* blk_targets;          // blocks x TB target boxes (-1: empty slot)
     const unsigned short* ent_dir;   // per entry: direction index
-    const int* ent_src;              // per entry: TB source boxes (-1: none)
-    const int* units1;               // rotation work units n | mt << 8 | ct << 16
+    const int* ent_off;              // per entry: TB source expansion offsets (-1: none)
+    const int* units1;               // forward rotation units: n | ct << 16 (all row tiles)
     int n_units1;
-    const int* units2;               // coaxial work units (m + p) | mt << 8 | ct << 16
+    const int* units2;               // coaxial units: (m + p) | ct << 16 (all row tiles)
     int n_units2;
+    const int* units3;               // back rotation units: n | mt << 8 | ct << 16
+    int n_units3;
     const void* rot;                 // rotation tables (T)
+    const void* rot_t;               // the same tables with every degree block transposed
     int rot_stride;
     const void* phase;               // e^{i m phi} per direction (T complex)
     int phase_stride;
     const DispInfo* dinfo;
-    const void* coax;                // coaxial M2L tables of the level (T complex)
+    const void* coax;                // coaxial M2L tables of the level, each m block transposed
     int coax_stride;
     const void* M;                   // multipoles (T complex)
     double2* L;                      // local expansions (FP64), accumulated into
     const int64_t* exp_off;
 };
 size_t m2l_block_lds_bytes(int p, int tb, int precision_bytes);
-void prepare_m2l_block(int p, int n_units1, int tb, int fp32);
+void prepare_m2l_block(int p, int n_units3, int tb, int fp32);
 void launch_m2l_block(const M2LBlockArgs& a, int tb, int fp32, hipStream_t s);
 void launch_to_float(const double2* in, float2* out, int64_t n, hipStream_t s);
 

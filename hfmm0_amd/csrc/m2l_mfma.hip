@@ -6,10 +6,10 @@
 // For each displacement d present in the block it
 //   0. gathers the TB source multipoles into LDS, multiplied by e^{i m phi_d}
 //      (columns = (pair, re/im), rows = coefficient n*n+n+m);
-//   1. rotates each degree block: Y_n = E_n(theta_d)^T X_n        (real x complex)
-//   2. applies the coaxial operator per order m: Z^m = T^{|m|}(|d|) Y^m, as
-//      Re(T) * Y + Im(T) * (Y with re/im columns swapped and signed);
-//   3. rotates back, W_n = E_n Z_n, multiplies by e^{-i m phi_d} (partner lane holds the
+//   1. rotates each degree block in place: X_n <- E_n(theta_d)^T X_n   (real x complex)
+//   2. applies the coaxial operator per order m in place: X^m <- T^{|m|}(|d|) X^m, as
+//      Re(T) * X + Im(T) * (X with re/im columns swapped and signed);
+//   3. rotates back, W_n = E_n X_n, multiplies by e^{-i m phi_d} (partner lane holds the
 //      other half of the complex value) and accumulates in registers.
 // Every product is a 16x16x4 MFMA tile (A from the operator tables, B from LDS; the
 // FP32 and FP64 instructions place output rows differently, see Mma<T>::row);
@@ -24,7 +24,7 @@ namespace hfmm {
 
 size_t m2l_block_lds_bytes(int p, int tb, int precision_bytes) {
     const size_t nc = (size_t)(p + 1) * (p + 1);
-    return 2 * nc * (2 * tb + 1) * precision_bytes;
+    return nc * (2 * tb + 1) * precision_bytes;
 }
 
 namespace {
@@ -59,23 +59,30 @@ __device__ inline void decode_nm(int c, int& n, int& m) {
     m = c - n * n - n;
 }
 
+constexpr int kBlockMaxP = 23;                          // largest degree of the block kernel
+constexpr int kMaxTiles = (2 * kBlockMaxP + 1 + 15) / 16; // row tiles of a degree block
+constexpr int kMaxTiles2 = (kBlockMaxP + 1 + 15) / 16;    // row tiles of an order block
+
 // sum_{i=1}^{x} i^2
 __device__ inline int sq_sum(int x) { return x * (x + 1) * (2 * x + 1) / 6; }
 
+// Work item of the kernel: entries [e0, e1) of one target block.  When a block is split
+// over several items (coarse levels, to fill the GPU) the results are added atomically.
 template <typename T, int TB, int MAXU>
 __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     using V4 = typename Mma<T>::v4;
     using C2 = typename Mma<T>::c2;
     constexpr int NCOL = 2 * TB;            // (pair, re/im) columns
     constexpr int LDX = NCOL + 1;           // odd row stride in LDS
-    constexpr int NCT = (NCOL + 15) / 16;   // 16-wide column tiles
+    constexpr int KC = 8;                   // k-steps whose operands are loaded together
+    constexpr int NG = 16;                  // gathered values per thread and batch
     extern __shared__ char smem_raw[];
     const int p = a.p, P1 = p + 1, nc = P1 * P1;
     T* X = (T*)smem_raw;
-    T* Y = X + (size_t)nc * LDX;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int blk = blockIdx.x;
+    const int4 item = a.items[blockIdx.x];  // (block, e0, e1, atomic)
+    const int blk = item.x;
     const int* tg = a.blk_targets + (size_t)blk * TB;
     const C2* Mexp = (const C2*)a.M;
 
@@ -83,116 +90,184 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
     for (int u = 0; u < MAXU; ++u) acc[u] = V4{0, 0, 0, 0};
 
-    for (int e = a.blk_ptr[blk]; e < a.blk_ptr[blk + 1]; ++e) {
+    for (int e = item.y; e < item.z; ++e) {
         const int d = a.ent_dir[e];
         const DispInfo di = a.dinfo[d];
+        const T* Et = (const T*)a.rot_t + (size_t)di.theta * a.rot_stride;   // transposed blocks
         const T* E = (const T*)a.rot + (size_t)di.theta * a.rot_stride;
         const C2* ph = (const C2*)a.phase + (size_t)d * a.phase_stride;
         const C2* Tz = (const C2*)a.coax + (size_t)di.dist * a.coax_stride;
-        const int* srcs = a.ent_src + (size_t)e * TB;
+        const int* soff = a.ent_off + (size_t)e * TB;
 
-        // stage 0: gather + e^{i m phi}
-        for (int idx = threadIdx.x; idx < TB * nc; idx += 256) {
-            const int j = idx / nc, c = idx - j * nc;
-            const int s = srcs[j];
-            T vr = 0, vi = 0;
-            if (s >= 0) {
-                const C2 mv = Mexp[a.exp_off[s] + c];
-                int n, m;
-                decode_nm(c, n, m);
-                C2 pm = ph[m < 0 ? -m : m];
-                if (m < 0) pm.y = -pm.y;
-                vr = mv.x * pm.x - mv.y * pm.y;
-                vi = mv.x * pm.y + mv.y * pm.x;
-            }
-            X[c * LDX + 2 * j] = vr;
-            X[c * LDX + 2 * j + 1] = vi;
-        }
-        __syncthreads();
-
-        // stage 1: Y_n = E_n^T X_n
-        for (int u = wave; u < a.n_units1; u += 4) {
-            const int code = a.units1[u];
-            const int n = code & 255, mt = (code >> 8) & 255, ct = code >> 16;
-            const int w = 2 * n + 1, base = n * n;
-            const T* En = E + n * (2 * n - 1) * (2 * n + 1) / 3;
-            const int mo = mt * 16 + li;
-            const int col = ct * 16 + li;
-            const bool colok = col < NCOL;
-            V4 c4 = V4{0, 0, 0, 0};
-            for (int k0 = 0; k0 < w; k0 += 4) {
-                const int kin = k0 + lk;
-                const T av = (mo < w && kin < w) ? En[kin * w + mo] : (T)0;
-                const T bv = (kin < w && colok) ? X[(base + kin) * LDX + col] : (T)0;
-                c4 = Mma<T>::mma(av, bv, c4);
-            }
-            if (colok) {
+        // stage 0: gather + e^{i m phi}; loads of a batch are issued before any is used
+        for (int base = 0; base < TB * nc; base += 256 * NG) {
+            C2 raw[NG];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = mt * 16 + Mma<T>::row(lk, r);
-                    if (row < w) Y[(base + row) * LDX + col] = c4[r];
+            for (int g = 0; g < NG; ++g) {
+                const int idx = base + threadIdx.x + 256 * g;
+                raw[g].x = 0;
+                raw[g].y = 0;
+                if (idx < TB * nc) {
+                    const int j = idx / nc, c = idx - j * nc;
+                    const int o = soff[j];
+                    if (o >= 0) raw[g] = Mexp[(int64_t)o + c];
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const int idx = base + threadIdx.x + 256 * g;
+                if (idx < TB * nc) {
+                    const int j = idx / nc, c = idx - j * nc;
+                    int n, m;
+                    decode_nm(c, n, m);
+                    C2 pm = ph[m < 0 ? -m : m];
+                    if (m < 0) pm.y = -pm.y;
+                    X[c * LDX + 2 * j] = raw[g].x * pm.x - raw[g].y * pm.y;
+                    X[c * LDX + 2 * j + 1] = raw[g].x * pm.y + raw[g].y * pm.x;
                 }
             }
         }
         __syncthreads();
 
-        // stage 2: Z^m = T^{|m|} Y^m  (written over X)
-        for (int u = wave; u < a.n_units2; u += 4) {
-            const int code = a.units2[u];
-            const int m = (code & 255) - p, mt = (code >> 8) & 255, ct = code >> 16;
-            const int am = m < 0 ? -m : m;
-            const int len = P1 - am;
-            const C2* Tm = Tz + (sq_sum(P1) - sq_sum(P1 - am));
-            const int nrow = mt * 16 + li;
+        // stage 1: X_n <- E_n^T X_n in place.  A unit is one degree n and one 16-column
+        // group; the wave computes all its row tiles before writing them back, and no other
+        // wave touches those rows and columns.  A[i][k] = E[k][i]: rows of E are contiguous.
+        for (int u = wave; u < a.n_units1; u += 4) {
+            const int code = a.units1[u];
+            const int n = code & 255, ct = code >> 16;
+            const int w = 2 * n + 1, base = n * n;
+            const int ntile = (w + 15) >> 4;
+            const T* En = E + n * (2 * n - 1) * (2 * n + 1) / 3;
             const int col = ct * 16 + li;
             const bool colok = col < NCOL;
-            const T sgn = (col & 1) ? (T)1 : (T)-1;
-            V4 c4 = V4{0, 0, 0, 0};
-            for (int k0 = 0; k0 < len; k0 += 4) {
-                const int kin = k0 + lk;
-                const bool kv = kin < len;
-                C2 t;
-                t.x = 0;
-                t.y = 0;
-                if (nrow < len && kv) t = Tm[nrow * len + kin];
-                const int nn = am + kin;
-                const int cin = nn * nn + nn + m;
-                const T b0 = (kv && colok) ? Y[cin * LDX + col] : (T)0;
-                const T b1 = (kv && colok) ? sgn * Y[cin * LDX + (col ^ 1)] : (T)0;
-                c4 = Mma<T>::mma(t.x, b0, c4);
-                c4 = Mma<T>::mma(t.y, b1, c4);
-            }
-            if (colok) {
+            V4 c4[kMaxTiles];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = mt * 16 + Mma<T>::row(lk, r);
-                    if (row < len) {
-                        const int nn = am + row;
-                        X[(nn * nn + nn + m) * LDX + col] = c4[r];
+            for (int mt = 0; mt < kMaxTiles; ++mt) c4[mt] = V4{0, 0, 0, 0};
+            for (int k0 = 0; k0 < w; k0 += 4 * KC) {
+                T bv[KC];
+#pragma unroll
+                for (int q = 0; q < KC; ++q) {
+                    const int kin = k0 + 4 * q + lk;
+                    bv[q] = (kin < w && colok) ? X[(base + kin) * LDX + col] : (T)0;
+                }
+#pragma unroll
+                for (int mt = 0; mt < kMaxTiles; ++mt) {
+                    if (mt < ntile) {
+                        const int mo = mt * 16 + li;
+                        T av[KC];
+#pragma unroll
+                        for (int q = 0; q < KC; ++q) {
+                            const int kin = k0 + 4 * q + lk;
+                            av[q] = (mo < w && kin < w) ? En[kin * w + mo] : (T)0;
+                        }
+#pragma unroll
+                        for (int q = 0; q < KC; ++q)
+                            if (k0 + 4 * q < w) c4[mt] = Mma<T>::mma(av[q], bv[q], c4[mt]);
                     }
                 }
             }
+            if (colok) {
+#pragma unroll
+                for (int mt = 0; mt < kMaxTiles; ++mt)
+                    if (mt < ntile) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = mt * 16 + Mma<T>::row(lk, r);
+                            if (row < w) X[(base + row) * LDX + col] = c4[mt][r];
+                        }
+                    }
+            }
         }
         __syncthreads();
 
-        // stage 3: W_n = E_n Z_n, times e^{-i m phi}, accumulate
+        // stage 2: X^m <- T^{|m|} X^m in place, one order m and one column group per unit;
+        // A[i][k] = T[n][n'] from the transposed table (16 rows of a k-step contiguous)
+        for (int u = wave; u < a.n_units2; u += 4) {
+            const int code = a.units2[u];
+            const int m = (code & 255) - p, ct = code >> 16;
+            const int am = m < 0 ? -m : m;
+            const int len = P1 - am;
+            const int ntile = (len + 15) >> 4;
+            const C2* Tm = Tz + (sq_sum(P1) - sq_sum(P1 - am));
+            const int col = ct * 16 + li;
+            const bool colok = col < NCOL;
+            const T sgn = (col & 1) ? (T)1 : (T)-1;
+            V4 c4[kMaxTiles2];
+#pragma unroll
+            for (int nt = 0; nt < kMaxTiles2; ++nt) c4[nt] = V4{0, 0, 0, 0};
+            for (int k0 = 0; k0 < len; k0 += 4 * KC) {
+                T b0[KC], b1[KC];
+#pragma unroll
+                for (int q = 0; q < KC; ++q) {
+                    const int kin = k0 + 4 * q + lk;
+                    const bool kv = kin < len;
+                    const int nn = am + kin;
+                    const int cin = nn * nn + nn + m;
+                    b0[q] = (kv && colok) ? X[cin * LDX + col] : (T)0;
+                    b1[q] = (kv && colok) ? sgn * X[cin * LDX + (col ^ 1)] : (T)0;
+                }
+#pragma unroll
+                for (int nt = 0; nt < kMaxTiles2; ++nt) {
+                    if (nt < ntile) {
+                        const int nrow = nt * 16 + li;
+                        C2 tv[KC];
+#pragma unroll
+                        for (int q = 0; q < KC; ++q) {
+                            const int kin = k0 + 4 * q + lk;
+                            tv[q].x = 0;
+                            tv[q].y = 0;
+                            if (nrow < len && kin < len) tv[q] = Tm[kin * len + nrow];
+                        }
+#pragma unroll
+                        for (int q = 0; q < KC; ++q)
+                            if (k0 + 4 * q < len) {
+                                c4[nt] = Mma<T>::mma(tv[q].x, b0[q], c4[nt]);
+                                c4[nt] = Mma<T>::mma(tv[q].y, b1[q], c4[nt]);
+                            }
+                    }
+                }
+            }
+            if (colok) {
+#pragma unroll
+                for (int nt = 0; nt < kMaxTiles2; ++nt)
+                    if (nt < ntile) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = nt * 16 + Mma<T>::row(lk, r);
+                            if (row < len) {
+                                const int nn = am + row;
+                                X[(nn * nn + nn + m) * LDX + col] = c4[nt][r];
+                            }
+                        }
+                    }
+            }
+        }
+        __syncthreads();
+
+        // stage 3: W_n = E_n Z_n (A[i][k] = E[i][k] = Et[k][i]), times e^{-i m phi}, accumulate
 #pragma unroll
         for (int uu = 0; uu < MAXU; ++uu) {
             const int u = wave + 4 * uu;
-            if (u < a.n_units1) {
-                const int code = a.units1[u];
+            if (u < a.n_units3) {
+                const int code = a.units3[u];
                 const int n = code & 255, mt = (code >> 8) & 255, ct = code >> 16;
                 const int w = 2 * n + 1, base = n * n;
-                const T* En = E + n * (2 * n - 1) * (2 * n + 1) / 3;
+                const T* Etn = Et + n * (2 * n - 1) * (2 * n + 1) / 3;
                 const int mo = mt * 16 + li;
                 const int col = ct * 16 + li;
                 const bool colok = col < NCOL;
                 V4 c4 = V4{0, 0, 0, 0};
-                for (int k0 = 0; k0 < w; k0 += 4) {
-                    const int kin = k0 + lk;
-                    const T av = (mo < w && kin < w) ? En[mo * w + kin] : (T)0;
-                    const T bv = (kin < w && colok) ? X[(base + kin) * LDX + col] : (T)0;
-                    c4 = Mma<T>::mma(av, bv, c4);
+                for (int k0 = 0; k0 < w; k0 += 4 * KC) {
+                    T av[KC], bv[KC];
+#pragma unroll
+                    for (int q = 0; q < KC; ++q) {
+                        const int kin = k0 + 4 * q + lk;
+                        av[q] = (mo < w && kin < w) ? Etn[kin * w + mo] : (T)0;
+                        bv[q] = (kin < w && colok) ? X[(base + kin) * LDX + col] : (T)0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < KC; ++q)
+                        if (k0 + 4 * q < w) c4 = Mma<T>::mma(av[q], bv[q], c4);
                 }
                 const bool im_part = (col & 1) != 0;
 #pragma unroll
@@ -218,8 +293,8 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
     for (int uu = 0; uu < MAXU; ++uu) {
         const int u = wave + 4 * uu;
-        if (u < a.n_units1) {
-            const int code = a.units1[u];
+        if (u < a.n_units3) {
+            const int code = a.units3[u];
             const int n = code & 255, mt = (code >> 8) & 255, ct = code >> 16;
             const int col = ct * 16 + li;
             if (col < NCOL) {
@@ -230,7 +305,13 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int row = mt * 16 + Mma<T>::row(lk, r);
-                        if (row < 2 * n + 1) dst[2 * (n * n + row) + ri] += (double)acc[uu][r];
+                        if (row < 2 * n + 1) {
+                            double* x = dst + 2 * (n * n + row) + ri;
+                            if (item.w)
+                                atomicAdd(x, (double)acc[uu][r]);
+                            else
+                                *x += (double)acc[uu][r];
+                        }
                     }
                 }
             }
@@ -255,7 +336,7 @@ void launch_one(const M2LBlockArgs* a, size_t shm, hipStream_t s) {
         if (e != hipSuccess) fail(-3, std::string("hipFuncSetAttribute(m2l) failed: ") + hipGetErrorString(e));
         return;
     }
-    hipLaunchKernelGGL((m2l_block_kernel<T, TB, MAXU>), dim3(a->n_blocks), dim3(256), shm, s, *a);
+    hipLaunchKernelGGL((m2l_block_kernel<T, TB, MAXU>), dim3(a->n_items), dim3(256), shm, s, *a);
 }
 
 template <typename T, int TB>
@@ -270,14 +351,12 @@ void launch_tb(const M2LBlockArgs* a, size_t shm, int maxu, hipStream_t s) {
         launch_one<T, TB, 24>(a, shm, s);
     else if (maxu <= 32)
         launch_one<T, TB, 32>(a, shm, s);
-    else if (maxu <= 48)
-        launch_one<T, TB, 48>(a, shm, s);
     else
         fail(-9, "internal: too many M2L units per wave");
 }
 
-void dispatch(const M2LBlockArgs* a, int p, int n_units1, int tb, int fp32, hipStream_t s) {
-    const int maxu = (n_units1 + 3) / 4;
+void dispatch(const M2LBlockArgs* a, int p, int n_units3, int tb, int fp32, hipStream_t s) {
+    const int maxu = (n_units3 + 3) / 4;
     const size_t shm = m2l_block_lds_bytes(p, tb, fp32 ? 4 : 8);
     if (fp32) {
         if (tb == 16)
@@ -298,11 +377,11 @@ void dispatch(const M2LBlockArgs* a, int p, int n_units1, int tb, int fp32, hipS
 
 }  // namespace
 
-void prepare_m2l_block(int p, int n_units1, int tb, int fp32) { dispatch(nullptr, p, n_units1, tb, fp32, nullptr); }
+void prepare_m2l_block(int p, int n_units3, int tb, int fp32) { dispatch(nullptr, p, n_units3, tb, fp32, nullptr); }
 
 void launch_m2l_block(const M2LBlockArgs& a, int tb, int fp32, hipStream_t s) {
-    if (a.n_blocks <= 0) return;
-    dispatch(&a, a.p, a.n_units1, tb, fp32, s);
+    if (a.n_items <= 0) return;
+    dispatch(&a, a.p, a.n_units3, tb, fp32, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) fail(-3, std::string("kernel launch failed (m2l_block): ") + hipGetErrorString(e));
 }

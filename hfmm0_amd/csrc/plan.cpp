@@ -36,15 +36,17 @@ constexpr int kNumDirs = 359;
 struct M2LBlockLevel {
     bool use = false;
     int tb = 0;
-    int n_blocks = 0;
+    int n_items = 0;
+    int4* items = nullptr;
     int* targets = nullptr;
-    int* ptr = nullptr;
     unsigned short* dir = nullptr;
-    int* src = nullptr;
+    int* off = nullptr;
     int* units1 = nullptr;
     int n_units1 = 0;
     int* units2 = nullptr;
     int n_units2 = 0;
+    int* units3 = nullptr;
+    int n_units3 = 0;
     void* coax = nullptr;          // table in the kernel's precision
     int coax_stride = 0;
 };
@@ -100,6 +102,8 @@ struct hfmm_plan {
     int m2l_kernel = 0;
     size_t max_lds = 65536;
     float* d_rot32 = nullptr;
+    float* d_rot32_t = nullptr;
+    double* d_rot_t = nullptr;
     float2* d_phase32 = nullptr;
     float2* dM32 = nullptr;
     double2 *dM = nullptr, *dL = nullptr, *d_q = nullptr, *d_far = nullptr;
@@ -161,36 +165,41 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
                       const std::vector<double>& coax_d, int n_dist, const std::vector<DispInfo>& dinfo) {
     if (P.m2l_kernel != 0) return;
     const int p = P.p_level[L];
+    if (p > 23) return;   // degrees above 23 use the generic kernel (coarse levels only)
     const int fp32 = P.precision == HFMM_PREC_MIXED;
     const int eb = fp32 ? 4 : 8;
-    int tb = 0;
+    int tb = 0;   // largest block whose LDS buffer fits two workgroups per CU, else one
     for (int c : {16, 8, 4})
-        if (!tb && m2l_block_lds_bytes(p, c, eb) <= 80 * 1024) tb = c;
+        if (!tb && 2 * m2l_block_lds_bytes(p, c, eb) <= P.max_lds) tb = c;
     for (int c : {16, 8, 4})
         if (!tb && m2l_block_lds_bytes(p, c, eb) <= P.max_lds) tb = c;
     if (!tb) return;
     const int ncol = 2 * tb, nct = (ncol + 15) / 16;
-    std::vector<std::pair<int, int>> u1, u2;   // (cost, code)
+    std::vector<std::pair<int, int>> u1, u2, u3;   // (cost, code)
     for (int n = 0; n <= p; ++n)
-        for (int mt = 0; mt < (2 * n + 1 + 15) / 16; ++mt)
-            for (int ct = 0; ct < nct; ++ct) u1.push_back({(2 * n + 1 + 3) / 4, n | (mt << 8) | (ct << 16)});
+        for (int ct = 0; ct < nct; ++ct) {
+            const int tiles = (2 * n + 1 + 15) / 16, ks = (2 * n + 1 + 3) / 4;
+            u1.push_back({tiles * ks, n | (ct << 16)});
+            for (int mt = 0; mt < tiles; ++mt) u3.push_back({ks, n | (mt << 8) | (ct << 16)});
+        }
     for (int m = -p; m <= p; ++m) {
         const int len = p + 1 - std::abs(m);
-        for (int mt = 0; mt < (len + 15) / 16; ++mt)
-            for (int ct = 0; ct < nct; ++ct) u2.push_back({2 * ((len + 3) / 4), (m + p) | (mt << 8) | (ct << 16)});
+        for (int ct = 0; ct < nct; ++ct) u2.push_back({2 * ((len + 15) / 16) * ((len + 3) / 4), (m + p) | (ct << 16)});
     }
     auto by_cost = [](const std::pair<int, int>& a, const std::pair<int, int>& b) { return a.first > b.first; };
     std::stable_sort(u1.begin(), u1.end(), by_cost);
     std::stable_sort(u2.begin(), u2.end(), by_cost);
-    if ((int)(u1.size() + 3) / 4 > 48) return;
-    std::vector<int> units1, units2;
+    std::stable_sort(u3.begin(), u3.end(), by_cost);
+    if ((int)(u3.size() + 3) / 4 > 32) return;
+    std::vector<int> units1, units2, units3;
     for (auto& x : u1) units1.push_back(x.second);
     for (auto& x : u2) units2.push_back(x.second);
+    for (auto& x : u3) units3.push_back(x.second);
 
     const Tree& t = P.tree;
     std::vector<int> tpos(t.n_boxes(), -1);     // target ordinal in `targets`
     for (size_t i = 0; i < targets.size(); ++i) tpos[targets[i]] = (int)i;
-    std::vector<int> blk_targets, blk_ptr{0}, ent_src;
+    std::vector<int> blk_targets, blk_ptr{0}, ent_off;
     std::vector<unsigned short> ent_dir;
     for (int o = 0; o < 8; ++o) {
         std::vector<int> boxes;
@@ -208,37 +217,65 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
                     auto key = std::make_tuple(dinfo[d].theta, dinfo[d].dist, d);
                     auto it = ents.find(key);
                     if (it == ents.end()) it = ents.emplace(key, std::vector<int>(tb, -1)).first;
-                    it->second[j] = src[e];
+                    it->second[j] = (int)P.exp_off[src[e]];
                 }
             }
             for (auto& kv : ents) {
                 ent_dir.push_back((unsigned short)std::get<2>(kv.first));
-                for (int j = 0; j < tb; ++j) ent_src.push_back(kv.second[j]);
+                for (int j = 0; j < tb; ++j) ent_off.push_back(kv.second[j]);
             }
             blk_ptr.push_back((int)ent_dir.size());
         }
     }
+    // work items: split the entry lists of the blocks when there are few blocks
+    const int nblk = (int)blk_ptr.size() - 1;
+    const int nsplit = std::max(1, std::min(16, (2048 + nblk - 1) / std::max(nblk, 1)));
+    std::vector<int4> items;
+    for (int b = 0; b < nblk; ++b) {
+        const int e0 = blk_ptr[b], e1 = blk_ptr[b + 1];
+        const int chunk = (e1 - e0 + nsplit - 1) / nsplit;
+        for (int s = e0; s < e1; s += std::max(chunk, 1))
+            items.push_back(make_int4(b, s, std::min(e1, s + std::max(chunk, 1)), nsplit > 1 ? 1 : 0));
+    }
     M2LBlockLevel& B = P.m2lb[L];
     B.use = true;
     B.tb = tb;
-    B.n_blocks = (int)blk_ptr.size() - 1;
+    B.n_items = (int)items.size();
+    B.items = P.upload(items);
     B.targets = P.upload(blk_targets);
-    B.ptr = P.upload(blk_ptr);
     B.dir = P.upload(ent_dir);
-    B.src = P.upload(ent_src);
+    B.off = P.upload(ent_off);
     B.units1 = P.upload(units1);
     B.n_units1 = (int)units1.size();
     B.units2 = P.upload(units2);
     B.n_units2 = (int)units2.size();
+    B.units3 = P.upload(units3);
+    B.n_units3 = (int)units3.size();
     B.coax_stride = coax_size(p, p);
+    // coaxial tables with every m block transposed ([n'][n])
+    std::vector<double> ct(coax_d.size());
+    for (int di = 0; di < n_dist; ++di) {
+        const double* src_t = &coax_d[(size_t)di * B.coax_stride * 2];
+        double* dst_t = &ct[(size_t)di * B.coax_stride * 2];
+        int off = 0;
+        for (int m = 0; m <= p; ++m) {
+            const int len = p + 1 - m;
+            for (int r = 0; r < len; ++r)
+                for (int c = 0; c < len; ++c) {
+                    dst_t[2 * (off + c * len + r)] = src_t[2 * (off + r * len + c)];
+                    dst_t[2 * (off + c * len + r) + 1] = src_t[2 * (off + r * len + c) + 1];
+                }
+            off += len * len;
+        }
+    }
     if (fp32) {
         std::vector<float2> c32((size_t)n_dist * B.coax_stride);
-        for (size_t i = 0; i < c32.size(); ++i) c32[i] = make_float2((float)coax_d[2 * i], (float)coax_d[2 * i + 1]);
+        for (size_t i = 0; i < c32.size(); ++i) c32[i] = make_float2((float)ct[2 * i], (float)ct[2 * i + 1]);
         B.coax = P.upload(c32);
     } else {
-        B.coax = P.upload(coax_d);
+        B.coax = P.upload(ct);
     }
-    prepare_m2l_block(p, B.n_units1, tb, fp32);
+    prepare_m2l_block(p, B.n_units3, tb, fp32);
 }
 
 void build_tables(hfmm_plan& P) {
@@ -284,9 +321,20 @@ void build_tables(hfmm_plan& P) {
     std::vector<double> rot((size_t)thetas.size() * rsz);
     for (size_t i = 0; i < thetas.size(); ++i) rotation_table(thetas[i], pmax, &rot[i * rsz]);
     P.d_rot = P.upload(rot);
+    std::vector<double> rot_t(rot.size());
+    for (size_t i = 0; i < thetas.size(); ++i)
+        for (int n = 0; n <= pmax; ++n) {
+            const int w = 2 * n + 1;
+            const double* a = &rot[i * rsz + rot_offset(n)];
+            double* b = &rot_t[i * rsz + rot_offset(n)];
+            for (int r = 0; r < w; ++r)
+                for (int c = 0; c < w; ++c) b[c * w + r] = a[r * w + c];
+        }
+    P.d_rot_t = P.upload(rot_t);
     if (P.precision == HFMM_PREC_MIXED) {
-        std::vector<float> r32(rot.begin(), rot.end());
+        std::vector<float> r32(rot.begin(), rot.end()), rt32(rot_t.begin(), rot_t.end());
         P.d_rot32 = P.upload(r32);
+        P.d_rot32_t = P.upload(rt32);
     }
     std::vector<double2> phase((size_t)kNumDirs * (pmax + 1));
     for (int d = 0; d < kNumDirs; ++d)
@@ -587,16 +635,19 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
         }
         M2LBlockArgs a;
         a.p = P.p_level[L];
-        a.n_blocks = B.n_blocks;
+        a.n_items = B.n_items;
+        a.items = B.items;
         a.blk_targets = B.targets;
-        a.blk_ptr = B.ptr;
         a.ent_dir = B.dir;
-        a.ent_src = B.src;
+        a.ent_off = B.off;
         a.units1 = B.units1;
         a.n_units1 = B.n_units1;
         a.units2 = B.units2;
         a.n_units2 = B.n_units2;
+        a.units3 = B.units3;
+        a.n_units3 = B.n_units3;
         a.rot = fp32 ? (const void*)P.d_rot32 : (const void*)P.d_rot;
+        a.rot_t = fp32 ? (const void*)P.d_rot32_t : (const void*)P.d_rot_t;
         a.rot_stride = rot_size(P.pmax);
         a.phase = fp32 ? (const void*)P.d_phase32 : (const void*)P.d_phase;
         a.phase_stride = P.pmax + 1;

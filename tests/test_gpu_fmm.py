@@ -142,8 +142,10 @@ def test_graph_replay_matches_stream_run(hf):
     b = hf.setup(x, 6.0, p=9, N_max=32, use_graph=False)
     ra = [hf.evaluate(a, q) for _ in range(3)]
     rb = hf.evaluate(b, q)
+    # coarse-level M2L work items add into the local expansions atomically, so the
+    # summation order (and the last bits) may differ between runs
     for r in ra:
-        np.testing.assert_array_equal(r, rb)
+        np.testing.assert_allclose(r, rb, rtol=0, atol=1e-13 * np.abs(rb).max())
 
 
 def test_device_pointer_path(hf):
@@ -156,7 +158,7 @@ def test_device_pointer_path(hf):
     od = torch.empty(5000, dtype=torch.complex128, device="cuda")
     hf.evaluate_device(st, qd, od)
     torch.cuda.synchronize()
-    np.testing.assert_array_equal(od.cpu().numpy(), host)
+    np.testing.assert_allclose(od.cpu().numpy(), host, rtol=0, atol=1e-13 * np.abs(host).max())
 
 
 def test_per_level_degrees(hf):
