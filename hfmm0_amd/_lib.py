@@ -35,7 +35,8 @@ class PlanInfo(ctypes.Structure):
                 ("n_leaves", ctypes.c_int), ("maxleaf", ctypes.c_int), ("n_v", ctypes.c_int64),
                 ("n_u", ctypes.c_int64), ("n_w", ctypes.c_int64), ("n_x", ctypes.c_int64),
                 ("n_p2p_pairs", ctypes.c_int64), ("p_level", ctypes.c_int * MAX_LEVELS),
-                ("extent", ctypes.c_double), ("pmin", ctypes.c_double * 3)]
+                ("extent", ctypes.c_double), ("pmin", ctypes.c_double * 3),
+                ("p_table", ctypes.c_int * MAX_LEVELS)]
 
 
 class HFMMError(RuntimeError):
@@ -55,6 +56,7 @@ def _declare(lib):
         "hfmm_plan_export_tree": (I, [vp, vp, vp, vp, vp, vp, vp, vp]),
         "hfmm_plan_export_list": (I, [vp, ctypes.c_char, vp]),
         "hfmm_evaluate": (I, [vp, vp, I, vp, I, vp]),
+        "hfmm_plan_set_degrees": (I, [vp, vp]),
         "hfmm_benchmark": (I, [vp, vp, I, vp, I, I, ctypes.POINTER(D), vp]),
         "hfmm_plan_destroy": (I, [vp]),
         "hfmm_direct": (I, [vp, L, vp, D, vp, L, vp]),
@@ -77,7 +79,7 @@ def _declare(lib):
 
 
 EXPORTED = [
-    "hfmm_opts_default", "hfmm_plan_create", "hfmm_plan_info_get", "hfmm_plan_export_tree",
+    "hfmm_opts_default", "hfmm_plan_create", "hfmm_plan_info_get", "hfmm_plan_export_tree", "hfmm_plan_set_degrees",
     "hfmm_plan_export_list", "hfmm_evaluate", "hfmm_benchmark", "hfmm_plan_destroy", "hfmm_direct",
     "hfmm_tree_create", "hfmm_tree_info_get", "hfmm_tree_export", "hfmm_tree_export_list",
     "hfmm_tree_destroy", "hfmm_host_jhat", "hfmm_host_hhat", "hfmm_host_rotation", "hfmm_host_coax",

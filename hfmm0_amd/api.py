@@ -44,7 +44,26 @@ class HFMMState:
 
     @property
     def p_level(self):
+        """Degree evaluated at levels 1..depth."""
         return [self.info.p_level[i] for i in range(1, self.info.depth + 1)]
+
+    @property
+    def p_table(self):
+        """Degree the tables of levels 1..depth were built for."""
+        return [self.info.p_table[i] for i in range(1, self.info.depth + 1)]
+
+    def set_degrees(self, p_level):
+        """Evaluate levels 1..depth at the given degrees (each <= its table degree; 0 or
+        None keeps the table degree) without rebuilding the plan."""
+        import numpy as np
+        arr = np.zeros(L.MAX_LEVELS, dtype=np.int32)
+        for i, v in enumerate(p_level):
+            if i + 1 < L.MAX_LEVELS and v:
+                arr[i + 1] = int(v)
+        L.check(L.lib().hfmm_plan_set_degrees(self.handle, L.ptr(arr)))
+        info = L.PlanInfo()
+        L.check(L.lib().hfmm_plan_info_get(self.handle, ctypes.byref(info)))
+        self.info = info
 
     def close(self):
         if self._h is not None:

@@ -53,7 +53,8 @@ struct XlateArgs {
     const double* rot;         // rotation tables, rot_size(pmax) doubles per theta index
     const double2* coax;       // coaxial tables of this launch, indexed by dinfo.dist
     int64_t coax_stride;       // complex entries per coaxial table
-    int ps, pt, pmax;
+    int ps, pt, pmax;          // effective source / target degrees (<= table degrees)
+    int tps, tpt;              // degrees the coaxial tables were built for
     int accumulate;            // 1: dst += result, 0: dst = result
     const double2* srcbuf;
     double2* dstbuf;
@@ -71,7 +72,8 @@ void launch_direct(const double* X, const double* Y, const double* Z, const doub
 
 // M2L on matrix cores over target blocks (see m2l_mfma.hip).
 struct M2LBlockArgs {
-    int p;                           // degree of this level
+    int p;                           // effective degree of this level (<= p_tab)
+    int p_tab;                       // degree the tables, units and LDS were sized for
     int n_items;
     const int4* items;               // (block, first entry, end entry, atomic add)
     const int* blk_targets;          // blocks x TB target boxes (-1: empty slot)

@@ -296,12 +296,13 @@ __global__ __launch_bounds__(kXlateThreads) void xlate_kernel(XlateArgs a) {
     double2* sA = sIn + nmx;
     double2* sB = sIn;
     int* s_off = (int*)(sA + nmx);
-    const int mmax = min(ps, pt);
+    const int tps = a.tps, tpt = a.tpt;   // degrees the coaxial table was packed for
+    const int mmax = min(tps, tpt);
     if (threadIdx.x == 0) {
         int o = 0;
         for (int m = 0; m <= mmax; ++m) {
             s_off[m] = o;
-            o += (pt + 1 - m) * (ps + 1 - m);
+            o += (tpt + 1 - m) * (tps + 1 - m);
         }
     }
     const int t_ord = blockIdx.x;
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(kXlateThreads) void xlate_kernel(XlateArgs a) {
             const int am = m < 0 ? -m : m;
             double2 v = make_double2(0.0, 0.0);
             if (am <= ps) {
-                const double2* T = cx + s_off[am] + (n - am) * (ps + 1 - am);
+                const double2* T = cx + s_off[am] + (n - am) * (tps + 1 - am);
                 for (int n2 = am; n2 <= ps; ++n2) v = cmadd(v, T[n2 - am], sA[n2 * n2 + n2 + m]);
             }
             sB[c] = v;

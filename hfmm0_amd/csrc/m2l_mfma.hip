@@ -78,6 +78,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     constexpr int NG = 16;                  // gathered values per thread and batch
     extern __shared__ char smem_raw[];
     const int p = a.p, P1 = p + 1, nc = P1 * P1;
+    const int PT1 = a.p_tab + 1;            // coaxial tables are packed for degree p_tab
     T* X = (T*)smem_raw;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
@@ -135,6 +136,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
         for (int u = wave; u < a.n_units1; u += 4) {
             const int code = a.units1[u];
             const int n = code & 255, ct = code >> 16;
+            if (n > p) continue;
             const int w = 2 * n + 1, base = n * n;
             const int ntile = (w + 15) >> 4;
             const T* En = E + n * (2 * n - 1) * (2 * n + 1) / 3;
@@ -184,11 +186,13 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
         // A[i][k] = T[n][n'] from the transposed table (16 rows of a k-step contiguous)
         for (int u = wave; u < a.n_units2; u += 4) {
             const int code = a.units2[u];
-            const int m = (code & 255) - p, ct = code >> 16;
+            const int m = (code & 255) - a.p_tab, ct = code >> 16;
             const int am = m < 0 ? -m : m;
+            if (am > p) continue;
             const int len = P1 - am;
+            const int lent = PT1 - am;          // row length of the packed (transposed) block
             const int ntile = (len + 15) >> 4;
-            const C2* Tm = Tz + (sq_sum(P1) - sq_sum(P1 - am));
+            const C2* Tm = Tz + (sq_sum(PT1) - sq_sum(PT1 - am));
             const int col = ct * 16 + li;
             const bool colok = col < NCOL;
             const T sgn = (col & 1) ? (T)1 : (T)-1;
@@ -216,7 +220,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                             const int kin = k0 + 4 * q + lk;
                             tv[q].x = 0;
                             tv[q].y = 0;
-                            if (nrow < len && kin < len) tv[q] = Tm[kin * len + nrow];
+                            if (nrow < len && kin < len) tv[q] = Tm[kin * lent + nrow];
                         }
 #pragma unroll
                         for (int q = 0; q < KC; ++q)
@@ -248,7 +252,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
         for (int uu = 0; uu < MAXU; ++uu) {
             const int u = wave + 4 * uu;
-            if (u < a.n_units3) {
+            if (u < a.n_units3 && (a.units3[u] & 255) <= p) {
                 const int code = a.units3[u];
                 const int n = code & 255, mt = (code >> 8) & 255, ct = code >> 16;
                 const int w = 2 * n + 1, base = n * n;
@@ -293,7 +297,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
     for (int uu = 0; uu < MAXU; ++uu) {
         const int u = wave + 4 * uu;
-        if (u < a.n_units3) {
+        if (u < a.n_units3 && (a.units3[u] & 255) <= p) {
             const int code = a.units3[u];
             const int n = code & 255, mt = (code >> 8) & 255, ct = code >> 16;
             const int col = ct * 16 + li;
@@ -381,7 +385,7 @@ void prepare_m2l_block(int p, int n_units3, int tb, int fp32) { dispatch(nullptr
 
 void launch_m2l_block(const M2LBlockArgs& a, int tb, int fp32, hipStream_t s) {
     if (a.n_items <= 0) return;
-    dispatch(&a, a.p, a.n_units3, tb, fp32, s);
+    dispatch(&a, a.p_tab, a.n_units3, tb, fp32, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) fail(-3, std::string("kernel launch failed (m2l_block): ") + hipGetErrorString(e));
 }

@@ -59,7 +59,8 @@ struct XlateLaunch {
     unsigned short* dir = nullptr;
     double2* coax = nullptr;
     int64_t coax_stride = 0;
-    int ps = 0, pt = 0;
+    int ps = 0, pt = 0;            // table degrees of the source / target level
+    int src_level = 0, tgt_level = 0;
 };
 
 }  // namespace hfmm
@@ -76,7 +77,8 @@ struct hfmm_plan {
     int use_graph = 1;
     int depth = 0;
     int pmax = 0;
-    int p_level[kMaxLevels] = {0};
+    int p_level[kMaxLevels] = {0};  // table degree per level (tables, storage, LDS)
+    int p_eff[kMaxLevels] = {0};    // degree evaluated per level (<= p_level)
     double h[kMaxLevels] = {0}, R[kMaxLevels] = {0};
     std::vector<int64_t> exp_off;
     int64_t n_coef = 0;
@@ -381,6 +383,7 @@ void build_tables(hfmm_plan& P) {
             X.src = P.upload(src);
             X.dir = P.upload(dir);
             X.ps = X.pt = P.p_level[L];
+            X.src_level = X.tgt_level = L;
             X.coax_stride = coax_size(X.pt, X.ps);
             std::vector<double> cx((size_t)dist_d2.size() * X.coax_stride * 2);
             for (size_t i = 0; i < dist_d2.size(); ++i)
@@ -419,6 +422,8 @@ void build_tables(hfmm_plan& P) {
         A.dir = P.upload(dir);
         A.ps = P.p_level[L + 1];
         A.pt = P.p_level[L];
+        A.src_level = L + 1;
+        A.tgt_level = L;
         A.coax_stride = coax_size(A.pt, A.ps);
         std::vector<double> cx((size_t)A.coax_stride * 2);
         coax_table(1, G, P.k, P.R[L + 1], P.R[L + 1], P.R[L], A.ps, A.pt, cx.data());
@@ -431,6 +436,8 @@ void build_tables(hfmm_plan& P) {
         B.dir = P.upload(cdir);
         B.ps = P.p_level[L];
         B.pt = P.p_level[L + 1];
+        B.src_level = L;
+        B.tgt_level = L + 1;
         B.coax_stride = coax_size(B.pt, B.ps);
         std::vector<double> cy((size_t)B.coax_stride * 2);
         coax_table(2, G, P.k, P.R[L + 1], P.R[L], P.R[L + 1], B.ps, B.pt, cy.data());
@@ -522,6 +529,7 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
         P.h[L] = t.extent * std::ldexp(1.0, -(L + 1));
         P.R[L] = std::sqrt(3.0) * P.h[L];
         if (L >= 1 && L <= t.depth) P.pmax = std::max(P.pmax, P.p_level[L]);
+        P.p_eff[L] = P.p_level[L];
     }
     HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
     // sorted positions (SoA, FP64) and permutation
@@ -593,8 +601,10 @@ XlateArgs xargs(const hfmm_plan& P, const XlateLaunch& X, const double2* srcbuf,
     a.rot = P.d_rot;
     a.coax = X.coax;
     a.coax_stride = X.coax_stride;
-    a.ps = X.ps;
-    a.pt = X.pt;
+    a.ps = P.p_eff[X.src_level];
+    a.pt = P.p_eff[X.tgt_level];
+    a.tps = X.ps;
+    a.tpt = X.pt;
     a.pmax = P.pmax;
     a.accumulate = accumulate;
     a.srcbuf = srcbuf;
@@ -634,7 +644,8 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
             converted = true;
         }
         M2LBlockArgs a;
-        a.p = P.p_level[L];
+        a.p = P.p_eff[L];
+        a.p_tab = P.p_level[L];
         a.n_items = B.n_items;
         a.items = B.items;
         a.blk_targets = B.targets;
@@ -775,7 +786,10 @@ int hfmm_plan_info_get(const hfmm_plan* P, hfmm_plan_info* info) {
         info->n_w = (int64_t)P->lists.w_tgt.size();
         info->n_x = (int64_t)P->lists.x_tgt.size();
         info->n_p2p_pairs = P->n_p2p_pairs;
-        for (int L = 0; L < HFMM_MAX_LEVELS; ++L) info->p_level[L] = (L >= 1 && L <= P->depth) ? P->p_level[L] : 0;
+        for (int L = 0; L < HFMM_MAX_LEVELS; ++L) {
+            info->p_level[L] = (L >= 1 && L <= P->depth) ? P->p_eff[L] : 0;
+            info->p_table[L] = (L >= 1 && L <= P->depth) ? P->p_level[L] : 0;
+        }
         info->extent = P->tree.extent;
         for (int a = 0; a < 3; ++a) info->pmin[a] = P->tree.pmin[a];
     });
@@ -793,6 +807,32 @@ int hfmm_plan_export_list(const hfmm_plan* P, char which, int* pairs) {
     return guarded([&] {
         if (!P || !pairs) fail(HFMM_ERR_ARG, "null argument");
         export_list(P->lists, which, pairs);
+    });
+}
+
+int hfmm_plan_set_degrees(hfmm_plan* P, const int* p_level) {
+    return guarded([&] {
+        if (!P || !p_level) fail(HFMM_ERR_ARG, "null argument");
+        int eff[kMaxLevels];
+        for (int L = 0; L < kMaxLevels; ++L) {
+            eff[L] = P->p_level[L];
+            if (L >= HFMM_MAX_LEVELS || L < 1 || L > P->depth || p_level[L] == 0) continue;
+            if (p_level[L] < 2 || p_level[L] > P->p_level[L])
+                fail(HFMM_ERR_ARG, "p_level entries must be 0 or in [2, table degree of the level]");
+            eff[L] = p_level[L];
+        }
+        HIPCHK(hipStreamSynchronize(P->stream));
+        std::copy(eff, eff + kMaxLevels, P->p_eff);
+        HIPCHK(hipMemcpy(P->d_plevel, P->p_eff, sizeof(int) * kMaxLevels, hipMemcpyHostToDevice));
+        if (P->gexec) {
+            HIPCHK(hipGraphExecDestroy(P->gexec));
+            P->gexec = nullptr;
+        }
+        if (P->graph) {
+            HIPCHK(hipGraphDestroy(P->graph));
+            P->graph = nullptr;
+        }
+        P->g_q = nullptr;
     });
 }
 

@@ -64,9 +64,10 @@ typedef struct hfmm_plan_info {
     int maxleaf;
     int64_t n_v, n_u, n_w, n_x;     /* list sizes (U without self pairs) */
     int64_t n_p2p_pairs;            /* point pairs evaluated directly (incl. r = 0 pairs) */
-    int p_level[HFMM_MAX_LEVELS];   /* degree used at each level */
+    int p_level[HFMM_MAX_LEVELS];   /* degree evaluated at each level */
     double extent;
     double pmin[3];
+    int p_table[HFMM_MAX_LEVELS];   /* degree the level's tables were built for */
 } hfmm_plan_info;
 
 void hfmm_opts_default(hfmm_opts* o);
@@ -81,6 +82,13 @@ int hfmm_plan_export_tree(const hfmm_plan* plan, int* level, uint64_t* code, int
                           int* parent, uint8_t* is_leaf, int64_t* perm);
 /* List export: which = 'V', 'U', 'W', 'X'; pairs (target, source) as int32 [n][2]. */
 int hfmm_plan_export_list(const hfmm_plan* plan, char which, int* pairs);
+
+/* Evaluate at lower degrees than the tables were built for, without a new setup:
+ * p_level[l] in [2, table degree of level l] sets level l; 0 keeps the table degree.
+ * Coefficients of degree <= p do not depend on the truncation, so the result equals a
+ * plan built at those degrees.  No counterpart in the reference (its tables are built
+ * per p in setup, jaxfmm/kifmm/operators.py:572-641). */
+int hfmm_plan_set_degrees(hfmm_plan* plan, const int* p_level /* [HFMM_MAX_LEVELS] */);
 
 /* phi for all N points in input order.  q: N charges of q_type (HFMM_F64 or HFMM_C128);
  * out: N complex128.  q and out are both host or both device pointers (ptr_kind).

@@ -192,3 +192,32 @@ def test_matrix_core_m2l_fp32(hf):
     say(f"  mixed {ea:.3e} fp64 {eb:.3e} diff {D.relative_error(a, b):.3e}")
     assert D.relative_error(a, b) < 3e-6
     assert ea < eb + 3e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_set_degrees_equals_dedicated_plan(hf, precision):
+    say(f"case: set_degrees {precision} N=20000 k=12 tables p=14 -> [12, 11, 9]")
+    x, q = D.uniform_problem(20000)
+    st = hf.setup(x, 12.0, p=14, N_max=64, precision=precision)
+    full = hf.evaluate(st, q)
+    st.set_degrees([12, 12, 11, 9])
+    assert st.p_level == [12, 12, 11] and st.p_table == [14, 14, 14]
+    a = hf.evaluate(st, q)
+    b = hf.evaluate(hf.setup(x, 12.0, p=12, N_max=64, p_level=[12, 12, 11, 9], precision=precision), q)
+    d = D.relative_error(a, b)
+    say(f"  truncated vs dedicated rel diff {d:.3e} (vs full {D.relative_error(a, full):.3e})")
+    assert d < (1e-13 if precision == "fp64" else 2e-6)
+    st.set_degrees([0, 0, 0])
+    np.testing.assert_allclose(hf.evaluate(st, q), full, rtol=0, atol=1e-12 * np.abs(full).max())
+
+
+def test_set_degrees_generic_kernel_levels(hf):
+    say("case: set_degrees above the block-kernel degree N=8000 k=30 tables p=26 -> 20")
+    x, q = D.uniform_problem(8000)
+    st = hf.setup(x, 30.0, p=26, N_max=32)
+    st.set_degrees([20] * st.depth)
+    a = hf.evaluate(st, q)
+    b = hf.evaluate(hf.setup(x, 30.0, p=20, N_max=32), q)
+    assert D.relative_error(a, b) < 1e-13
+    with pytest.raises(ValueError):
+        st.set_degrees([30] * st.depth)
