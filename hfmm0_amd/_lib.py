@@ -69,6 +69,12 @@ def _declare(lib):
         "hfmm_host_hhat": (I, [D, I, vp, vp]),
         "hfmm_host_rotation": (I, [D, I, vp]),
         "hfmm_host_coax": (I, [I, D, D, D, D, I, I, vp]),
+        "hfmm_mlp_create": (I, [vp, ctypes.POINTER(vp)]),
+        "hfmm_mlp_params": (I, [vp, vp]),
+        "hfmm_mlp_forward": (I, [vp, vp, L, vp]),
+        "hfmm_mlp_gradient": (I, [vp, vp, vp, L, vp, vp]),
+        "hfmm_mlp_train": (I, [vp, vp, vp, L, I, I, ctypes.c_float, I, vp]),
+        "hfmm_mlp_destroy": (I, [vp]),
         "hfmm_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
@@ -83,7 +89,8 @@ EXPORTED = [
     "hfmm_plan_export_list", "hfmm_evaluate", "hfmm_benchmark", "hfmm_plan_destroy", "hfmm_direct",
     "hfmm_tree_create", "hfmm_tree_info_get", "hfmm_tree_export", "hfmm_tree_export_list",
     "hfmm_tree_destroy", "hfmm_host_jhat", "hfmm_host_hhat", "hfmm_host_rotation", "hfmm_host_coax",
-    "hfmm_last_error",
+    "hfmm_mlp_create", "hfmm_mlp_params", "hfmm_mlp_forward", "hfmm_mlp_gradient", "hfmm_mlp_train",
+    "hfmm_mlp_destroy", "hfmm_last_error",
 ]
 
 

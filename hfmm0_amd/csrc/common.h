@@ -15,6 +15,7 @@ struct Error {
     std::string msg;
 };
 [[noreturn]] void fail(int code, const std::string& msg);
+void set_last_error(const std::string& msg);   // message returned by hfmm_last_error()
 
 // ---------------------------------------------------------------------------------
 // Octree (canonical order: level ascending, Morton code ascending)

@@ -21,6 +21,7 @@ namespace hfmm {
 void fail(int code, const std::string& msg) { throw Error{code, msg}; }
 
 static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 #define HIPCHK(call)                                                                              \
     do {                                                                                          \

@@ -1,0 +1,177 @@
+"""Learned truncation degree: per-level features, the MLP (HIP kernels via the C ABI),
+label generation by the FMM itself, and the certified evaluate with the plain fallback.
+
+Role in the reference: order selection belongs to ``jaxfmm/tune.py``'s
+``AutoEvaluator`` (certify on sampled targets, climb a (p, theta) ladder,
+``tune.py:375-516``).  Here a small network predicts the degree of every tree level
+from cheap features; the plain fixed-degree FMM is both its fallback and its judge.
+
+Features of level l (normalised to O(1)):
+  0 level / 10
+  1 k * box diameter / 50
+  2 log2(mean points per box of the level) / 10      (source count)
+  3 log2(max points per box of the level) / 10       (target count)
+  4 log10(||q||_2 / sqrt(N))                          (charge norm)
+  5 mean V-list size per box / 189                    (neighbour count)
+Target: (p_min - 6) / 26, p_min the smallest degree whose truncation error at that
+level, with every other level at the table degree, stays below the level budget.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+N_IN = 6
+N_PARAMS = 8833
+P_LO, P_SPAN = 6, 26
+
+
+def init_params(seed):
+    """Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights, zero biases."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fan_out, fan_in, bias in ((64, 6, 64), (64, 64, 64), (64, 64, 64), (1, 64, 1)):
+        a = 1.0 / math.sqrt(fan_in)
+        parts.append(rng.uniform(-a, a, fan_out * fan_in).astype(np.float32))
+        parts.append(np.zeros(bias, np.float32))
+    out = np.concatenate(parts)
+    assert out.size == N_PARAMS
+    return out
+
+
+class DegreeMLP:
+    """The network on the GPU (BF16-rounded weights, FP32 accumulation)."""
+
+    def __init__(self, params=None, seed=0):
+        p = init_params(seed) if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != N_PARAMS:
+            raise ValueError(f"expected {N_PARAMS} parameters")
+        h = ctypes.c_void_p()
+        L.check(L.lib().hfmm_mlp_create(L.ptr(p), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h is not None:
+            L.lib().hfmm_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        out = np.empty(N_PARAMS, np.float32)
+        L.check(L.lib().hfmm_mlp_params(self._h, L.ptr(out)))
+        return out
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, N_IN)
+        y = np.empty(len(X), np.float32)
+        L.check(L.lib().hfmm_mlp_forward(self._h, L.ptr(X), len(X), L.ptr(y)))
+        return y
+
+    def gradient(self, X, y):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, N_IN)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.empty(N_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        L.check(L.lib().hfmm_mlp_gradient(self._h, L.ptr(X), L.ptr(y), len(X), L.ptr(g), L.ptr(loss)))
+        return float(loss[0]), g
+
+    def train(self, X, y, epochs=200, batch=64, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, N_IN)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        hist = np.empty(epochs, np.float32)
+        L.check(L.lib().hfmm_mlp_train(self._h, L.ptr(X), L.ptr(y), len(X), int(epochs), int(batch),
+                                       ctypes.c_float(lr), int(seed), L.ptr(hist)))
+        return hist
+
+    def predict_degrees(self, feats, margin=1.0, p_min=4, p_max=23):
+        """Per-level degrees from features: ceil(prediction + margin), clamped."""
+        raw = self.forward(feats) * P_SPAN + P_LO
+        return np.clip(np.ceil(raw + margin), p_min, p_max).astype(int)
+
+    def save(self, path):
+        np.savez_compressed(path, params=self.params)
+
+    @classmethod
+    def load(cls, path):
+        with np.load(path, allow_pickle=False) as f:
+            return cls(params=f["params"])
+
+
+def level_features(tree, charges, k):
+    """Feature rows for levels 2..depth of a tree dict (hfmm0_amd.tree.build_tree)."""
+    q = np.asarray(charges)
+    n = len(q)
+    qn = math.log10(max(float(np.linalg.norm(q)) / math.sqrt(n), 1e-30))
+    level, count = np.asarray(tree["level"]), np.asarray(tree["count"])
+    V = np.asarray(tree["V"])
+    vt_level = level[V[:, 0]] if len(V) else np.zeros(0, int)
+    rows = []
+    for lv in range(2, tree["depth"] + 1):
+        sel = level == lv
+        nb = int(sel.sum())
+        if nb == 0:
+            rows.append([lv / 10.0, 0.0, 0.0, 0.0, qn, 0.0])
+            continue
+        diam = 2.0 * math.sqrt(3.0) * tree["extent"] * 0.5 ** (lv + 1)
+        mean_c = float(count[sel].mean())
+        max_c = float(count[sel].max())
+        nv = float((vt_level == lv).sum()) / nb
+        rows.append([lv / 10.0, k * diam / 50.0, math.log2(max(mean_c, 1.0)) / 10.0,
+                     math.log2(max(max_c, 1.0)) / 10.0, qn, nv / 189.0])
+    return np.array(rows, dtype=np.float32)
+
+
+def label_levels(state, q, budget=5e-7, p_lo=4):
+    """Smallest degree per level (2..depth) keeping that level's truncation error, with
+    the other levels at their table degree, at or below `budget` (relative to the result
+    at the table degrees).  Uses evaluate-time degrees: no new setup per trial."""
+    from .api import evaluate
+    table = state.p_table
+    state.set_degrees([0] * state.depth)
+    ref = evaluate(state, q)
+    nref = np.linalg.norm(ref)
+    labels = []
+    for lv in range(2, state.depth + 1):
+        lo, hi = p_lo, table[lv - 1]
+        while lo < hi:                      # smallest p with error <= budget
+            mid = (lo + hi) // 2
+            degs = [0] * state.depth
+            degs[lv - 1] = mid
+            state.set_degrees(degs)
+            err = np.linalg.norm(evaluate(state, q) - ref) / nref
+            if err <= budget:
+                hi = mid
+            else:
+                lo = mid + 1
+        labels.append(lo)
+    state.set_degrees([0] * state.depth)
+    return np.array(labels)
+
+
+def sampled_error(positions, charges, k, phi, n_samp=256, seed=0):
+    """Relative error of phi against the GPU FP64 direct sum on sampled targets."""
+    from .api import direct
+    n = len(positions)
+    idx = np.sort(np.random.default_rng(seed).choice(n, min(n_samp, n), replace=False))
+    ref = direct(positions, charges, k, idx)
+    return float(np.linalg.norm(phi[idx] - ref) / max(np.linalg.norm(ref), 1e-300))
+
+
+def evaluate_certified(tuned, plain, positions, charges, k, tol=1e-6, n_samp=256):
+    """Evaluate with the tuned plan; if the sampled-target error misses `tol`, rerun
+    with the plain fixed-degree plan.  Returns (phi, error, used_plain)."""
+    from .api import evaluate
+    phi = evaluate(tuned, charges)
+    err = sampled_error(positions, charges, k, phi, n_samp)
+    if err <= tol:
+        return phi, err, False
+    phi = evaluate(plain, charges)
+    return phi, sampled_error(positions, charges, k, phi, n_samp), True

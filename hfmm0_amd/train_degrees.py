@@ -1,0 +1,69 @@
+"""Generates the learned-degree training data with the FMM itself and trains the MLP on
+the GPU.  Run on a GPU:  python -m hfmm0_amd.train_degrees --problems 200 --out DIR
+
+For every seeded problem (hfmm0_amd.workloads.mixed_problem, N log-uniform) it builds one
+plan at the table degree, labels every level by the smallest degree whose truncation
+error stays within the budget (hfmm0_amd.learned.label_levels), and records the level
+features.  The network is then trained by the HIP kernels (hfmm_mlp_train).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import api
+from .learned import P_LO, P_SPAN, DegreeMLP, label_levels, level_features
+from .tree import build_tree
+from .workloads import mixed_problem
+
+
+def generate(n_problems, seed0=1000, n_range=(10_000, 40_000), k_range=(1.0, 60.0), leaf=64, p_tab=23,
+             budget=5e-7, log=None):
+    feats, labels, meta = [], [], []
+    for i in range(n_problems):
+        rng = np.random.default_rng(seed0 + i)
+        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        x, q, k, kind = mixed_problem(n, k_range, seed=seed0 + i)
+        t0 = time.time()
+        st = api.setup(x, k, p=p_tab, N_max=leaf)
+        lab = label_levels(st, q, budget=budget)
+        tree = st.tree()
+        f = level_features(tree, q, k)
+        st.close()
+        feats.append(f)
+        labels.append(lab)
+        meta += [(i, n, k, kind, lv) for lv in range(2, len(lab) + 2)]
+        if log:
+            print(f"problem {i}: n={n} k={k:.1f} {kind} depth={len(lab) + 1} labels={lab.tolist()} "
+                  f"({time.time() - t0:.2f}s)", file=log, flush=True)
+    return np.vstack(feats), np.concatenate(labels), meta
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--problems", type=int, default=200)
+    ap.add_argument("--out", default="gpurun_out")
+    ap.add_argument("--epochs", type=int, default=400)
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+    os.makedirs(args.out, exist_ok=True)
+    X, p, meta = generate(args.problems, log=sys.stdout)
+    np.savez_compressed(os.path.join(args.out, "degree_data.npz"), X=X, p=p,
+                        meta=np.array([json.dumps(m) for m in meta]))
+    y = ((p - P_LO) / P_SPAN).astype(np.float32)
+    n_train = int(0.8 * len(X))
+    net = DegreeMLP(seed=args.seed)
+    hist = net.train(X[:n_train], y[:n_train], epochs=args.epochs, batch=64, lr=2e-3, seed=args.seed)
+    pred = net.predict_degrees(X[n_train:], margin=0.0, p_min=2, p_max=40)
+    resid = pred - p[n_train:]
+    print(f"train loss {hist[0]:.4f} -> {hist[-1]:.5f}; held-out degree error: mean {resid.mean():+.2f}, "
+          f"under-predicted {np.mean(resid < 0):.1%}, |err|<=1 {np.mean(np.abs(resid) <= 1):.1%}", flush=True)
+    net.save(os.path.join(args.out, "degree_mlp.npz"))
+
+
+if __name__ == "__main__":
+    main()

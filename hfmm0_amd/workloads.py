@@ -28,3 +28,33 @@ def f_m2l(p):
 
 
 P2P_FLOPS_PER_PAIR = 23      # 3 sub, 5 r^2, rsqrt, r, kr, sin, cos, 2 scale, 8 complex MAC
+
+
+def mixed_problem(n, k_range=(1.0, 60.0), seed=0):
+    """One seeded problem of the learned-degree training distribution: N points that are
+    uniform in the unit cube, on the sphere |x - c| = 0.45 (c = (0.5, 0.5, 0.5)), in
+    Gaussian clusters (sigma 0.02, centres uniform in [0.1, 0.9]^3), or a mix of the
+    three; complex N(0,1) charges; k uniform in k_range."""
+    rng = np.random.default_rng(seed)
+    kind = int(rng.integers(0, 4))
+    k = float(rng.uniform(*k_range))
+
+    def sphere(m):
+        v = rng.normal(size=(m, 3))
+        return 0.5 + 0.45 * v / np.linalg.norm(v, axis=1, keepdims=True)
+
+    def clusters(m):
+        c = rng.uniform(0.1, 0.9, (8, 3))
+        return c[rng.integers(0, 8, m)] + rng.normal(scale=0.02, size=(m, 3))
+
+    if kind == 0:
+        x = rng.uniform(0.0, 1.0, (n, 3))
+    elif kind == 1:
+        x = sphere(n)
+    elif kind == 2:
+        x = clusters(n)
+    else:
+        a, b = n // 3, n // 3
+        x = np.vstack([rng.uniform(0.0, 1.0, (a, 3)), sphere(b), clusters(n - a - b)])
+    q = rng.normal(size=n) + 1j * rng.normal(size=n)
+    return x, q, k, ("uniform", "sphere", "clusters", "mixed")[kind]

@@ -133,6 +133,23 @@ int hfmm_host_hhat(double x, int p, double* out_re, double* out_im);
 int hfmm_host_rotation(double theta, int p, double* out);
 int hfmm_host_coax(int kind, double k, double t, double rs, double rt, int ps, int pt, double* out);
 
+/* Degree-predicting MLP (learned truncation degree).  No counterpart in the reference:
+ * it takes the role of jaxfmm/tune.py's order selection (tune.py:375-516).
+ * 6 inputs -> 3 x 64 GELU -> 1 output; BF16-rounded weights, FP32 accumulation; FP32
+ * master weights trained with Adam on minibatches (MSE loss).  Host pointers throughout.
+ * Parameter layout: W1[64][6], b1[64], W2[64][64], b2[64], W3[64][64], b3[64], W4[64], b4. */
+#define HFMM_MLP_INPUTS 6
+#define HFMM_MLP_PARAMS 8833
+typedef struct hfmm_mlp hfmm_mlp;
+int hfmm_mlp_create(const float* params /* HFMM_MLP_PARAMS */, hfmm_mlp** out);
+int hfmm_mlp_params(const hfmm_mlp* mlp, float* out /* HFMM_MLP_PARAMS, FP32 master weights */);
+int hfmm_mlp_forward(hfmm_mlp* mlp, const float* x /* n x 6 */, int64_t n, float* y /* n */);
+/* gradient of mean((f(x) - y)^2) over the n samples with respect to the rounded weights */
+int hfmm_mlp_gradient(hfmm_mlp* mlp, const float* x, const float* y, int64_t n, float* grad, float* loss);
+int hfmm_mlp_train(hfmm_mlp* mlp, const float* x, const float* y, int64_t n, int epochs, int batch, float lr,
+                   int seed, float* loss_per_epoch /* epochs, may be NULL */);
+int hfmm_mlp_destroy(hfmm_mlp* mlp);
+
 const char* hfmm_last_error(void);
 
 #ifdef __cplusplus
