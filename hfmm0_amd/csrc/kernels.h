@@ -97,9 +97,9 @@ struct M2LBlockArgs {
     double2* L;                      // local expansions (FP64), accumulated into
     const int64_t* exp_off;
 };
-size_t m2l_block_lds_bytes(int p, int tb, int precision_bytes);
-void prepare_m2l_block(int p, int n_units3, int tb, int fp32);
-void launch_m2l_block(const M2LBlockArgs& a, int tb, int fp32, hipStream_t s);
+size_t m2l_block_lds_bytes(int p, int tb, int precision_bytes, int ops);
+void prepare_m2l_block(int p, int n_units3, int tb, int fp32, int ops);
+void launch_m2l_block(const M2LBlockArgs& a, int tb, int fp32, int ops, hipStream_t s);
 void launch_to_float(const double2* in, float2* out, int64_t n, hipStream_t s);
 
 }  // namespace hfmm

@@ -22,9 +22,14 @@
 
 namespace hfmm {
 
-size_t m2l_block_lds_bytes(int p, int tb, int precision_bytes) {
-    const size_t nc = (size_t)(p + 1) * (p + 1);
-    return nc * (2 * tb + 1) * precision_bytes;
+size_t m2l_block_lds_bytes(int p, int tb, int precision_bytes, int ops) {
+    const size_t P1 = p + 1, nc = P1 * P1;
+    size_t b = nc * (2 * tb + 1);
+    if (!ops) return b * precision_bytes;
+    const size_t rot_n = P1 * (2 * P1 - 1) * (2 * P1 + 1) / 3;
+    const size_t tsz = P1 * (P1 + 1) * (2 * P1 + 1) / 6;
+    b = ((b + 3) & ~(size_t)3) + ((rot_n + 3) & ~(size_t)3);
+    return b * precision_bytes + (tsz + P1) * 2 * precision_bytes;
 }
 
 namespace {
@@ -68,7 +73,10 @@ __device__ inline int sq_sum(int x) { return x * (x + 1) * (2 * x + 1) / 6; }
 
 // Work item of the kernel: entries [e0, e1) of one target block.  When a block is split
 // over several items (coarse levels, to fill the GPU) the results are added atomically.
-template <typename T, int TB, int MAXU>
+// OPS: the rotation block set, the coaxial table and the phases of the current
+// displacement are copied into LDS (reloaded only when theta or |d| change; entries are
+// sorted by them) so that the MFMA loops read every operand from LDS.
+template <typename T, int TB, int MAXU, bool OPS>
 __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     using V4 = typename Mma<T>::v4;
     using C2 = typename Mma<T>::c2;
@@ -80,6 +88,12 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     const int p = a.p, P1 = p + 1, nc = P1 * P1;
     const int PT1 = a.p_tab + 1;            // coaxial tables are packed for degree p_tab
     T* X = (T*)smem_raw;
+    const int rot_n = P1 * (2 * P1 - 1) * (2 * P1 + 1) / 3;      // rotation blocks n <= p
+    const int tsz = sq_sum(PT1);                                   // coaxial table entries
+    T* Es = X + (((size_t)nc * LDX + 3) & ~(size_t)3);            // OPS only, 16-byte aligned
+    C2* Ts = (C2*)(Es + ((rot_n + 3) & ~3));
+    C2* Ps = Ts + tsz;
+    int cur_theta = -1, cur_dist = -1;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int4 item = a.items[blockIdx.x];  // (block, e0, e1, atomic)
@@ -94,11 +108,49 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     for (int e = item.y; e < item.z; ++e) {
         const int d = a.ent_dir[e];
         const DispInfo di = a.dinfo[d];
-        const T* Et = (const T*)a.rot_t + (size_t)di.theta * a.rot_stride;   // transposed blocks
-        const T* E = (const T*)a.rot + (size_t)di.theta * a.rot_stride;
-        const C2* ph = (const C2*)a.phase + (size_t)d * a.phase_stride;
-        const C2* Tz = (const C2*)a.coax + (size_t)di.dist * a.coax_stride;
+        const T* Etg = (const T*)a.rot_t + (size_t)di.theta * a.rot_stride;  // transposed blocks
+        const T* Eg = (const T*)a.rot + (size_t)di.theta * a.rot_stride;
+        const C2* phg = (const C2*)a.phase + (size_t)d * a.phase_stride;
+        const C2* Tg = (const C2*)a.coax + (size_t)di.dist * a.coax_stride;
         const int* soff = a.ent_off + (size_t)e * TB;
+        const T* E = OPS ? Es : Eg;
+        const C2* Tz = OPS ? Ts : Tg;
+        const C2* ph = OPS ? Ps : phg;
+        if (OPS) {
+            if (di.theta != cur_theta) {
+                for (int base = 0; base < rot_n; base += 256 * 8) {
+                    T v[8];
+#pragma unroll
+                    for (int g = 0; g < 8; ++g) {
+                        const int i = base + threadIdx.x + 256 * g;
+                        v[g] = i < rot_n ? Eg[i] : (T)0;
+                    }
+#pragma unroll
+                    for (int g = 0; g < 8; ++g) {
+                        const int i = base + threadIdx.x + 256 * g;
+                        if (i < rot_n) Es[i] = v[g];
+                    }
+                }
+                cur_theta = di.theta;
+            }
+            if (di.dist != cur_dist) {
+                for (int base = 0; base < tsz; base += 256 * 4) {
+                    C2 v[4];
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const int i = base + threadIdx.x + 256 * g;
+                        if (i < tsz) v[g] = Tg[i];
+                    }
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const int i = base + threadIdx.x + 256 * g;
+                        if (i < tsz) Ts[i] = v[g];
+                    }
+                }
+                cur_dist = di.dist;
+            }
+            if (threadIdx.x < P1) Ps[threadIdx.x] = phg[threadIdx.x];
+        }
 
         // stage 0: gather + e^{i m phi}; loads of a batch are issued before any is used
         for (int base = 0; base < TB * nc; base += 256 * NG) {
@@ -121,7 +173,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                     const int j = idx / nc, c = idx - j * nc;
                     int n, m;
                     decode_nm(c, n, m);
-                    C2 pm = ph[m < 0 ? -m : m];
+                    C2 pm = phg[m < 0 ? -m : m];
                     if (m < 0) pm.y = -pm.y;
                     X[c * LDX + 2 * j] = raw[g].x * pm.x - raw[g].y * pm.y;
                     X[c * LDX + 2 * j + 1] = raw[g].x * pm.y + raw[g].y * pm.x;
@@ -256,7 +308,9 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                 const int code = a.units3[u];
                 const int n = code & 255, mt = (code >> 8) & 255, ct = code >> 16;
                 const int w = 2 * n + 1, base = n * n;
-                const T* Etn = Et + n * (2 * n - 1) * (2 * n + 1) / 3;
+                const int roff = n * (2 * n - 1) * (2 * n + 1) / 3;
+                const T* Etn = Etg + roff;      // global: transposed table, coalesced
+                const T* Esn = Es + roff;       // LDS: row-major block, read transposed
                 const int mo = mt * 16 + li;
                 const int col = ct * 16 + li;
                 const bool colok = col < NCOL;
@@ -266,7 +320,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
                     for (int q = 0; q < KC; ++q) {
                         const int kin = k0 + 4 * q + lk;
-                        av[q] = (mo < w && kin < w) ? Etn[kin * w + mo] : (T)0;
+                        av[q] = (mo < w && kin < w) ? (OPS ? Esn[mo * w + kin] : Etn[kin * w + mo]) : (T)0;
                         bv[q] = (kin < w && colok) ? X[(base + kin) * LDX + col] : (T)0;
                     }
 #pragma unroll
@@ -329,63 +383,71 @@ __global__ void to_float_kernel(const double2* in, float2* out, int64_t n) {
     for (; i < n; i += stride) out[i] = make_float2((float)in[i].x, (float)in[i].y);
 }
 
-template <typename T, int TB, int MAXU>
+template <typename T, int TB, int MAXU, bool OPS>
 void launch_one(const M2LBlockArgs* a, size_t shm, hipStream_t s) {
     if (!a) {   // prepare: allow the dynamic LDS this instantiation may use
         int dev = 0, lds = 0;
         (void)hipGetDevice(&dev);
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) lds = 65536;
-        hipError_t e = hipFuncSetAttribute((const void*)m2l_block_kernel<T, TB, MAXU>,
+        hipError_t e = hipFuncSetAttribute((const void*)m2l_block_kernel<T, TB, MAXU, OPS>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) fail(-3, std::string("hipFuncSetAttribute(m2l) failed: ") + hipGetErrorString(e));
         return;
     }
-    hipLaunchKernelGGL((m2l_block_kernel<T, TB, MAXU>), dim3(a->n_items), dim3(256), shm, s, *a);
+    hipLaunchKernelGGL((m2l_block_kernel<T, TB, MAXU, OPS>), dim3(a->n_items), dim3(256), shm, s, *a);
 }
 
-template <typename T, int TB>
+template <typename T, int TB, bool OPS>
 void launch_tb(const M2LBlockArgs* a, size_t shm, int maxu, hipStream_t s) {
     if (maxu <= 8)
-        launch_one<T, TB, 8>(a, shm, s);
+        launch_one<T, TB, 8, OPS>(a, shm, s);
     else if (maxu <= 12)
-        launch_one<T, TB, 12>(a, shm, s);
+        launch_one<T, TB, 12, OPS>(a, shm, s);
     else if (maxu <= 16)
-        launch_one<T, TB, 16>(a, shm, s);
+        launch_one<T, TB, 16, OPS>(a, shm, s);
     else if (maxu <= 24)
-        launch_one<T, TB, 24>(a, shm, s);
+        launch_one<T, TB, 24, OPS>(a, shm, s);
     else if (maxu <= 32)
-        launch_one<T, TB, 32>(a, shm, s);
+        launch_one<T, TB, 32, OPS>(a, shm, s);
     else
         fail(-9, "internal: too many M2L units per wave");
 }
 
-void dispatch(const M2LBlockArgs* a, int p, int n_units3, int tb, int fp32, hipStream_t s) {
+template <typename T, bool OPS>
+void launch_t(const M2LBlockArgs* a, size_t shm, int tb, int maxu, hipStream_t s) {
+    if (tb == 16)
+        launch_tb<T, 16, OPS>(a, shm, maxu, s);
+    else if (tb == 8)
+        launch_tb<T, 8, OPS>(a, shm, maxu, s);
+    else
+        launch_tb<T, 4, OPS>(a, shm, maxu, s);
+}
+
+void dispatch(const M2LBlockArgs* a, int p, int n_units3, int tb, int fp32, int ops, hipStream_t s) {
     const int maxu = (n_units3 + 3) / 4;
-    const size_t shm = m2l_block_lds_bytes(p, tb, fp32 ? 4 : 8);
+    const size_t shm = m2l_block_lds_bytes(p, tb, fp32 ? 4 : 8, ops);
     if (fp32) {
-        if (tb == 16)
-            launch_tb<float, 16>(a, shm, maxu, s);
-        else if (tb == 8)
-            launch_tb<float, 8>(a, shm, maxu, s);
+        if (ops)
+            launch_t<float, true>(a, shm, tb, maxu, s);
         else
-            launch_tb<float, 4>(a, shm, maxu, s);
+            launch_t<float, false>(a, shm, tb, maxu, s);
     } else {
-        if (tb == 16)
-            launch_tb<double, 16>(a, shm, maxu, s);
-        else if (tb == 8)
-            launch_tb<double, 8>(a, shm, maxu, s);
+        if (ops)
+            launch_t<double, true>(a, shm, tb, maxu, s);
         else
-            launch_tb<double, 4>(a, shm, maxu, s);
+            launch_t<double, false>(a, shm, tb, maxu, s);
     }
 }
 
 }  // namespace
 
-void prepare_m2l_block(int p, int n_units3, int tb, int fp32) { dispatch(nullptr, p, n_units3, tb, fp32, nullptr); }
+void prepare_m2l_block(int p, int n_units3, int tb, int fp32, int ops) {
+    dispatch(nullptr, p, n_units3, tb, fp32, ops, nullptr);
+}
 
-void launch_m2l_block(const M2LBlockArgs& a, int tb, int fp32, hipStream_t s) {
+void launch_m2l_block(const M2LBlockArgs& a, int tb, int fp32, int ops, hipStream_t s) {
     if (a.n_items <= 0) return;
-    dispatch(&a, a.p_tab, a.n_units3, tb, fp32, s);
+    dispatch(&a, a.p_tab, a.n_units3, tb, fp32, ops, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) fail(-3, std::string("kernel launch failed (m2l_block): ") + hipGetErrorString(e));
 }

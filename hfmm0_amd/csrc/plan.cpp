@@ -37,6 +37,7 @@ constexpr int kNumDirs = 359;
 struct M2LBlockLevel {
     bool use = false;
     int tb = 0;
+    int ops = 0;                   // operators staged in LDS
     int n_items = 0;
     int4* items = nullptr;
     int* targets = nullptr;
@@ -171,11 +172,16 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     if (p > 23) return;   // degrees above 23 use the generic kernel (coarse levels only)
     const int fp32 = P.precision == HFMM_PREC_MIXED;
     const int eb = fp32 ? 4 : 8;
-    int tb = 0;   // largest block whose LDS buffer fits two workgroups per CU, else one
-    for (int c : {16, 8, 4})
-        if (!tb && 2 * m2l_block_lds_bytes(p, c, eb) <= P.max_lds) tb = c;
-    for (int c : {16, 8, 4})
-        if (!tb && m2l_block_lds_bytes(p, c, eb) <= P.max_lds) tb = c;
+    // largest block whose LDS buffer fits two workgroups per CU, else one; operators are
+    // staged in LDS only when the buffer alone does not fit (measured: the larger LDS
+    // footprint costs more occupancy than the staged operands save)
+    int tb = 0, ops = 0;
+    for (int o : {0, 1}) {
+        for (int c : {16, 8, 4})
+            if (!tb && 2 * m2l_block_lds_bytes(p, c, eb, o) <= P.max_lds) tb = c, ops = o;
+        for (int c : {16, 8, 4})
+            if (!tb && m2l_block_lds_bytes(p, c, eb, o) <= P.max_lds) tb = c, ops = o;
+    }
     if (!tb) return;
     const int ncol = 2 * tb, nct = (ncol + 15) / 16;
     std::vector<std::pair<int, int>> u1, u2, u3;   // (cost, code)
@@ -243,6 +249,7 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     M2LBlockLevel& B = P.m2lb[L];
     B.use = true;
     B.tb = tb;
+    B.ops = ops;
     B.n_items = (int)items.size();
     B.items = P.upload(items);
     B.targets = P.upload(blk_targets);
@@ -278,7 +285,7 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     } else {
         B.coax = P.upload(ct);
     }
-    prepare_m2l_block(p, B.n_units3, tb, fp32);
+    prepare_m2l_block(p, B.n_units3, tb, fp32, ops);
 }
 
 void build_tables(hfmm_plan& P) {
@@ -669,7 +676,7 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
         a.M = fp32 ? (const void*)P.dM32 : (const void*)P.dM;
         a.L = P.dL;
         a.exp_off = P.d_expoff;
-        launch_m2l_block(a, B.tb, fp32, s);
+        launch_m2l_block(a, B.tb, fp32, B.ops, s);
     }
     mark(4);
     for (int L = 2; L < P.depth; ++L)
