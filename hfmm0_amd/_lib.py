@@ -58,7 +58,13 @@ def _declare(lib):
         "hfmm_evaluate": (I, [vp, vp, I, vp, I, vp]),
         "hfmm_plan_set_degrees": (I, [vp, vp]),
         "hfmm_benchmark": (I, [vp, vp, I, vp, I, I, ctypes.POINTER(D), vp]),
+        "hfmm_benchmark_m2l_levels": (I, [vp, vp, I, vp, I, vp]),
         "hfmm_plan_destroy": (I, [vp]),
+        "hfmm_batch_create": (I, [I, vp, vp, vp, vp, ctypes.POINTER(vp)]),
+        "hfmm_batch_plan": (I, [vp, I, ctypes.POINTER(vp)]),
+        "hfmm_batch_evaluate": (I, [vp, vp, I, vp, I, vp]),
+        "hfmm_batch_benchmark": (I, [vp, vp, I, vp, I, I, ctypes.POINTER(D)]),
+        "hfmm_batch_destroy": (I, [vp]),
         "hfmm_direct": (I, [vp, L, vp, D, vp, L, vp]),
         "hfmm_tree_create": (I, [vp, L, I, I, ctypes.POINTER(vp)]),
         "hfmm_tree_info_get": (I, [vp, ctypes.POINTER(PlanInfo)]),
@@ -86,11 +92,13 @@ def _declare(lib):
 
 EXPORTED = [
     "hfmm_opts_default", "hfmm_plan_create", "hfmm_plan_info_get", "hfmm_plan_export_tree", "hfmm_plan_set_degrees",
-    "hfmm_plan_export_list", "hfmm_evaluate", "hfmm_benchmark", "hfmm_plan_destroy", "hfmm_direct",
+    "hfmm_plan_export_list", "hfmm_evaluate", "hfmm_benchmark", "hfmm_benchmark_m2l_levels",
+    "hfmm_plan_destroy", "hfmm_direct",
     "hfmm_tree_create", "hfmm_tree_info_get", "hfmm_tree_export", "hfmm_tree_export_list",
     "hfmm_tree_destroy", "hfmm_host_jhat", "hfmm_host_hhat", "hfmm_host_rotation", "hfmm_host_coax",
     "hfmm_mlp_create", "hfmm_mlp_params", "hfmm_mlp_forward", "hfmm_mlp_gradient", "hfmm_mlp_train",
-    "hfmm_mlp_destroy", "hfmm_last_error",
+    "hfmm_mlp_destroy", "hfmm_batch_create", "hfmm_batch_plan", "hfmm_batch_evaluate", "hfmm_batch_benchmark",
+    "hfmm_batch_destroy", "hfmm_last_error",
 ]
 
 

@@ -91,17 +91,33 @@ def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_le
     ``p_level`` optionally gives the truncation degree per tree level (index 0 is
     level 1); ``precision`` is ``"fp64"`` (plain FMM) or ``"mixed"`` (FP32 near field and
     FP32 M2L on matrix cores).  ``m2l_kernel="generic"`` forces the thread-per-coefficient
-    M2L kernel instead of the matrix-core block kernel.
+    M2L kernel instead of the matrix-core block kernel; an integer 8, 16 or 32 asks the
+    block kernel for that many target boxes per workgroup (used where it fits).
     """
+    x, k = _check_problem(positions, k)
+    o = make_opts(p, N_max, max_depth, precision=precision, p_level=p_level, use_graph=use_graph,
+                  m2l_kernel=m2l_kernel, _k=k)
+    h = ctypes.c_void_p()
+    L.check(L.lib().hfmm_plan_create(L.ptr(x), len(x), k, ctypes.byref(o), ctypes.byref(h)))
+    return HFMMState(h, len(x), k, o)
+
+
+def _check_problem(positions, k):
     x = np.asarray(positions, dtype=np.float64)
     if x.ndim != 2 or x.shape[1] != 3 or len(x) == 0 or not np.isfinite(x).all():
         raise ValueError("positions must be a nonempty finite (N,3) array")
+    return np.ascontiguousarray(x), k
+
+
+def make_opts(p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, use_graph=True,
+              m2l_kernel="auto", _k=0.0):
+    """Validated plan options (the C ``hfmm_opts``), in the reference's validation order
+    after the positions check (``HFMM.py:57-64``, ``HFMM.py:14-16``)."""
     if p < 2 or int(p) != p or N_max < 1 or int(N_max) != N_max:
         raise ValueError("p must be an integer >=2; N_max must be a positive integer")
     if max_depth is not None and (max_depth < 1 or int(max_depth) != max_depth):
         raise ValueError("max_depth must be a positive integer or None")
-    k = float(k)
-    if not np.isfinite(k):
+    if not np.isfinite(float(_k)):
         raise ValueError("k must be a finite real number")
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
@@ -111,18 +127,15 @@ def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_le
     o.max_depth = int(max_depth) if max_depth is not None else 0
     o.precision = PRECISIONS[precision]
     o.use_graph = 1 if use_graph else 0
-    if m2l_kernel not in ("auto", "generic"):
-        raise ValueError("m2l_kernel must be 'auto' or 'generic'")
-    o.m2l_kernel = 0 if m2l_kernel == "auto" else 1
+    if m2l_kernel not in ("auto", "generic", 8, 16, 32):
+        raise ValueError("m2l_kernel must be 'auto', 'generic', 8, 16 or 32")
+    o.m2l_kernel = {"auto": 0, "generic": 1}.get(m2l_kernel, m2l_kernel)
     if p_level is not None:
         for i, v in enumerate(p_level):
             if i + 1 >= L.MAX_LEVELS:
                 break
             o.p_level[i + 1] = int(v)
-    x = np.ascontiguousarray(x)
-    h = ctypes.c_void_p()
-    L.check(L.lib().hfmm_plan_create(L.ptr(x), len(x), k, ctypes.byref(o), ctypes.byref(h)))
-    return HFMMState(h, len(x), k, o)
+    return o
 
 
 def evaluate(state, charges):
@@ -172,6 +185,16 @@ def benchmark(state, q_dev, out_dev, warmup=3, reps=20, phases=False):
         names = ["gather", "p2m", "m2m", "m2l", "l2l", "l2p", "p2p"]
         return med.value, dict(zip(names, ph.tolist()))
     return med.value
+
+
+def benchmark_m2l_levels(state, q_dev, out_dev, reps=5):
+    """Average milliseconds of each level's M2L launch ({level: ms}), hipEvents around
+    every launch on the plan's stream over `reps` instrumented evaluates."""
+    qt = L.C128 if q_dev.is_complex() else L.F64
+    ms = np.zeros(L.MAX_LEVELS)
+    L.check(L.lib().hfmm_benchmark_m2l_levels(state.handle, ctypes.c_void_p(q_dev.data_ptr()), qt,
+                                              ctypes.c_void_p(out_dev.data_ptr()), int(reps), L.ptr(ms)))
+    return {lv: float(ms[lv]) for lv in range(L.MAX_LEVELS) if ms[lv] > 0}
 
 
 def direct(positions, charges, k, targets):

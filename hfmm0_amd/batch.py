@@ -1,0 +1,113 @@
+"""Batches of independent problems evaluated together (libhfmm ``hfmm_batch_*``).
+
+One plan per problem; an evaluate forks the problems over a few streams balanced by
+point count and joins them inside one captured hipGraph, so that many small problems
+share the GPU instead of running one after another.  The reference has no batch API:
+its tuner driver (``jaxfmm/tune.py:32-172``) loops over problems.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from .api import HFMMState, _check_problem, make_opts
+
+
+class Batch:
+    """``Batch(positions_list, k_list, p=..., N_max=..., precision=...)``; every keyword of
+    :func:`hfmm0_amd.setup` is accepted and applies to all problems (``p_level`` may be
+    a list of per-problem lists)."""
+
+    def __init__(self, positions_list, k_list, p=4, N_max=16, max_depth=None, *, precision="fp64",
+                 p_level=None, m2l_kernel="auto"):
+        if len(positions_list) != len(k_list) or not positions_list:
+            raise ValueError("need one k per problem and at least one problem")
+        xs, ks, opts = [], [], (L.Opts * len(positions_list))()
+        for i, (x, k) in enumerate(zip(positions_list, k_list)):
+            x, k = _check_problem(x, k)
+            pl = p_level[i] if (p_level is not None and np.ndim(p_level[0]) > 0) else p_level
+            opts[i] = make_opts(p, N_max, max_depth, precision=precision, p_level=pl, m2l_kernel=m2l_kernel, _k=k)
+            xs.append(x)
+            ks.append(float(k))
+        self._xs = xs                      # keep the position arrays alive during create
+        ptrs = (ctypes.c_void_p * len(xs))(*[x.ctypes.data for x in xs])
+        ns = np.array([len(x) for x in xs], dtype=np.int64)
+        karr = np.array(ks, dtype=np.float64)
+        h = ctypes.c_void_p()
+        L.check(L.lib().hfmm_batch_create(len(xs), ptrs, L.ptr(ns), L.ptr(karr), opts, ctypes.byref(h)))
+        self._h = h
+        self.n = [int(v) for v in ns]
+        self.k = ks
+
+    def __len__(self):
+        return len(self.n)
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise ValueError("the batch has been released")
+        return self._h
+
+    def plan(self, i):
+        """State view of problem i (owned by the batch; valid while the batch lives)."""
+        h = ctypes.c_void_p()
+        L.check(L.lib().hfmm_batch_plan(self.handle, int(i), ctypes.byref(h)))
+        st = HFMMState(h, self.n[i], self.k[i], None)
+        st.close = lambda: None            # the batch owns it
+        return st
+
+    def evaluate(self, charges_list):
+        """phi of every problem (complex128, input order), host arrays in and out."""
+        if len(charges_list) != len(self):
+            raise ValueError("need one charge vector per problem")
+        qs, complex_ = [], any(np.iscomplexobj(q) for q in charges_list)
+        for q, n in zip(charges_list, self.n):
+            q = np.asarray(q)
+            if q.shape != (n,):
+                raise ValueError("charges must be a real array of shape (N,)")
+            qs.append(np.ascontiguousarray(q, dtype=np.complex128 if complex_ else np.float64))
+        outs = [np.empty(n, dtype=np.complex128) for n in self.n]
+        qp = (ctypes.c_void_p * len(qs))(*[q.ctypes.data for q in qs])
+        op = (ctypes.c_void_p * len(outs))(*[o.ctypes.data for o in outs])
+        L.check(L.lib().hfmm_batch_evaluate(self.handle, qp, L.C128 if complex_ else L.F64, op, L.HOST, None))
+        return outs
+
+    def _dev_ptrs(self, q_devs, out_devs):
+        import torch
+        if len(q_devs) != len(self) or len(out_devs) != len(self):
+            raise ValueError("need one charge vector and one output per problem")
+        cplx = q_devs[0].is_complex()
+        for q, o, n in zip(q_devs, out_devs, self.n):
+            if q.shape != (n,) or o.shape != (n,) or o.dtype != torch.complex128 or q.is_complex() != cplx:
+                raise ValueError("charges must be a real array of shape (N,)")
+            if not (q.is_cuda and o.is_cuda and q.is_contiguous() and o.is_contiguous()):
+                raise ValueError("device evaluate needs contiguous GPU tensors")
+        qp = (ctypes.c_void_p * len(self))(*[q.data_ptr() for q in q_devs])
+        op = (ctypes.c_void_p * len(self))(*[o.data_ptr() for o in out_devs])
+        return qp, op, (L.C128 if cplx else L.F64)
+
+    def evaluate_device(self, q_devs, out_devs):
+        """All problems with device-resident torch tensors (one graph replay)."""
+        import torch
+        qp, op, qt = self._dev_ptrs(q_devs, out_devs)
+        stream = torch.cuda.current_stream().cuda_stream
+        L.check(L.lib().hfmm_batch_evaluate(self.handle, qp, qt, op, L.DEVICE, ctypes.c_void_p(stream)))
+        return out_devs
+
+    def benchmark(self, q_devs, out_devs, warmup=2, reps=10):
+        """Median milliseconds of one batch evaluate (hipEvents around graph replays)."""
+        qp, op, qt = self._dev_ptrs(q_devs, out_devs)
+        med = ctypes.c_double()
+        L.check(L.lib().hfmm_batch_benchmark(self.handle, qp, qt, op, int(warmup), int(reps), ctypes.byref(med)))
+        return med.value
+
+    def close(self):
+        if self._h is not None:
+            L.lib().hfmm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -34,6 +34,11 @@ void launch_fill_zero(double2* buf, int64_t n, hipStream_t s);
 void launch_gather_charges(const void* q, int q_type, const int* perm, int64_t n, int conj, double2* q_sorted,
                            hipStream_t s);
 
+// LDS bytes per staged point of the P2M kernel at degree p
+inline size_t p2m_bytes_per_point(int p) {
+    const size_t P1 = p + 1;
+    return 2 * P1 * 16 + 2 * 16 + P1 * (P1 + 1) / 2 * 8;
+}
 void launch_p2m(const int* leaves, int n_items, int chunk, DevBoxes bx, DevLevels lv, const double* X,
                 const double* Y, const double* Z, const double2* q, double k, int pmax, double2* M, hipStream_t s);
 
@@ -70,7 +75,21 @@ void launch_p2p(int precision, int n_leaves, const int* leaf_box, DevBoxes bx, c
 void launch_direct(const double* X, const double* Y, const double* Z, const double2* q, int64_t n, double k,
                    const double* tx, const double* ty, const double* tz, int64_t nt, double2* out, hipStream_t s);
 
-// M2L on matrix cores over target blocks (see m2l_mfma.hip).
+// M2L on matrix cores over target blocks (m2l_mfma.hip).  The operator tables are
+// stored as MFMA A fragments: for a tile (row tile mt, k group kg of four 4-wide
+// k-steps) lane l holds, as four consecutive values j = 0..3,
+// A[16 mt + l % 16][16 kg + 4 j + l / 16], zero outside the block.
+constexpr int kM2LBlockMaxP = 23;  // largest degree handled by the block kernel
+constexpr int kM2LMaxAccRegs = 128; // VGPRs per lane for the stage-3 accumulators
+__host__ __device__ constexpr int m2l_tiles(int len) { return (len + 15) / 16; }
+// accumulator VGPRs of a (MAXU units per wave, TB targets, element size) configuration
+__host__ __device__ constexpr int m2l_acc_regs(int maxu, int tb, int elem_bytes) {
+    return maxu * (2 * tb / 16) * 4 * (elem_bytes / 4);
+}
+// LDS elements of the expansion block ((p+1)^2 rows x 2 TB + 1), padded to 16 bytes
+__host__ __device__ constexpr int m2l_lds_x_elems(int p, int tb) {
+    return ((p + 1) * (p + 1) * (2 * tb + 1) + 3) & ~3;
+}
 struct M2LBlockArgs {
     int p;                           // effective degree of this level (<= p_tab)
     int p_tab;                       // degree the tables, units and LDS were sized for
@@ -79,27 +98,29 @@ struct M2LBlockArgs {
     const int* blk_targets;          // blocks x TB target boxes (-1: empty slot)
     const unsigned short* ent_dir;   // per entry: direction index
     const int* ent_off;              // per entry: TB source expansion offsets (-1: none)
-    const int* units1;               // forward rotation units: n | ct << 16 (all row tiles)
+    const int* units1;               // forward rotation units: degree n
     int n_units1;
-    const int* units2;               // coaxial units: (m + p) | ct << 16 (all row tiles)
+    const int* units2;               // coaxial units: m + p_tab
     int n_units2;
-    const int* units3;               // back rotation units: n | mt << 8 | ct << 16
+    const int* units3;               // back rotation units: n | row tile << 8
     int n_units3;
-    const void* rot;                 // rotation tables (T)
-    const void* rot_t;               // the same tables with every degree block transposed
+    const void* rot1;                // per rotation: fragments of E_n^T, n = 0..kM2LBlockMaxP (T)
+    const void* rot3;                // per rotation: fragments of E_n (T)
+    const int* rot_off;              // offset of degree n's fragments within one rotation
     int rot_stride;
     const void* phase;               // e^{i m phi} per direction (T complex)
     int phase_stride;
     const DispInfo* dinfo;
-    const void* coax;                // coaxial M2L tables of the level, each m block transposed
+    const void* coax;                // per distance: fragments of T^m, re then im per lane (T)
+    const int* coax_off;             // offset of order m's fragments within one distance
     int coax_stride;
     const void* M;                   // multipoles (T complex)
     double2* L;                      // local expansions (FP64), accumulated into
     const int64_t* exp_off;
 };
-size_t m2l_block_lds_bytes(int p, int tb, int precision_bytes, int ops);
-void prepare_m2l_block(int p, int n_units3, int tb, int fp32, int ops);
-void launch_m2l_block(const M2LBlockArgs& a, int tb, int fp32, int ops, hipStream_t s);
+size_t m2l_block_lds_bytes(int p, int tb, int precision_bytes);
+void prepare_m2l_block(int p, int n_units3, int tb, int fp32);
+void launch_m2l_block(const M2LBlockArgs& a, int tb, int fp32, hipStream_t s);
 void launch_to_float(const double2* in, float2* out, int64_t n, hipStream_t s);
 
 }  // namespace hfmm

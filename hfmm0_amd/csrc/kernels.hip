@@ -37,6 +37,21 @@ __device__ inline void nm_decode(int c, int& n, int& m) {
     m = c - n * n - n;
 }
 
+// Legendre recurrence coefficients (fully normalised, Condon-Shortley phase):
+//   P_m^m = -d_m sin(theta) P_{m-1}^{m-1},  d_m = sqrt((2m+1)/(2m))
+//   P_{m+1}^m = e_m cos(theta) P_m^m,       e_m = sqrt(2m+3)
+//   P_n^m = a_nm (cos(theta) P_{n-1}^m - b_nm P_{n-2}^m)
+// Also the downward-recurrence factors of jhat, miller[n] = 1 / ((2n+1)(2n+3)).
+constexpr int kMillerN = 512;
+struct LegCoef {
+    double a[kMaxP + 1][kMaxP + 1];
+    double b[kMaxP + 1][kMaxP + 1];
+    double d[kMaxP + 1];
+    double e[kMaxP + 1];
+    double miller[kMillerN];
+};
+__constant__ LegCoef c_leg;
+
 // jhat_n(x) for n = 0..p by Miller's downward recurrence
 //   jhat_{n-1} = jhat_n - x^2 jhat_{n+1} / ((2n+1)(2n+3)),
 // normalised with jhat_0 = sin(x)/x (or jhat_1 = 3(sin x - x cos x)/x^3 when that is
@@ -51,7 +66,8 @@ __device__ void jhat_dev(double x, int p, double* out, int stride) {
     double a_next = 0.0, a = 1.0;
     const double x2 = x * x;
     for (int n = N; n >= 1; --n) {
-        const double am1 = a - x2 * a_next / ((2.0 * n + 1.0) * (2.0 * n + 3.0));
+        const double f = n < kMillerN ? c_leg.miller[n] : 1.0 / ((2.0 * n + 1.0) * (2.0 * n + 3.0));
+        const double am1 = a - x2 * a_next * f;
         a_next = a;
         a = am1;
         if (n - 1 <= p) out[(n - 1) * stride] = a;
@@ -75,17 +91,6 @@ __device__ void jhat_dev(double x, int p, double* out, int stride) {
     for (int n = 0; n <= p; ++n) out[n * stride] *= scale;
 }
 
-// Legendre recurrence coefficients (fully normalised, Condon-Shortley phase):
-//   P_m^m = -d_m sin(theta) P_{m-1}^{m-1},  d_m = sqrt((2m+1)/(2m))
-//   P_{m+1}^m = e_m cos(theta) P_m^m,       e_m = sqrt(2m+3)
-//   P_n^m = a_nm (cos(theta) P_{n-1}^m - b_nm P_{n-2}^m)
-struct LegCoef {
-    double a[kMaxP + 1][kMaxP + 1];
-    double b[kMaxP + 1][kMaxP + 1];
-    double d[kMaxP + 1];
-    double e[kMaxP + 1];
-};
-__constant__ LegCoef c_leg;
 
 // ---------------------------------------------------------------------------------
 // small utility kernels
@@ -112,9 +117,11 @@ __global__ void gather_charges_kernel(const void* q, int q_type, const int* perm
 
 // ---------------------------------------------------------------------------------
 // P2M: one workgroup per leaf; points processed in chunks staged in LDS.
-// Phase A: work items (point j, order m) build P_n^m(cos beta_j), e^{-i m alpha_j}
-//          and (m == 0) the radial factors q_j jhat_n(k rho_j)(rho_j/R)^n.
-// Phase B: one thread per coefficient (n, m) accumulates over the chunk.
+//  1. one thread per point: offsets, angles and the radial factors
+//     q_j jhat_n(k rho_j) (rho_j / R)^n;
+//  2. one thread per (point j, order m): the normalised Legendre column
+//     P_n^m(cos beta_j), n = m..p, and e^{-i m alpha_j} (powers of e^{-i alpha_j});
+//  3. one thread per coefficient (n, m): accumulation over the chunk's points.
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void p2m_kernel(const int* leaves, int chunk, DevBoxes bx, DevLevels lv,
                                                   const double* X, const double* Y, const double* Z,
@@ -125,9 +132,10 @@ __global__ __launch_bounds__(256) void p2m_kernel(const int* leaves, int chunk, 
     const int p = lv.p[L];
     const double R = lv.R[L];
     const int P1 = p + 1, tri = P1 * (P1 + 1) / 2, nc = P1 * P1;
-    double2* s_qrad = (double2*)smem;
-    double2* s_ph = s_qrad + chunk * P1;
-    double* s_leg = (double*)(s_ph + chunk * P1);
+    double2* s_qrad = (double2*)smem;             // chunk x P1
+    double2* s_ph = s_qrad + chunk * P1;          // chunk x P1
+    double2* s_geo = s_ph + chunk * P1;           // chunk x 2: (cos b, sin b), (cos a, sin a)
+    double* s_leg = (double*)(s_geo + 2 * chunk); // chunk x tri
     const double cx = bx.centre[3 * b], cy = bx.centre[3 * b + 1], cz = bx.centre[3 * b + 2];
     const int ps = bx.pstart[b], cnt = bx.count[b];
     double2 acc[kAccPerThread];
@@ -136,21 +144,37 @@ __global__ __launch_bounds__(256) void p2m_kernel(const int* leaves, int chunk, 
 
     for (int j0 = 0; j0 < cnt; j0 += chunk) {
         const int nch = min(chunk, cnt - j0);
-        for (int w = threadIdx.x; w < nch * P1; w += blockDim.x) {
-            const int j = w / P1, m = w - j * P1;
+        for (int j = threadIdx.x; j < nch; j += blockDim.x) {
             const int i = ps + j0 + j;
             const double dx = X[i] - cx, dy = Y[i] - cy, dz = Z[i] - cz;
-            const double rho = sqrt(dx * dx + dy * dy + dz * dz);
-            const double rxy = sqrt(dx * dx + dy * dy);
-            const double ct = rho > 0.0 ? dz / rho : 1.0;
-            const double st = rho > 0.0 ? rxy / rho : 0.0;
-            const double alpha = atan2(dy, dx);
-            double sa, ca;
-            sincos(m * alpha, &sa, &ca);
-            s_ph[j * P1 + m] = make_double2(ca, -sa);
-            double* Pl = s_leg + j * tri;
+            const double rxy2 = dx * dx + dy * dy;
+            const double rho = sqrt(rxy2 + dz * dz), rxy = sqrt(rxy2);
+            s_geo[2 * j] = rho > 0.0 ? make_double2(dz / rho, rxy / rho) : make_double2(1.0, 0.0);
+            s_geo[2 * j + 1] = rxy > 0.0 ? make_double2(dx / rxy, dy / rxy) : make_double2(1.0, 0.0);
+            double* rad = &s_qrad[j * P1].x;
+            jhat_dev(k * rho, p, rad, 2);
+            const double2 qq = q[i];
+            const double rr = rho / R;
+            double pw = 1.0;
+            for (int n = 0; n <= p; ++n) {
+                const double g = rad[2 * n] * pw;
+                s_qrad[j * P1 + n] = make_double2(qq.x * g, qq.y * g);
+                pw *= rr;
+            }
+        }
+        __syncthreads();
+        for (int w = threadIdx.x; w < nch * P1; w += blockDim.x) {
+            const int j = w / P1, m = w - j * P1;
+            const double2 cb = s_geo[2 * j], ea = s_geo[2 * j + 1];
+            const double ct = cb.x, st = cb.y;
             double pmm = 0.28209479177387814;   // 1/sqrt(4 pi)
-            for (int i2 = 1; i2 <= m; ++i2) pmm *= -c_leg.d[i2] * st;
+            double2 z = make_double2(1.0, 0.0);
+            for (int i2 = 1; i2 <= m; ++i2) {
+                pmm *= -c_leg.d[i2] * st;
+                z = cmul(z, ea);
+            }
+            s_ph[j * P1 + m] = make_double2(z.x, -z.y);
+            double* Pl = s_leg + j * tri;
             Pl[m * (m + 1) / 2 + m] = pmm;
             if (m < p) {
                 double p2 = pmm, p1 = c_leg.e[m] * ct * pmm;
@@ -162,18 +186,6 @@ __global__ __launch_bounds__(256) void p2m_kernel(const int* leaves, int chunk, 
                     p1 = pn;
                 }
             }
-            if (m == 0) {
-                double* rad = &s_qrad[j * P1].x;
-                jhat_dev(k * rho, p, rad, 2);
-                const double2 qq = q[i];
-                const double rr = rho / R;
-                double pw = 1.0;
-                for (int n = 0; n <= p; ++n) {
-                    const double g = rad[2 * n] * pw;
-                    s_qrad[j * P1 + n] = make_double2(qq.x * g, qq.y * g);
-                    pw *= rr;
-                }
-            }
         }
         __syncthreads();
 #pragma unroll
@@ -183,16 +195,14 @@ __global__ __launch_bounds__(256) void p2m_kernel(const int* leaves, int chunk, 
                 int n, m;
                 nm_decode(c, n, m);
                 const int am = m < 0 ? -m : m;
+                const double sg = (m < 0 && (am & 1)) ? -1.0 : 1.0;
+                const double sy = m < 0 ? -1.0 : 1.0;
                 double2 a = acc[t];
                 for (int j = 0; j < nch; ++j) {
                     const double2 qr = s_qrad[j * P1 + n];
-                    double pl = s_leg[j * tri + n * (n + 1) / 2 + am];
-                    double2 ph = s_ph[j * P1 + am];
-                    if (m < 0) {
-                        ph.y = -ph.y;
-                        if (am & 1) pl = -pl;
-                    }
-                    a = cmadd(a, qr, make_double2(pl * ph.x, pl * ph.y));
+                    const double pl = sg * s_leg[j * tri + n * (n + 1) / 2 + am];
+                    const double2 ph = s_ph[j * P1 + am];
+                    a = cmadd(a, qr, make_double2(pl * ph.x, pl * sy * ph.y));
                 }
                 acc[t] = a;
             }
@@ -578,9 +588,14 @@ __global__ __launch_bounds__(256) void direct_kernel(const double* X, const doub
 // launchers
 // ---------------------------------------------------------------------------------
 void upload_legendre_coefficients() {
-    static bool done = false;
-    if (done) return;
+    // constant memory is per device: upload once on every device used
+    constexpr int kMaxDevices = 64;
+    static bool done[kMaxDevices] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) fail(-3, "hipGetDevice failed");
+    if (done[dev]) return;
     LegCoef h{};
+    for (int n = 0; n < kMillerN; ++n) h.miller[n] = 1.0 / ((2.0 * n + 1.0) * (2.0 * n + 3.0));
     for (int m = 0; m <= kMaxP; ++m) {
         h.d[m] = m > 0 ? std::sqrt((2.0 * m + 1.0) / (2.0 * m)) : 0.0;
         h.e[m] = std::sqrt(2.0 * m + 3.0);
@@ -593,7 +608,7 @@ void upload_legendre_coefficients() {
     }
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_leg), &h, sizeof(h)) != hipSuccess)
         fail(-3, "hipMemcpyToSymbol(Legendre coefficients) failed");
-    done = true;
+    done[dev] = true;
 }
 
 static void check_launch(const char* what) {
@@ -619,8 +634,7 @@ void launch_p2m(const int* leaves, int n_items, int chunk, DevBoxes bx, DevLevel
                 const double* Y, const double* Z, const double2* q, double k, int pmax, double2* M, hipStream_t s) {
     if (n_items <= 0) return;
     upload_legendre_coefficients();
-    const int P1 = pmax + 1, tri = P1 * (P1 + 1) / 2;
-    const size_t shm = (size_t)chunk * (2 * P1 * sizeof(double2) + tri * sizeof(double));
+    const size_t shm = (size_t)chunk * p2m_bytes_per_point(pmax);
     hipLaunchKernelGGL(p2m_kernel, dim3(n_items), dim3(256), shm, s, leaves, chunk, bx, lv, X, Y, Z, q, k, M);
     check_launch("p2m");
 }

@@ -9,12 +9,15 @@
 //   1. rotates each degree block in place: X_n <- E_n(theta_d)^T X_n   (real x complex)
 //   2. applies the coaxial operator per order m in place: X^m <- T^{|m|}(|d|) X^m, as
 //      Re(T) * X + Im(T) * (X with re/im columns swapped and signed);
-//   3. rotates back, W_n = E_n X_n, multiplies by e^{-i m phi_d} (partner lane holds the
-//      other half of the complex value) and accumulates in registers.
-// Every product is a 16x16x4 MFMA tile (A from the operator tables, B from LDS; the
-// FP32 and FP64 instructions place output rows differently, see Mma<T>::row);
-// the accumulators of stage 3 stay in VGPRs across all displacements and are added
-// to the FP64 local expansions once at the end.
+//   3. rotates back, W_n = E_n X_n, multiplies by e^{-i m phi_d} (the partner lane holds
+//      the other half of the complex value) and accumulates in registers.
+// Every product is a 16x16x4 MFMA tile.  The A operands come from operator tables that
+// the host stores in fragment order, zero-padded to whole 16x16 tiles, so a lane loads
+// the A values of four k-steps with one 16-byte (FP32) or two 16-byte (FP64) loads and
+// no bounds test.  B operands are read from LDS with row indices clamped into the block
+// (their products with the zero padding of A vanish), and each A fragment is applied to
+// every 16-column tile of the block.  The stage-3 accumulators stay in registers across
+// all displacements and are added to the FP64 local expansions once at the end.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -22,14 +25,9 @@
 
 namespace hfmm {
 
-size_t m2l_block_lds_bytes(int p, int tb, int precision_bytes, int ops) {
-    const size_t P1 = p + 1, nc = P1 * P1;
-    size_t b = nc * (2 * tb + 1);
-    if (!ops) return b * precision_bytes;
-    const size_t rot_n = P1 * (2 * P1 - 1) * (2 * P1 + 1) / 3;
-    const size_t tsz = P1 * (P1 + 1) * (2 * P1 + 1) / 6;
-    b = ((b + 3) & ~(size_t)3) + ((rot_n + 3) & ~(size_t)3);
-    return b * precision_bytes + (tsz + P1) * 2 * precision_bytes;
+// LDS: the expansion block, then p+1 complex phases
+size_t m2l_block_lds_bytes(int p, int tb, int precision_bytes) {
+    return ((size_t)m2l_lds_x_elems(p, tb) + 2 * (size_t)(p + 1)) * precision_bytes;
 }
 
 namespace {
@@ -45,6 +43,11 @@ struct Mma<float> {
     }
     // output element r of lane group lk (= lane / 16) is row 4 lk + r of the 16x16 tile
     __device__ static inline int row(int lk, int r) { return 4 * lk + r; }
+    __device__ static inline v4 load4(const float* p) { return *(const v4*)p; }
+    // value of the neighbouring lane (lane ^ 1): quad permutation (1, 0, 3, 2)
+    __device__ static inline float swap_pair(float v) {
+        return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+    }
 };
 template <>
 struct Mma<double> {
@@ -55,6 +58,16 @@ struct Mma<double> {
     }
     // output element r of lane group lk (= lane / 16) is row lk + 4 r of the 16x16 tile
     __device__ static inline int row(int lk, int r) { return lk + 4 * r; }
+    __device__ static inline v4 load4(const double* p) {
+        const double2 a = *(const double2*)p, b = *(const double2*)(p + 2);
+        return v4{a.x, a.y, b.x, b.y};
+    }
+    __device__ static inline double swap_pair(double v) {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_mov_dpp((int)b, 0xB1, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+        return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+    }
 };
 
 __device__ inline void decode_nm(int c, int& n, int& m) {
@@ -64,95 +77,46 @@ __device__ inline void decode_nm(int c, int& n, int& m) {
     m = c - n * n - n;
 }
 
-constexpr int kBlockMaxP = 23;                          // largest degree of the block kernel
-constexpr int kMaxTiles = (2 * kBlockMaxP + 1 + 15) / 16; // row tiles of a degree block
-constexpr int kMaxTiles2 = (kBlockMaxP + 1 + 15) / 16;    // row tiles of an order block
+constexpr int kMaxTiles = (2 * kM2LBlockMaxP + 1 + 15) / 16;   // row tiles of a degree block
+constexpr int kMaxTiles2 = (kM2LBlockMaxP + 1 + 15) / 16;      // row tiles of an order block
 
-// sum_{i=1}^{x} i^2
-__device__ inline int sq_sum(int x) { return x * (x + 1) * (2 * x + 1) / 6; }
-
-// Work item of the kernel: entries [e0, e1) of one target block.  When a block is split
-// over several items (coarse levels, to fill the GPU) the results are added atomically.
-// OPS: the rotation block set, the coaxial table and the phases of the current
-// displacement are copied into LDS (reloaded only when theta or |d| change; entries are
-// sorted by them) so that the MFMA loops read every operand from LDS.
-template <typename T, int TB, int MAXU, bool OPS>
+// Work item: entries [e0, e1) of one target block.  When a block is split over several
+// items (coarse levels, to fill the GPU) the results are added atomically.
+template <typename T, int TB, int MAXU>
 __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     using V4 = typename Mma<T>::v4;
     using C2 = typename Mma<T>::c2;
     constexpr int NCOL = 2 * TB;            // (pair, re/im) columns
     constexpr int LDX = NCOL + 1;           // odd row stride in LDS
-    constexpr int KC = 8;                   // k-steps whose operands are loaded together
+    constexpr int NCT = NCOL / 16;          // 16-wide column tiles
     constexpr int NG = 16;                  // gathered values per thread and batch
     extern __shared__ char smem_raw[];
     const int p = a.p, P1 = p + 1, nc = P1 * P1;
-    const int PT1 = a.p_tab + 1;            // coaxial tables are packed for degree p_tab
     T* X = (T*)smem_raw;
-    const int rot_n = P1 * (2 * P1 - 1) * (2 * P1 + 1) / 3;      // rotation blocks n <= p
-    const int tsz = sq_sum(PT1);                                   // coaxial table entries
-    T* Es = X + (((size_t)nc * LDX + 3) & ~(size_t)3);            // OPS only, 16-byte aligned
-    C2* Ts = (C2*)(Es + ((rot_n + 3) & ~3));
-    C2* Ps = Ts + tsz;
-    int cur_theta = -1, cur_dist = -1;
+    C2* Ps = (C2*)(X + m2l_lds_x_elems(a.p_tab, TB));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int4 item = a.items[blockIdx.x];  // (block, e0, e1, atomic)
-    const int blk = item.x;
-    const int* tg = a.blk_targets + (size_t)blk * TB;
+    const int* tg = a.blk_targets + (size_t)item.x * TB;
     const C2* Mexp = (const C2*)a.M;
 
-    V4 acc[MAXU];
+    V4 acc[MAXU][NCT];
 #pragma unroll
-    for (int u = 0; u < MAXU; ++u) acc[u] = V4{0, 0, 0, 0};
+    for (int u = 0; u < MAXU; ++u)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct) acc[u][ct] = V4{0, 0, 0, 0};
 
     for (int e = item.y; e < item.z; ++e) {
         const int d = a.ent_dir[e];
         const DispInfo di = a.dinfo[d];
-        const T* Etg = (const T*)a.rot_t + (size_t)di.theta * a.rot_stride;  // transposed blocks
-        const T* Eg = (const T*)a.rot + (size_t)di.theta * a.rot_stride;
+        const T* R1 = (const T*)a.rot1 + (size_t)di.theta * a.rot_stride;
+        const T* R3 = (const T*)a.rot3 + (size_t)di.theta * a.rot_stride;
+        const T* Tz = (const T*)a.coax + (size_t)di.dist * a.coax_stride;
         const C2* phg = (const C2*)a.phase + (size_t)d * a.phase_stride;
-        const C2* Tg = (const C2*)a.coax + (size_t)di.dist * a.coax_stride;
         const int* soff = a.ent_off + (size_t)e * TB;
-        const T* E = OPS ? Es : Eg;
-        const C2* Tz = OPS ? Ts : Tg;
-        const C2* ph = OPS ? Ps : phg;
-        if (OPS) {
-            if (di.theta != cur_theta) {
-                for (int base = 0; base < rot_n; base += 256 * 8) {
-                    T v[8];
-#pragma unroll
-                    for (int g = 0; g < 8; ++g) {
-                        const int i = base + threadIdx.x + 256 * g;
-                        v[g] = i < rot_n ? Eg[i] : (T)0;
-                    }
-#pragma unroll
-                    for (int g = 0; g < 8; ++g) {
-                        const int i = base + threadIdx.x + 256 * g;
-                        if (i < rot_n) Es[i] = v[g];
-                    }
-                }
-                cur_theta = di.theta;
-            }
-            if (di.dist != cur_dist) {
-                for (int base = 0; base < tsz; base += 256 * 4) {
-                    C2 v[4];
-#pragma unroll
-                    for (int g = 0; g < 4; ++g) {
-                        const int i = base + threadIdx.x + 256 * g;
-                        if (i < tsz) v[g] = Tg[i];
-                    }
-#pragma unroll
-                    for (int g = 0; g < 4; ++g) {
-                        const int i = base + threadIdx.x + 256 * g;
-                        if (i < tsz) Ts[i] = v[g];
-                    }
-                }
-                cur_dist = di.dist;
-            }
-            if (threadIdx.x < P1) Ps[threadIdx.x] = phg[threadIdx.x];
-        }
 
-        // stage 0: gather + e^{i m phi}; loads of a batch are issued before any is used
+        // stage 0: gather + e^{i m phi}; the loads of a batch are issued before any is used
+        if (threadIdx.x < P1) Ps[threadIdx.x] = phg[threadIdx.x];
         for (int base = 0; base < TB * nc; base += 256 * NG) {
             C2 raw[NG];
 #pragma unroll
@@ -182,165 +146,164 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
         }
         __syncthreads();
 
-        // stage 1: X_n <- E_n^T X_n in place.  A unit is one degree n and one 16-column
-        // group; the wave computes all its row tiles before writing them back, and no other
-        // wave touches those rows and columns.  A[i][k] = E[k][i]: rows of E are contiguous.
+        // stage 1: X_n <- E_n^T X_n in place.  A unit is one degree n with all its row and
+        // column tiles: the wave writes back only after all products are formed, and no
+        // other wave touches the rows of degree n.
         for (int u = wave; u < a.n_units1; u += 4) {
-            const int code = a.units1[u];
-            const int n = code & 255, ct = code >> 16;
+            const int n = a.units1[u];
             if (n > p) continue;
             const int w = 2 * n + 1, base = n * n;
-            const int ntile = (w + 15) >> 4;
-            const T* En = E + n * (2 * n - 1) * (2 * n + 1) / 3;
-            const int col = ct * 16 + li;
-            const bool colok = col < NCOL;
-            V4 c4[kMaxTiles];
+            const int nt = m2l_tiles(w), nks = (w + 3) >> 2;
+            const T* A = R1 + a.rot_off[n];
+            V4 c4[kMaxTiles][NCT];
 #pragma unroll
-            for (int mt = 0; mt < kMaxTiles; ++mt) c4[mt] = V4{0, 0, 0, 0};
-            for (int k0 = 0; k0 < w; k0 += 4 * KC) {
-                T bv[KC];
+            for (int mt = 0; mt < kMaxTiles; ++mt)
 #pragma unroll
-                for (int q = 0; q < KC; ++q) {
-                    const int kin = k0 + 4 * q + lk;
-                    bv[q] = (kin < w && colok) ? X[(base + kin) * LDX + col] : (T)0;
-                }
-#pragma unroll
-                for (int mt = 0; mt < kMaxTiles; ++mt) {
-                    if (mt < ntile) {
-                        const int mo = mt * 16 + li;
-                        T av[KC];
-#pragma unroll
-                        for (int q = 0; q < KC; ++q) {
-                            const int kin = k0 + 4 * q + lk;
-                            av[q] = (mo < w && kin < w) ? En[kin * w + mo] : (T)0;
-                        }
-#pragma unroll
-                        for (int q = 0; q < KC; ++q)
-                            if (k0 + 4 * q < w) c4[mt] = Mma<T>::mma(av[q], bv[q], c4[mt]);
-                    }
-                }
-            }
-            if (colok) {
+                for (int ct = 0; ct < NCT; ++ct) c4[mt][ct] = V4{0, 0, 0, 0};
+            for (int kg = 0; kg < nt; ++kg) {
+                V4 av[kMaxTiles];
 #pragma unroll
                 for (int mt = 0; mt < kMaxTiles; ++mt)
-                    if (mt < ntile) {
+                    if (mt < nt) av[mt] = Mma<T>::load4(A + ((size_t)(mt * nt + kg) * 64 + lane) * 4);
+                T bv[4][NCT];
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int row = mt * 16 + Mma<T>::row(lk, r);
-                            if (row < w) X[(base + row) * LDX + col] = c4[mt][r];
-                        }
-                    }
+                for (int j = 0; j < 4; ++j) {
+                    const int row = base + min(16 * kg + 4 * j + lk, w - 1);
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct) bv[j][ct] = X[row * LDX + ct * 16 + li];
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (4 * kg + j < nks)
+#pragma unroll
+                        for (int mt = 0; mt < kMaxTiles; ++mt)
+                            if (mt < nt)
+#pragma unroll
+                                for (int ct = 0; ct < NCT; ++ct)
+                                    c4[mt][ct] = Mma<T>::mma(av[mt][j], bv[j][ct], c4[mt][ct]);
             }
+#pragma unroll
+            for (int mt = 0; mt < kMaxTiles; ++mt)
+                if (mt < nt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = mt * 16 + Mma<T>::row(lk, r);
+                        if (row < w)
+#pragma unroll
+                            for (int ct = 0; ct < NCT; ++ct) X[(base + row) * LDX + ct * 16 + li] = c4[mt][ct][r];
+                    }
         }
         __syncthreads();
 
-        // stage 2: X^m <- T^{|m|} X^m in place, one order m and one column group per unit;
-        // A[i][k] = T[n][n'] from the transposed table (16 rows of a k-step contiguous)
+        // stage 2: X^m <- T^{|m|} X^m in place, one order m (all tiles) per unit.  The
+        // table blocks are built for p_tab; source degrees beyond the evaluated p are
+        // read as zero.
         for (int u = wave; u < a.n_units2; u += 4) {
-            const int code = a.units2[u];
-            const int m = (code & 255) - a.p_tab, ct = code >> 16;
+            const int m = a.units2[u] - a.p_tab;
             const int am = m < 0 ? -m : m;
             if (am > p) continue;
             const int len = P1 - am;
-            const int lent = PT1 - am;          // row length of the packed (transposed) block
-            const int ntile = (len + 15) >> 4;
-            const C2* Tm = Tz + (sq_sum(PT1) - sq_sum(PT1 - am));
-            const int col = ct * 16 + li;
-            const bool colok = col < NCOL;
-            const T sgn = (col & 1) ? (T)1 : (T)-1;
-            V4 c4[kMaxTiles2];
+            const int ntab = m2l_tiles(a.p_tab + 1 - am);   // tiles of the table block
+            const int nt = m2l_tiles(len), nks = (len + 3) >> 2;
+            const T* A = Tz + a.coax_off[am];
+            V4 c4[kMaxTiles2][NCT];
 #pragma unroll
-            for (int nt = 0; nt < kMaxTiles2; ++nt) c4[nt] = V4{0, 0, 0, 0};
-            for (int k0 = 0; k0 < len; k0 += 4 * KC) {
-                T b0[KC], b1[KC];
+            for (int t = 0; t < kMaxTiles2; ++t)
 #pragma unroll
-                for (int q = 0; q < KC; ++q) {
-                    const int kin = k0 + 4 * q + lk;
+                for (int ct = 0; ct < NCT; ++ct) c4[t][ct] = V4{0, 0, 0, 0};
+            for (int kg = 0; kg < nt; ++kg) {
+                V4 are[kMaxTiles2], aim[kMaxTiles2];
+#pragma unroll
+                for (int t = 0; t < kMaxTiles2; ++t)
+                    if (t < nt) {
+                        const T* At = A + ((size_t)(t * ntab + kg) * 64 + lane) * 8;
+                        are[t] = Mma<T>::load4(At);
+                        aim[t] = Mma<T>::load4(At + 4);
+                    }
+                T b0[4][NCT], b1[4][NCT];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int kin = 16 * kg + 4 * j + lk;
                     const bool kv = kin < len;
-                    const int nn = am + kin;
+                    const int nn = am + min(kin, len - 1);
                     const int cin = nn * nn + nn + m;
-                    b0[q] = (kv && colok) ? X[cin * LDX + col] : (T)0;
-                    b1[q] = (kv && colok) ? sgn * X[cin * LDX + (col ^ 1)] : (T)0;
-                }
 #pragma unroll
-                for (int nt = 0; nt < kMaxTiles2; ++nt) {
-                    if (nt < ntile) {
-                        const int nrow = nt * 16 + li;
-                        C2 tv[KC];
-#pragma unroll
-                        for (int q = 0; q < KC; ++q) {
-                            const int kin = k0 + 4 * q + lk;
-                            tv[q].x = 0;
-                            tv[q].y = 0;
-                            if (nrow < len && kin < len) tv[q] = Tm[kin * lent + nrow];
-                        }
-#pragma unroll
-                        for (int q = 0; q < KC; ++q)
-                            if (k0 + 4 * q < len) {
-                                c4[nt] = Mma<T>::mma(tv[q].x, b0[q], c4[nt]);
-                                c4[nt] = Mma<T>::mma(tv[q].y, b1[q], c4[nt]);
-                            }
+                    for (int ct = 0; ct < NCT; ++ct) {
+                        const int col = ct * 16 + li;
+                        const T v0 = X[cin * LDX + col], v1 = X[cin * LDX + (col ^ 1)];
+                        b0[j][ct] = kv ? v0 : (T)0;
+                        b1[j][ct] = kv ? ((col & 1) ? v1 : -v1) : (T)0;
                     }
                 }
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (4 * kg + j < nks)
+#pragma unroll
+                        for (int t = 0; t < kMaxTiles2; ++t)
+                            if (t < nt)
+#pragma unroll
+                                for (int ct = 0; ct < NCT; ++ct) {
+                                    c4[t][ct] = Mma<T>::mma(are[t][j], b0[j][ct], c4[t][ct]);
+                                    c4[t][ct] = Mma<T>::mma(aim[t][j], b1[j][ct], c4[t][ct]);
+                                }
             }
-            if (colok) {
 #pragma unroll
-                for (int nt = 0; nt < kMaxTiles2; ++nt)
-                    if (nt < ntile) {
+            for (int t = 0; t < kMaxTiles2; ++t)
+                if (t < nt)
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int row = nt * 16 + Mma<T>::row(lk, r);
-                            if (row < len) {
-                                const int nn = am + row;
-                                X[(nn * nn + nn + m) * LDX + col] = c4[nt][r];
-                            }
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = t * 16 + Mma<T>::row(lk, r);
+                        if (row < len) {
+                            const int nn = am + row;
+#pragma unroll
+                            for (int ct = 0; ct < NCT; ++ct) X[(nn * nn + nn + m) * LDX + ct * 16 + li] = c4[t][ct][r];
                         }
                     }
-            }
         }
         __syncthreads();
 
-        // stage 3: W_n = E_n Z_n (A[i][k] = E[i][k] = Et[k][i]), times e^{-i m phi}, accumulate
+        // stage 3: W_n = E_n X_n, times e^{-i m phi}, accumulated; unit = (n, row tile)
 #pragma unroll
         for (int uu = 0; uu < MAXU; ++uu) {
             const int u = wave + 4 * uu;
             if (u < a.n_units3 && (a.units3[u] & 255) <= p) {
                 const int code = a.units3[u];
-                const int n = code & 255, mt = (code >> 8) & 255, ct = code >> 16;
+                const int n = code & 255, mt = code >> 8;
                 const int w = 2 * n + 1, base = n * n;
-                const int roff = n * (2 * n - 1) * (2 * n + 1) / 3;
-                const T* Etn = Etg + roff;      // global: transposed table, coalesced
-                const T* Esn = Es + roff;       // LDS: row-major block, read transposed
-                const int mo = mt * 16 + li;
-                const int col = ct * 16 + li;
-                const bool colok = col < NCOL;
-                V4 c4 = V4{0, 0, 0, 0};
-                for (int k0 = 0; k0 < w; k0 += 4 * KC) {
-                    T av[KC], bv[KC];
+                const int nt = m2l_tiles(w), nks = (w + 3) >> 2;
+                const T* A = R3 + a.rot_off[n] + (size_t)mt * nt * 256;
+                V4 c4[NCT];
 #pragma unroll
-                    for (int q = 0; q < KC; ++q) {
-                        const int kin = k0 + 4 * q + lk;
-                        av[q] = (mo < w && kin < w) ? (OPS ? Esn[mo * w + kin] : Etn[kin * w + mo]) : (T)0;
-                        bv[q] = (kin < w && colok) ? X[(base + kin) * LDX + col] : (T)0;
+                for (int ct = 0; ct < NCT; ++ct) c4[ct] = V4{0, 0, 0, 0};
+                for (int kg = 0; kg < nt; ++kg) {
+                    const V4 av = Mma<T>::load4(A + ((size_t)kg * 64 + lane) * 4);
+                    T bv[4][NCT];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int row = base + min(16 * kg + 4 * j + lk, w - 1);
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ++ct) bv[j][ct] = X[row * LDX + ct * 16 + li];
                     }
 #pragma unroll
-                    for (int q = 0; q < KC; ++q)
-                        if (k0 + 4 * q < w) c4 = Mma<T>::mma(av[q], bv[q], c4);
+                    for (int j = 0; j < 4; ++j)
+                        if (4 * kg + j < nks)
+#pragma unroll
+                            for (int ct = 0; ct < NCT; ++ct) c4[ct] = Mma<T>::mma(av[j], bv[j][ct], c4[ct]);
                 }
-                const bool im_part = (col & 1) != 0;
+                // rows past the block are zero (A is zero-padded); their phase is irrelevant
+                const bool im_part = (li & 1) != 0;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int row = mt * 16 + Mma<T>::row(lk, r);
-                    const int m = row - n;
+                    const int m = mt * 16 + Mma<T>::row(lk, r) - n;
                     const int am = m < 0 ? -m : m;
-                    C2 pm;
-                    pm.x = 1;
-                    pm.y = 0;
-                    if (row < w) pm = ph[am];
+                    const C2 pm = Ps[min(am, p)];
                     const T cs = pm.x, sn = m < 0 ? -pm.y : pm.y;
-                    const T v = c4[r];
-                    const T partner = __shfl_xor(v, 1);
-                    acc[uu][r] += im_part ? (v * cs - partner * sn) : (v * cs + partner * sn);
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct) {
+                        const T v = c4[ct][r];
+                        const T partner = Mma<T>::swap_pair(v);
+                        acc[uu][ct][r] += im_part ? (v * cs - partner * sn) : (v * cs + partner * sn);
+                    }
                 }
             }
         }
@@ -353,22 +316,22 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
         const int u = wave + 4 * uu;
         if (u < a.n_units3 && (a.units3[u] & 255) <= p) {
             const int code = a.units3[u];
-            const int n = code & 255, mt = (code >> 8) & 255, ct = code >> 16;
-            const int col = ct * 16 + li;
-            if (col < NCOL) {
-                const int j = col >> 1, ri = col & 1;
-                const int t = tg[j];
+            const int n = code & 255, mt = code >> 8;
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) {
+                const int col = ct * 16 + li;
+                const int t = tg[col >> 1];
                 if (t >= 0) {
-                    double* dst = (double*)(a.L + a.exp_off[t]);
+                    double* dst = (double*)(a.L + a.exp_off[t]) + (col & 1);
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int row = mt * 16 + Mma<T>::row(lk, r);
                         if (row < 2 * n + 1) {
-                            double* x = dst + 2 * (n * n + row) + ri;
+                            double* x = dst + 2 * (n * n + row);
                             if (item.w)
-                                atomicAdd(x, (double)acc[uu][r]);
+                                atomicAdd(x, (double)acc[uu][ct][r]);
                             else
-                                *x += (double)acc[uu][r];
+                                *x += (double)acc[uu][ct][r];
                         }
                     }
                 }
@@ -383,71 +346,71 @@ __global__ void to_float_kernel(const double2* in, float2* out, int64_t n) {
     for (; i < n; i += stride) out[i] = make_float2((float)in[i].x, (float)in[i].y);
 }
 
-template <typename T, int TB, int MAXU, bool OPS>
+template <typename T, int TB, int MAXU>
 void launch_one(const M2LBlockArgs* a, size_t shm, hipStream_t s) {
     if (!a) {   // prepare: allow the dynamic LDS this instantiation may use
         int dev = 0, lds = 0;
         (void)hipGetDevice(&dev);
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) lds = 65536;
-        hipError_t e = hipFuncSetAttribute((const void*)m2l_block_kernel<T, TB, MAXU, OPS>,
+        hipError_t e = hipFuncSetAttribute((const void*)m2l_block_kernel<T, TB, MAXU>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) fail(-3, std::string("hipFuncSetAttribute(m2l) failed: ") + hipGetErrorString(e));
         return;
     }
-    hipLaunchKernelGGL((m2l_block_kernel<T, TB, MAXU, OPS>), dim3(a->n_items), dim3(256), shm, s, *a);
+    hipLaunchKernelGGL((m2l_block_kernel<T, TB, MAXU>), dim3(a->n_items), dim3(256), shm, s, *a);
 }
 
-template <typename T, int TB, bool OPS>
+template <typename T, int TB, int MAXU>
+void launch_checked(const M2LBlockArgs* a, size_t shm, hipStream_t s) {
+    if constexpr (m2l_acc_regs(MAXU, TB, sizeof(T)) <= kM2LMaxAccRegs)
+        launch_one<T, TB, MAXU>(a, shm, s);
+    else
+        fail(-9, "internal: M2L block configuration exceeds the register budget");
+}
+
+template <typename T, int TB>
 void launch_tb(const M2LBlockArgs* a, size_t shm, int maxu, hipStream_t s) {
-    if (maxu <= 8)
-        launch_one<T, TB, 8, OPS>(a, shm, s);
+    if (maxu <= 4)
+        launch_checked<T, TB, 4>(a, shm, s);
+    else if (maxu <= 8)
+        launch_checked<T, TB, 8>(a, shm, s);
     else if (maxu <= 12)
-        launch_one<T, TB, 12, OPS>(a, shm, s);
+        launch_checked<T, TB, 12>(a, shm, s);
     else if (maxu <= 16)
-        launch_one<T, TB, 16, OPS>(a, shm, s);
-    else if (maxu <= 24)
-        launch_one<T, TB, 24, OPS>(a, shm, s);
-    else if (maxu <= 32)
-        launch_one<T, TB, 32, OPS>(a, shm, s);
+        launch_checked<T, TB, 16>(a, shm, s);
     else
         fail(-9, "internal: too many M2L units per wave");
 }
 
-template <typename T, bool OPS>
-void launch_t(const M2LBlockArgs* a, size_t shm, int tb, int maxu, hipStream_t s) {
-    if (tb == 16)
-        launch_tb<T, 16, OPS>(a, shm, maxu, s);
-    else if (tb == 8)
-        launch_tb<T, 8, OPS>(a, shm, maxu, s);
-    else
-        launch_tb<T, 4, OPS>(a, shm, maxu, s);
-}
-
-void dispatch(const M2LBlockArgs* a, int p, int n_units3, int tb, int fp32, int ops, hipStream_t s) {
+void dispatch(const M2LBlockArgs* a, int p, int n_units3, int tb, int fp32, hipStream_t s) {
     const int maxu = (n_units3 + 3) / 4;
-    const size_t shm = m2l_block_lds_bytes(p, tb, fp32 ? 4 : 8, ops);
+    const size_t shm = m2l_block_lds_bytes(p, tb, fp32 ? 4 : 8);
     if (fp32) {
-        if (ops)
-            launch_t<float, true>(a, shm, tb, maxu, s);
+        if (tb == 32)
+            launch_tb<float, 32>(a, shm, maxu, s);
+        else if (tb == 16)
+            launch_tb<float, 16>(a, shm, maxu, s);
+        else if (tb == 8)
+            launch_tb<float, 8>(a, shm, maxu, s);
         else
-            launch_t<float, false>(a, shm, tb, maxu, s);
+            fail(-9, "internal: unsupported M2L block size");
     } else {
-        if (ops)
-            launch_t<double, true>(a, shm, tb, maxu, s);
+        if (tb == 16)
+            launch_tb<double, 16>(a, shm, maxu, s);
+        else if (tb == 8)
+            launch_tb<double, 8>(a, shm, maxu, s);
         else
-            launch_t<double, false>(a, shm, tb, maxu, s);
+            fail(-9, "internal: unsupported M2L block size");
     }
 }
 
 }  // namespace
 
-void prepare_m2l_block(int p, int n_units3, int tb, int fp32, int ops) {
-    dispatch(nullptr, p, n_units3, tb, fp32, ops, nullptr);
-}
+void prepare_m2l_block(int p, int n_units3, int tb, int fp32) { dispatch(nullptr, p, n_units3, tb, fp32, nullptr); }
 
-void launch_m2l_block(const M2LBlockArgs& a, int tb, int fp32, int ops, hipStream_t s) {
+void launch_m2l_block(const M2LBlockArgs& a, int tb, int fp32, hipStream_t s) {
     if (a.n_items <= 0) return;
-    dispatch(&a, a.p_tab, a.n_units3, tb, fp32, ops, s);
+    dispatch(&a, a.p_tab, a.n_units3, tb, fp32, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) fail(-3, std::string("kernel launch failed (m2l_block): ") + hipGetErrorString(e));
 }

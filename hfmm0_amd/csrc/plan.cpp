@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <tuple>
 #include <numeric>
 #include <string>
@@ -37,7 +38,6 @@ constexpr int kNumDirs = 359;
 struct M2LBlockLevel {
     bool use = false;
     int tb = 0;
-    int ops = 0;                   // operators staged in LDS
     int n_items = 0;
     int4* items = nullptr;
     int* targets = nullptr;
@@ -49,7 +49,8 @@ struct M2LBlockLevel {
     int n_units2 = 0;
     int* units3 = nullptr;
     int n_units3 = 0;
-    void* coax = nullptr;          // table in the kernel's precision
+    void* coax = nullptr;          // A fragments of the coaxial tables, kernel precision
+    int* coax_off = nullptr;       // per order m
     int coax_stride = 0;
 };
 
@@ -85,7 +86,7 @@ struct hfmm_plan {
     std::vector<int64_t> exp_off;
     int64_t n_coef = 0;
     int64_t n_p2p_pairs = 0;
-    int n_leaves = 0, n_far_leaves = 0, p2m_chunk = 8;
+    int n_leaves = 0, n_far_leaves = 0, p2m_chunk = 8, p_leafmax = 2;
 
     std::vector<void*> allocs;
     double *dX = nullptr, *dY = nullptr, *dZ = nullptr;
@@ -105,9 +106,11 @@ struct hfmm_plan {
     M2LBlockLevel m2lb[kMaxLevels];
     int m2l_kernel = 0;
     size_t max_lds = 65536;
-    float* d_rot32 = nullptr;
-    float* d_rot32_t = nullptr;
-    double* d_rot_t = nullptr;
+    // rotation blocks as MFMA A fragments (E^T for the forward, E for the back rotation)
+    double *d_rf1 = nullptr, *d_rf3 = nullptr;
+    float *d_rf1_32 = nullptr, *d_rf3_32 = nullptr;
+    int* d_rf_off = nullptr;
+    int rf_stride = 0;
     float2* d_phase32 = nullptr;
     float2* dM32 = nullptr;
     double2 *dM = nullptr, *dL = nullptr, *d_q = nullptr, *d_far = nullptr;
@@ -167,39 +170,43 @@ int octant(uint64_t code) { return (int)(code & 7); }
 void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, const std::vector<int>& ptr,
                       const std::vector<int>& src, const std::vector<unsigned short>& dir,
                       const std::vector<double>& coax_d, int n_dist, const std::vector<DispInfo>& dinfo) {
-    if (P.m2l_kernel != 0) return;
+    if (P.m2l_kernel == 1) return;
     const int p = P.p_level[L];
-    if (p > 23) return;   // degrees above 23 use the generic kernel (coarse levels only)
+    if (p > kM2LBlockMaxP) return;   // higher degrees use the generic kernel (coarse levels only)
     const int fp32 = P.precision == HFMM_PREC_MIXED;
     const int eb = fp32 ? 4 : 8;
-    // largest block whose LDS buffer fits two workgroups per CU, else one; operators are
-    // staged in LDS only when the buffer alone does not fit (measured: the larger LDS
-    // footprint costs more occupancy than the staged operands save)
-    int tb = 0, ops = 0;
-    for (int o : {0, 1}) {
-        for (int c : {16, 8, 4})
-            if (!tb && 2 * m2l_block_lds_bytes(p, c, eb, o) <= P.max_lds) tb = c, ops = o;
-        for (int c : {16, 8, 4})
-            if (!tb && m2l_block_lds_bytes(p, c, eb, o) <= P.max_lds) tb = c, ops = o;
-    }
+    int n_units3 = 0;
+    for (int n = 0; n <= p; ++n) n_units3 += m2l_tiles(2 * n + 1);
+    const int maxu = (n_units3 + 3) / 4;
+    // block size: a requested one if it fits, else the largest whose accumulators fit the
+    // register budget and whose LDS buffer admits two workgroups per CU (else one)
+    auto fits = [&](int c, int per_cu) {
+        if (fp32 ? (c != 8 && c != 16 && c != 32) : (c != 8 && c != 16)) return false;
+        if (m2l_acc_regs((maxu + 3) / 4 * 4, c, eb) > kM2LMaxAccRegs) return false;
+        return per_cu * m2l_block_lds_bytes(p, c, eb) <= P.max_lds;
+    };
+    int tb = 0;
+    if (P.m2l_kernel > 1 && fits(P.m2l_kernel, 1)) tb = P.m2l_kernel;
+    for (int c : {32, 16, 8})
+        if (!tb && fits(c, 2)) tb = c;
+    for (int c : {32, 16, 8})
+        if (!tb && fits(c, 1)) tb = c;
     if (!tb) return;
-    const int ncol = 2 * tb, nct = (ncol + 15) / 16;
+    // units of work, costliest first (dealt round-robin to the 4 waves)
     std::vector<std::pair<int, int>> u1, u2, u3;   // (cost, code)
-    for (int n = 0; n <= p; ++n)
-        for (int ct = 0; ct < nct; ++ct) {
-            const int tiles = (2 * n + 1 + 15) / 16, ks = (2 * n + 1 + 3) / 4;
-            u1.push_back({tiles * ks, n | (ct << 16)});
-            for (int mt = 0; mt < tiles; ++mt) u3.push_back({ks, n | (mt << 8) | (ct << 16)});
-        }
+    for (int n = 0; n <= p; ++n) {
+        const int w = 2 * n + 1, nt = m2l_tiles(w), ks = (w + 3) / 4;
+        u1.push_back({nt * ks, n});
+        for (int mt = 0; mt < nt; ++mt) u3.push_back({ks, n | (mt << 8)});
+    }
     for (int m = -p; m <= p; ++m) {
         const int len = p + 1 - std::abs(m);
-        for (int ct = 0; ct < nct; ++ct) u2.push_back({2 * ((len + 15) / 16) * ((len + 3) / 4), (m + p) | (ct << 16)});
+        u2.push_back({m2l_tiles(len) * ((len + 3) / 4), m + p});
     }
     auto by_cost = [](const std::pair<int, int>& a, const std::pair<int, int>& b) { return a.first > b.first; };
     std::stable_sort(u1.begin(), u1.end(), by_cost);
     std::stable_sort(u2.begin(), u2.end(), by_cost);
     std::stable_sort(u3.begin(), u3.end(), by_cost);
-    if ((int)(u3.size() + 3) / 4 > 32) return;
     std::vector<int> units1, units2, units3;
     for (auto& x : u1) units1.push_back(x.second);
     for (auto& x : u2) units2.push_back(x.second);
@@ -249,7 +256,6 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     M2LBlockLevel& B = P.m2lb[L];
     B.use = true;
     B.tb = tb;
-    B.ops = ops;
     B.n_items = (int)items.size();
     B.items = P.upload(items);
     B.targets = P.upload(blk_targets);
@@ -261,31 +267,41 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     B.n_units2 = (int)units2.size();
     B.units3 = P.upload(units3);
     B.n_units3 = (int)units3.size();
-    B.coax_stride = coax_size(p, p);
-    // coaxial tables with every m block transposed ([n'][n])
-    std::vector<double> ct(coax_d.size());
+    // coaxial tables as A fragments: per distance, per order m = 0..p the (p+1-m)^2 block
+    // T[i][k] (row i: target degree m+i, column k: source degree m+k), padded to 16x16
+    // tiles; per (row tile, k group, lane) eight values: Re then Im of j = 0..3
+    std::vector<int> coff(p + 2, 0);
+    for (int m = 0; m <= p; ++m) coff[m + 1] = coff[m] + m2l_tiles(p + 1 - m) * m2l_tiles(p + 1 - m) * 64 * 8;
+    B.coax_stride = coff[p + 1];
+    std::vector<double> cf((size_t)n_dist * B.coax_stride, 0.0);
+    const int csz = coax_size(p, p);
     for (int di = 0; di < n_dist; ++di) {
-        const double* src_t = &coax_d[(size_t)di * B.coax_stride * 2];
-        double* dst_t = &ct[(size_t)di * B.coax_stride * 2];
+        const double* tab = &coax_d[(size_t)di * csz * 2];
+        double* dst = &cf[(size_t)di * B.coax_stride];
         int off = 0;
         for (int m = 0; m <= p; ++m) {
-            const int len = p + 1 - m;
-            for (int r = 0; r < len; ++r)
-                for (int c = 0; c < len; ++c) {
-                    dst_t[2 * (off + c * len + r)] = src_t[2 * (off + r * len + c)];
-                    dst_t[2 * (off + c * len + r) + 1] = src_t[2 * (off + r * len + c) + 1];
-                }
+            const int len = p + 1 - m, nt = m2l_tiles(len);
+            for (int rt = 0; rt < nt; ++rt)
+                for (int kg = 0; kg < nt; ++kg)
+                    for (int lane = 0; lane < 64; ++lane)
+                        for (int j = 0; j < 4; ++j) {
+                            const int i = 16 * rt + (lane & 15), k = 16 * kg + 4 * j + (lane >> 4);
+                            if (i >= len || k >= len) continue;
+                            double* f = dst + coff[m] + ((size_t)(rt * nt + kg) * 64 + lane) * 8;
+                            f[j] = tab[2 * (off + i * len + k)];
+                            f[4 + j] = tab[2 * (off + i * len + k) + 1];
+                        }
             off += len * len;
         }
     }
     if (fp32) {
-        std::vector<float2> c32((size_t)n_dist * B.coax_stride);
-        for (size_t i = 0; i < c32.size(); ++i) c32[i] = make_float2((float)ct[2 * i], (float)ct[2 * i + 1]);
+        std::vector<float> c32(cf.begin(), cf.end());
         B.coax = P.upload(c32);
     } else {
-        B.coax = P.upload(ct);
+        B.coax = P.upload(cf);
     }
-    prepare_m2l_block(p, B.n_units3, tb, fp32, ops);
+    B.coax_off = P.upload(coff);
+    prepare_m2l_block(p, B.n_units3, tb, fp32);
 }
 
 void build_tables(hfmm_plan& P) {
@@ -331,20 +347,38 @@ void build_tables(hfmm_plan& P) {
     std::vector<double> rot((size_t)thetas.size() * rsz);
     for (size_t i = 0; i < thetas.size(); ++i) rotation_table(thetas[i], pmax, &rot[i * rsz]);
     P.d_rot = P.upload(rot);
-    std::vector<double> rot_t(rot.size());
-    for (size_t i = 0; i < thetas.size(); ++i)
-        for (int n = 0; n <= pmax; ++n) {
-            const int w = 2 * n + 1;
-            const double* a = &rot[i * rsz + rot_offset(n)];
-            double* b = &rot_t[i * rsz + rot_offset(n)];
-            for (int r = 0; r < w; ++r)
-                for (int c = 0; c < w; ++c) b[c * w + r] = a[r * w + c];
+    if (P.m2l_kernel != 1) {
+        // rotation blocks n <= min(pmax, kM2LBlockMaxP) as A fragments, padded to 16x16
+        // tiles: rf1 holds A = E_n^T (forward rotation), rf3 holds A = E_n (back rotation)
+        const int pb = std::min(pmax, kM2LBlockMaxP);
+        std::vector<int> off(pb + 2, 0);
+        for (int n = 0; n <= pb; ++n) off[n + 1] = off[n] + m2l_tiles(2 * n + 1) * m2l_tiles(2 * n + 1) * 256;
+        const int stride = off[pb + 1];
+        std::vector<double> f1((size_t)thetas.size() * stride, 0.0), f3(f1.size(), 0.0);
+        for (size_t th = 0; th < thetas.size(); ++th)
+            for (int n = 0; n <= pb; ++n) {
+                const int w = 2 * n + 1, nt = m2l_tiles(w);
+                const double* E = &rot[th * rsz + rot_offset(n)];
+                for (int mt = 0; mt < nt; ++mt)
+                    for (int kg = 0; kg < nt; ++kg)
+                        for (int lane = 0; lane < 64; ++lane)
+                            for (int j = 0; j < 4; ++j) {
+                                const int i = 16 * mt + (lane & 15), k = 16 * kg + 4 * j + (lane >> 4);
+                                if (i >= w || k >= w) continue;
+                                const size_t at = th * stride + off[n] + ((size_t)(mt * nt + kg) * 64 + lane) * 4 + j;
+                                f1[at] = E[k * w + i];
+                                f3[at] = E[i * w + k];
+                            }
+            }
+        P.rf_stride = stride;
+        P.d_rf_off = P.upload(off);
+        if (P.precision == HFMM_PREC_MIXED) {
+            P.d_rf1_32 = P.upload(std::vector<float>(f1.begin(), f1.end()));
+            P.d_rf3_32 = P.upload(std::vector<float>(f3.begin(), f3.end()));
+        } else {
+            P.d_rf1 = P.upload(f1);
+            P.d_rf3 = P.upload(f3);
         }
-    P.d_rot_t = P.upload(rot_t);
-    if (P.precision == HFMM_PREC_MIXED) {
-        std::vector<float> r32(rot.begin(), rot.end()), rt32(rot_t.begin(), rot_t.end());
-        P.d_rot32 = P.upload(r32);
-        P.d_rot32_t = P.upload(rt32);
     }
     std::vector<double2> phase((size_t)kNumDirs * (pmax + 1));
     for (int d = 0; d < kNumDirs; ++d)
@@ -513,6 +547,8 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
         if (o.p_level[L] != 0 && (o.p_level[L] < 2 || o.p_level[L] > kMaxP))
             fail(HFMM_ERR_ARG, "p_level entries must be 0 or in [2, 40]");
     if (o.precision != HFMM_PREC_FP64 && o.precision != HFMM_PREC_MIXED) fail(HFMM_ERR_ARG, "unknown precision");
+    if (o.m2l_kernel != 0 && o.m2l_kernel != 1 && o.m2l_kernel != 8 && o.m2l_kernel != 16 && o.m2l_kernel != 32)
+        fail(HFMM_ERR_ARG, "m2l_kernel must be 0, 1, 8, 16 or 32");
     check_device();
     upload_legendre_coefficients();
     P.n = n;
@@ -582,12 +618,11 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
     P.d_R = P.upload(rl);
     build_p2p(P);
     build_tables(P);
-    // P2M chunk: points staged per pass, so that LDS stays within 60 KB
-    {
-        const int P1 = P.pmax + 1, tri = P1 * (P1 + 1) / 2;
-        const int per_point = 2 * P1 * (int)sizeof(double2) + tri * (int)sizeof(double);
-        P.p2m_chunk = std::max(1, std::min(32, 60 * 1024 / per_point));
-    }
+    // P2M: points staged per pass so that LDS stays within 48 KB at the largest leaf degree
+    P.p_leafmax = 2;
+    for (int b = 0; b < t.n_boxes(); ++b)
+        if (t.is_leaf[b]) P.p_leafmax = std::max(P.p_leafmax, P.p_level[t.level[b]]);
+    P.p2m_chunk = (int)std::max<size_t>(1, std::min<size_t>(64, 48 * 1024 / p2m_bytes_per_point(P.p_leafmax)));
     P.dM = P.alloc<double2>(P.n_coef);
     if (P.precision == HFMM_PREC_MIXED) P.dM32 = P.alloc<float2>(P.n_coef);
     P.dL = P.alloc<double2>(P.n_coef);
@@ -621,8 +656,10 @@ XlateArgs xargs(const hfmm_plan& P, const XlateLaunch& X, const double2* srcbuf,
     return a;
 }
 
-// The device sequence of one evaluate.  events (optional, 8 entries) bracket phases.
-void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStream_t s, hipEvent_t* ev) {
+// The device sequence of one evaluate.  ev (optional, 8 entries) brackets the phases;
+// lev (optional, 2 per level) brackets the M2L launch of every level.
+void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStream_t s, hipEvent_t* ev,
+                  hipEvent_t* lev = nullptr) {
     auto mark = [&](int i) {
         if (ev) HIPCHK(hipEventRecord(ev[i], s));
     };
@@ -633,7 +670,7 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
     mark(1);
     const DevBoxes bx = P.boxes();
     const DevLevels lv = P.levels();
-    launch_p2m(P.d_far_leaves, P.n_far_leaves, P.p2m_chunk, bx, lv, P.dX, P.dY, P.dZ, P.d_q, P.k, P.pmax, P.dM, s);
+    launch_p2m(P.d_far_leaves, P.n_far_leaves, P.p2m_chunk, bx, lv, P.dX, P.dY, P.dZ, P.d_q, P.k, P.p_leafmax, P.dM, s);
     mark(2);
     for (int L = P.depth - 1; L >= 2; --L)
         if (P.m2m[L].n_targets) launch_xlate(xargs(P, P.m2m[L], P.dM, P.dM, 0), P.m2m[L].n_targets, s);
@@ -644,7 +681,9 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
         if (!P.m2l[L].n_targets) continue;
         const M2LBlockLevel& B = P.m2lb[L];
         if (!B.use) {
+            if (lev) HIPCHK(hipEventRecord(lev[2 * L], s));
             launch_xlate(xargs(P, P.m2l[L], P.dM, P.dL, 1), P.m2l[L].n_targets, s);
+            if (lev) HIPCHK(hipEventRecord(lev[2 * L + 1], s));
             continue;
         }
         if (fp32 && !converted) {
@@ -665,18 +704,22 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
         a.n_units2 = B.n_units2;
         a.units3 = B.units3;
         a.n_units3 = B.n_units3;
-        a.rot = fp32 ? (const void*)P.d_rot32 : (const void*)P.d_rot;
-        a.rot_t = fp32 ? (const void*)P.d_rot32_t : (const void*)P.d_rot_t;
-        a.rot_stride = rot_size(P.pmax);
+        a.rot1 = fp32 ? (const void*)P.d_rf1_32 : (const void*)P.d_rf1;
+        a.rot3 = fp32 ? (const void*)P.d_rf3_32 : (const void*)P.d_rf3;
+        a.rot_off = P.d_rf_off;
+        a.rot_stride = P.rf_stride;
         a.phase = fp32 ? (const void*)P.d_phase32 : (const void*)P.d_phase;
         a.phase_stride = P.pmax + 1;
         a.dinfo = P.d_dinfo;
         a.coax = B.coax;
+        a.coax_off = B.coax_off;
         a.coax_stride = B.coax_stride;
         a.M = fp32 ? (const void*)P.dM32 : (const void*)P.dM;
         a.L = P.dL;
         a.exp_off = P.d_expoff;
-        launch_m2l_block(a, B.tb, fp32, B.ops, s);
+        if (lev) HIPCHK(hipEventRecord(lev[2 * L], s));
+        launch_m2l_block(a, B.tb, fp32, s);
+        if (lev) HIPCHK(hipEventRecord(lev[2 * L + 1], s));
     }
     mark(4);
     for (int L = 2; L < P.depth; ++L)
@@ -905,6 +948,28 @@ int hfmm_benchmark(hfmm_plan* P, const void* q_dev, int q_type, void* out_dev, i
     });
 }
 
+int hfmm_benchmark_m2l_levels(hfmm_plan* P, const void* q_dev, int q_type, void* out_dev, int reps,
+                              double* level_ms) {
+    return guarded([&] {
+        if (!P || !q_dev || !out_dev || !level_ms || reps < 1) fail(HFMM_ERR_ARG, "bad argument");
+        hipStream_t s = P->stream;
+        std::vector<hipEvent_t> ev(2 * kMaxLevels);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        for (int L = 0; L < HFMM_MAX_LEVELS; ++L) level_ms[L] = 0.0;
+        for (int r = 0; r < reps; ++r) {
+            run_sequence(*P, q_dev, q_type, (double2*)out_dev, s, nullptr, ev.data());
+            HIPCHK(hipStreamSynchronize(s));
+            for (int L = 2; L <= P->depth && L < HFMM_MAX_LEVELS; ++L) {
+                if (!P->m2l[L].n_targets) continue;
+                float f = 0;
+                HIPCHK(hipEventElapsedTime(&f, ev[2 * L], ev[2 * L + 1]));
+                level_ms[L] += f / reps;
+            }
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    });
+}
+
 int hfmm_plan_destroy(hfmm_plan* P) {
     return guarded([&] { delete P; });
 }
@@ -1065,5 +1130,191 @@ int hfmm_host_coax(int kind, double k, double t, double rs, double rt, int ps, i
 }
 
 const char* hfmm_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// Batches of independent problems: one plan per problem; an evaluate forks the
+// problems over a few streams (balanced by point count) and joins them, all inside
+// one captured hipGraph, so that small problems run side by side on the GPU.
+// ---------------------------------------------------------------------------------
+struct hfmm_batch {
+    std::vector<hfmm_plan*> plans;
+    std::vector<std::vector<int>> branch;   // problems per stream, largest first
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> join;
+    hipEvent_t fork = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<const void*> g_q;
+    std::vector<void*> g_out;
+    int g_qtype = -1;
+    ~hfmm_batch() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto e : join) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        for (auto st : streams) (void)hipStreamDestroy(st);
+        if (stream) (void)hipStreamDestroy(stream);
+        for (auto P : plans) delete P;
+    }
+};
+
+namespace {
+
+constexpr int kBatchStreams = 4;
+
+// fork on s, run every problem of a branch on its stream, join back on s
+void batch_sequence(hfmm_batch& B, const void* const* q, int q_type, void* const* out, hipStream_t s) {
+    HIPCHK(hipEventRecord(B.fork, s));
+    for (size_t b = 0; b < B.streams.size(); ++b) {
+        HIPCHK(hipStreamWaitEvent(B.streams[b], B.fork, 0));
+        for (int i : B.branch[b]) run_sequence(*B.plans[i], q[i], q_type, (double2*)out[i], B.streams[b], nullptr);
+        HIPCHK(hipEventRecord(B.join[b], B.streams[b]));
+        HIPCHK(hipStreamWaitEvent(s, B.join[b], 0));
+    }
+}
+
+void batch_evaluate_device(hfmm_batch& B, const void* const* q, int q_type, void* const* out, hipStream_t s) {
+    const size_t np = B.plans.size();
+    bool same = B.gexec && B.g_qtype == q_type;
+    for (size_t i = 0; same && i < np; ++i) same = B.g_q[i] == q[i] && B.g_out[i] == out[i];
+    if (!same) {
+        if (B.gexec) HIPCHK(hipGraphExecDestroy(B.gexec));
+        if (B.graph) HIPCHK(hipGraphDestroy(B.graph));
+        B.gexec = nullptr;
+        B.graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            batch_sequence(B, q, q_type, out, B.stream);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(B.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(B.stream, &B.graph));
+        HIPCHK(hipGraphInstantiate(&B.gexec, B.graph, nullptr, nullptr, 0));
+        B.g_q.assign(q, q + np);
+        B.g_out.assign(out, out + np);
+        B.g_qtype = q_type;
+    }
+    HIPCHK(hipGraphLaunch(B.gexec, s));
+}
+
+}  // namespace
+
+extern "C" {
+
+int hfmm_batch_create(int n_problems, const double* const* xyz, const int64_t* n, const double* k,
+                      const hfmm_opts* opts, hfmm_batch** out) {
+    return guarded([&] {
+        if (!out || !xyz || !n || !k || !opts || n_problems < 1) fail(HFMM_ERR_ARG, "bad argument");
+        *out = nullptr;
+        std::unique_ptr<hfmm_batch> B(new hfmm_batch());
+        for (int i = 0; i < n_problems; ++i) {
+            if (!xyz[i]) fail(HFMM_ERR_ARG, "null positions of problem " + std::to_string(i));
+            hfmm_plan* P = new hfmm_plan();
+            try {
+                build_plan(*P, xyz[i], n[i], k[i], opts[i]);
+            } catch (const Error& e) {
+                delete P;
+                fail(e.code, "problem " + std::to_string(i) + ": " + e.msg);
+            }
+            B->plans.push_back(P);
+        }
+        const int ns = std::min(kBatchStreams, n_problems);
+        B->branch.resize(ns);
+        std::vector<int> order(n_problems);
+        for (int i = 0; i < n_problems; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n[a] > n[b]; });
+        std::vector<int64_t> load(ns, 0);
+        for (int i : order) {
+            const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            B->branch[b].push_back(i);
+            load[b] += n[i];
+        }
+        HIPCHK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+        B->streams.resize(ns);
+        B->join.resize(ns);
+        for (int b = 0; b < ns; ++b) {
+            HIPCHK(hipStreamCreateWithFlags(&B->streams[b], hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&B->join[b], hipEventDisableTiming));
+        }
+        HIPCHK(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
+        *out = B.release();
+    });
+}
+
+int hfmm_batch_plan(hfmm_batch* B, int i, hfmm_plan** plan) {
+    return guarded([&] {
+        if (!B || !plan || i < 0 || i >= (int)B->plans.size()) fail(HFMM_ERR_ARG, "bad argument");
+        *plan = B->plans[i];
+    });
+}
+
+int hfmm_batch_evaluate(hfmm_batch* B, const void* const* q, int q_type, void* const* out, int ptr_kind,
+                        void* stream) {
+    return guarded([&] {
+        if (!B || !q || !out) fail(HFMM_ERR_ARG, "null argument");
+        if (q_type != HFMM_F64 && q_type != HFMM_C128) fail(HFMM_ERR_ARG, "unknown charge type");
+        const size_t np = B->plans.size();
+        for (size_t i = 0; i < np; ++i)
+            if (!q[i] || !out[i]) fail(HFMM_ERR_ARG, "null charges or output of problem " + std::to_string(i));
+        hipStream_t s = stream ? (hipStream_t)stream : B->stream;
+        if (ptr_kind == HFMM_HOST) {
+            std::vector<const void*> qd(np);
+            std::vector<void*> od(np);
+            for (size_t i = 0; i < np; ++i) {
+                hfmm_plan& P = *B->plans[i];
+                const size_t qbytes = (size_t)P.n * (q_type == HFMM_F64 ? 8 : 16);
+                HIPCHK(hipMemcpyAsync(P.d_qin, q[i], qbytes, hipMemcpyHostToDevice, s));
+                qd[i] = P.d_qin;
+                od[i] = P.d_out;
+            }
+            batch_evaluate_device(*B, qd.data(), q_type, od.data(), s);
+            for (size_t i = 0; i < np; ++i)
+                HIPCHK(hipMemcpyAsync(out[i], B->plans[i]->d_out, (size_t)B->plans[i]->n * 16,
+                                      hipMemcpyDeviceToHost, s));
+        } else if (ptr_kind == HFMM_DEVICE) {
+            batch_evaluate_device(*B, q, q_type, out, s);
+        } else {
+            fail(HFMM_ERR_ARG, "unknown pointer kind");
+        }
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+int hfmm_batch_benchmark(hfmm_batch* B, const void* const* q_dev, int q_type, void* const* out_dev, int warmup,
+                         int reps, double* median_ms) {
+    return guarded([&] {
+        if (!B || !q_dev || !out_dev || reps < 1 || !median_ms) fail(HFMM_ERR_ARG, "bad argument");
+        hipStream_t s = B->stream;
+        for (int i = 0; i < warmup; ++i) batch_evaluate_device(*B, q_dev, q_type, out_dev, s);
+        HIPCHK(hipStreamSynchronize(s));
+        std::vector<hipEvent_t> ev(2 * reps);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        for (int i = 0; i < reps; ++i) {
+            HIPCHK(hipEventRecord(ev[2 * i], s));
+            batch_evaluate_device(*B, q_dev, q_type, out_dev, s);
+            HIPCHK(hipEventRecord(ev[2 * i + 1], s));
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        std::vector<double> ms(reps);
+        for (int i = 0; i < reps; ++i) {
+            float f = 0;
+            HIPCHK(hipEventElapsedTime(&f, ev[2 * i], ev[2 * i + 1]));
+            ms[i] = f;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        std::sort(ms.begin(), ms.end());
+        *median_ms = ms[reps / 2];
+    });
+}
+
+int hfmm_batch_destroy(hfmm_batch* B) {
+    return guarded([&] { delete B; });
+}
 
 }  // extern "C"

@@ -18,11 +18,13 @@ level, with every other level at the table degree, stays below the level budget.
 """
 import ctypes
 import math
+import os
 
 import numpy as np
 
 from . import _lib as L
 
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "degree_mlp.npz")
 N_IN = 6
 N_PARAMS = 8833
 P_LO, P_SPAN = 6, 26
@@ -100,9 +102,31 @@ class DegreeMLP:
         np.savez_compressed(path, params=self.params)
 
     @classmethod
-    def load(cls, path):
+    def load(cls, path=None):
+        """The network saved at `path` (default: the weights shipped with the package)."""
+        path = DEFAULT_WEIGHTS if path is None else path
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"no trained degree network at {path} "
+                                    "(python -m hfmm0_amd.train_degrees writes one)")
         with np.load(path, allow_pickle=False) as f:
             return cls(params=f["params"])
+
+
+def predict_p_level(net, positions, charges, k, n_max=16, max_depth=None, margin=1.0, p_min=4, p_max=23):
+    """Per-level degrees for ``setup(p_level=...)`` (index 0 = level 1) predicted from the
+    host tree's level features.  Level 1 has no far field; it takes the level-2 degree."""
+    from .tree import build_tree
+    tree = build_tree(positions, n_max, max_depth)
+    if tree["depth"] < 2:
+        return [p_min] * max(tree["depth"], 1)
+    d = net.predict_degrees(level_features(tree, charges, k), margin=margin, p_min=p_min, p_max=p_max)
+    return [int(d[0])] + [int(v) for v in d]
+
+
+def m2l_flops(p_level, v_per_level):
+    """Algorithmic M2L flops of one evaluate: F_M2L(p_l) per V pair of level l."""
+    from .workloads import f_m2l
+    return sum(f_m2l(p_level[lv - 1]) * n for lv, n in v_per_level.items() if lv >= 2)
 
 
 def level_features(tree, charges, k):

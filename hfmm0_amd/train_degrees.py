@@ -1,10 +1,11 @@
 """Generates the learned-degree training data with the FMM itself and trains the MLP on
 the GPU.  Run on a GPU:  python -m hfmm0_amd.train_degrees --problems 200 --out DIR
 
-For every seeded problem (hfmm0_amd.workloads.mixed_problem, N log-uniform) it builds one
-plan at the table degree, labels every level by the smallest degree whose truncation
-error stays within the budget (hfmm0_amd.learned.label_levels), and records the level
-features.  The network is then trained by the HIP kernels (hfmm_mlp_train).
+For every seeded problem (hfmm0_amd.workloads.mixed_problem, N log-uniform, leaf size
+drawn from 32/64/128) it builds one plan at the table degree, labels every level by the
+smallest degree whose truncation error stays within the per-level budget
+(hfmm0_amd.learned.label_levels), and records the level features.  The network is then
+trained by the HIP kernels (hfmm_mlp_train) on 80% of the rows; the rest is held out.
 """
 import argparse
 import json
@@ -21,12 +22,13 @@ from .tree import build_tree
 from .workloads import mixed_problem
 
 
-def generate(n_problems, seed0=1000, n_range=(10_000, 40_000), k_range=(1.0, 60.0), leaf=64, p_tab=23,
-             budget=5e-7, log=None):
+def generate(n_problems, seed0=1000, n_range=(10_000, 200_000), k_range=(1.0, 60.0), leaves=(32, 64, 128),
+             p_tab=23, budget=2e-7, log=None):
     feats, labels, meta = [], [], []
     for i in range(n_problems):
         rng = np.random.default_rng(seed0 + i)
         n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        leaf = int(leaves[int(rng.integers(0, len(leaves)))])
         x, q, k, kind = mixed_problem(n, k_range, seed=seed0 + i)
         t0 = time.time()
         st = api.setup(x, k, p=p_tab, N_max=leaf)
@@ -36,9 +38,9 @@ def generate(n_problems, seed0=1000, n_range=(10_000, 40_000), k_range=(1.0, 60.
         st.close()
         feats.append(f)
         labels.append(lab)
-        meta += [(i, n, k, kind, lv) for lv in range(2, len(lab) + 2)]
+        meta += [(i, n, k, kind, leaf, lv) for lv in range(2, len(lab) + 2)]
         if log:
-            print(f"problem {i}: n={n} k={k:.1f} {kind} depth={len(lab) + 1} labels={lab.tolist()} "
+            print(f"problem {i}: n={n} k={k:.1f} {kind} leaf={leaf} depth={len(lab) + 1} labels={lab.tolist()} "
                   f"({time.time() - t0:.2f}s)", file=log, flush=True)
     return np.vstack(feats), np.concatenate(labels), meta
 
@@ -49,9 +51,10 @@ def main():
     ap.add_argument("--out", default="gpurun_out")
     ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--budget", type=float, default=2e-7)
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
-    X, p, meta = generate(args.problems, log=sys.stdout)
+    X, p, meta = generate(args.problems, budget=args.budget, log=sys.stdout)
     np.savez_compressed(os.path.join(args.out, "degree_data.npz"), X=X, p=p,
                         meta=np.array([json.dumps(m) for m in meta]))
     y = ((p - P_LO) / P_SPAN).astype(np.float32)
@@ -63,6 +66,11 @@ def main():
     print(f"train loss {hist[0]:.4f} -> {hist[-1]:.5f}; held-out degree error: mean {resid.mean():+.2f}, "
           f"under-predicted {np.mean(resid < 0):.1%}, |err|<=1 {np.mean(np.abs(resid) <= 1):.1%}", flush=True)
     net.save(os.path.join(args.out, "degree_mlp.npz"))
+    with open(os.path.join(args.out, "degree_mlp.json"), "w") as f:
+        json.dump({"problems": args.problems, "rows": int(len(X)), "train_rows": n_train, "epochs": args.epochs,
+                   "budget_per_level": args.budget, "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+                   "heldout_mean_resid": float(resid.mean()), "heldout_under": float(np.mean(resid < 0)),
+                   "heldout_within1": float(np.mean(np.abs(resid) <= 1))}, f, indent=1)
 
 
 if __name__ == "__main__":

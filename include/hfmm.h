@@ -51,7 +51,9 @@ typedef struct hfmm_opts {
     int max_depth;                  /* <= 0: automatic depth (reference rule) */
     int precision;                  /* HFMM_PREC_* */
     int use_graph;                  /* 1: replay evaluate as a captured hipGraph */
-    int m2l_kernel;                 /* 0: matrix-core block kernel where it fits, 1: generic kernel */
+    int m2l_kernel;                 /* 0: matrix-core block kernel where it fits, chosen block size;
+                                       1: generic kernel; 8, 16, 32: block kernel with that many
+                                       target boxes per workgroup where it fits */
 } hfmm_opts;
 
 typedef struct hfmm_plan hfmm_plan;
@@ -104,6 +106,13 @@ int hfmm_evaluate(hfmm_plan* plan, const void* q, int q_type, void* out, int ptr
 int hfmm_benchmark(hfmm_plan* plan, const void* q_dev, int q_type, void* out_dev, int warmup, int reps,
                    double* median_ms, double* phase_ms);
 
+/* Measurement helper: duration of the M2L launch of every level, averaged over `reps`
+ * instrumented (non-graph) evaluates, hipEvents recorded around each launch on the
+ * plan's stream.  level_ms[HFMM_MAX_LEVELS]; levels without M2L read 0.  No counterpart
+ * in the reference (its evaluate is one traced function). */
+int hfmm_benchmark_m2l_levels(hfmm_plan* plan, const void* q_dev, int q_type, void* out_dev, int reps,
+                              double* level_ms);
+
 int hfmm_plan_destroy(hfmm_plan* plan);
 
 /* FP64 direct sum on the GPU at selected targets: out[i] = phi(xyz[targets[i]]) from all
@@ -149,6 +158,26 @@ int hfmm_mlp_gradient(hfmm_mlp* mlp, const float* x, const float* y, int64_t n, 
 int hfmm_mlp_train(hfmm_mlp* mlp, const float* x, const float* y, int64_t n, int epochs, int batch, float lr,
                    int seed, float* loss_per_epoch /* epochs, may be NULL */);
 int hfmm_mlp_destroy(hfmm_mlp* mlp);
+
+/* ---------------------------------------------------------------------------------
+ * Batches of independent problems (many uneven problems, e.g. 256 problems of
+ * 2k-200k points).  No counterpart in the reference, whose setup/evaluate handle one
+ * problem; the role of its tuner driver (jaxfmm/tune.py:32-172) is to run many.
+ * One plan per problem (opts[i], k[i]); hfmm_batch_evaluate runs all problems, forked
+ * over a few streams balanced by point count and joined, captured once as one
+ * hipGraph per set of pointers and replayed.  q[i] / out[i] as in hfmm_evaluate.
+ * --------------------------------------------------------------------------------- */
+typedef struct hfmm_batch hfmm_batch;
+int hfmm_batch_create(int n_problems, const double* const* xyz, const int64_t* n, const double* k,
+                      const hfmm_opts* opts, hfmm_batch** out);
+/* the plan of problem i (owned by the batch: do not destroy) */
+int hfmm_batch_plan(hfmm_batch* batch, int i, hfmm_plan** plan);
+int hfmm_batch_evaluate(hfmm_batch* batch, const void* const* q, int q_type, void* const* out, int ptr_kind,
+                        void* stream);
+/* median milliseconds of one device-resident batch evaluate (hipEvents, graph replays) */
+int hfmm_batch_benchmark(hfmm_batch* batch, const void* const* q_dev, int q_type, void* const* out_dev,
+                         int warmup, int reps, double* median_ms);
+int hfmm_batch_destroy(hfmm_batch* batch);
 
 const char* hfmm_last_error(void);
 

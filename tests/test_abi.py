@@ -1,0 +1,46 @@
+"""The C ABI: libhfmm.so loads without a GPU and exports every function that
+include/hfmm.h declares; the ctypes binding declares exactly that set; error paths
+that need no device report through status codes and hfmm_last_error."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    with open(os.path.join(ROOT, "include", "hfmm.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hfmm_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    from hfmm0_amd import _lib as L
+    names = header_functions()
+    assert len(names) >= 30
+    lib = ctypes.CDLL(L.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(L.EXPORTED) == names
+
+
+def test_argument_errors_without_device():
+    from hfmm0_amd import _lib as L
+    lib = L.lib()
+    h = ctypes.c_void_p()
+    o = L.default_opts()
+    x = np.zeros((4, 3))
+    assert lib.hfmm_plan_create(L.ptr(x), 4, 1.0, None, ctypes.byref(h)) == L.HFMM_ERR_ARG
+    assert b"null" in lib.hfmm_last_error()
+    o.p = 1
+    assert lib.hfmm_plan_create(L.ptr(x), 4, 1.0, ctypes.byref(o), ctypes.byref(h)) == L.HFMM_ERR_ARG
+    assert b"p must be an integer >=2" in lib.hfmm_last_error()
+    o = L.default_opts()
+    o.m2l_kernel = 7
+    assert lib.hfmm_plan_create(L.ptr(x), 4, 1.0, ctypes.byref(o), ctypes.byref(h)) == L.HFMM_ERR_ARG
+    assert b"m2l_kernel" in lib.hfmm_last_error()
+    assert lib.hfmm_batch_create(0, None, None, None, None, ctypes.byref(h)) == L.HFMM_ERR_ARG
+    assert lib.hfmm_evaluate(None, None, 0, None, 0, None) == L.HFMM_ERR_ARG

@@ -1,0 +1,66 @@
+"""GPU tests of the batch API: every problem of a batch gives what its own plan gives,
+through host and device pointers and across graph replays, and meets the error of a
+single-problem evaluate against the FP64 direct sum."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems(m, seed0):
+    from hfmm0_amd.workloads import mixed_problem
+    rng = np.random.default_rng(seed0)
+    out = []
+    for i in range(m):
+        n = int(rng.integers(2_000, 30_000))
+        x, q, k, _ = mixed_problem(n, (1.0, 30.0), seed=seed0 + i)
+        out.append((x, q, k))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_matches_single_plans(precision):
+    import torch
+    import hfmm0_amd as hf
+    from hfmm0_amd.batch import Batch
+    from hfmm0_amd.learned import sampled_error
+    probs = _problems(7, 700)
+    print(f"case: batch of {len(probs)} problems, {precision}", flush=True)
+    B = Batch([x for x, _, _ in probs], [k for _, _, k in probs], p=12, N_max=48, precision=precision)
+    outs = B.evaluate([q for _, q, _ in probs])
+    for i, (x, q, k) in enumerate(probs):
+        st = hf.setup(x, k, p=12, N_max=48, precision=precision)
+        ref = hf.evaluate(st, q)
+        np.testing.assert_allclose(outs[i], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+        assert B.plan(i).p_level == st.p_level
+        assert sampled_error(x, q, k, outs[i], n_samp=200) < (1e-7 if precision == "fp64" else 2e-6)
+        st.close()
+    # device pointers, three replays of the captured graph, eager work in between
+    qd = [torch.from_numpy(q).cuda() for _, q, _ in probs]
+    od = [torch.empty(len(q), dtype=torch.complex128, device="cuda") for _, q, _ in probs]
+    for rep in range(3):
+        B.evaluate_device(qd, od)
+        torch.cuda.synchronize()
+        for i in range(len(probs)):
+            got = od[i].cpu().numpy()
+            np.testing.assert_allclose(got, outs[i], rtol=0, atol=1e-12 * np.abs(outs[i]).max())
+        _ = (qd[0] * 2).sum().item()
+    ms = B.benchmark(qd, od, warmup=1, reps=3)
+    print(f"  batch evaluate {ms:.2f} ms for {sum(B.n)} points", flush=True)
+    assert ms > 0
+    B.close()
+
+
+def test_batch_argument_errors():
+    from hfmm0_amd.batch import Batch
+    x = np.random.default_rng(0).uniform(size=(100, 3))
+    with pytest.raises(ValueError):
+        Batch([x], [1.0, 2.0])
+    with pytest.raises(ValueError):
+        Batch([x[:, :2]], [1.0])
+    B = Batch([x, x], [1.0, 2.0], p=4)
+    with pytest.raises(ValueError):
+        B.evaluate([np.ones(100)])
+    with pytest.raises(ValueError):
+        B.evaluate([np.ones(100), np.ones(99)])
+    B.close()

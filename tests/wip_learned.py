@@ -1,0 +1,67 @@
+"""GPU test of the learned truncation degree: on problems outside the training seeds, the
+shipped network's per-level degrees meet the 1e-6 target (checked against the FP64 direct
+sum) with less M2L work than the smallest fixed degree that meets it, and the certified
+evaluate falls back to the plain plan when the tuned one misses the tolerance."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+def _problem(n, k, seed):
+    from hfmm0_amd.workloads import mixed_problem
+    x, q, _, kind = mixed_problem(n, (k, k), seed=seed)
+    return x, q, kind
+
+
+def _v_per_level(st):
+    t = st.tree()
+    lv = t["level"][t["V"][:, 0]]
+    return {int(l): int((lv == l).sum()) for l in range(2, st.depth + 1)}
+
+
+@pytest.mark.parametrize("n,k,leaf,seed", [(60_000, 30.0, 64, 900_001), (120_000, 12.0, 128, 900_002)])
+def test_learned_degrees_beat_best_fixed_degree(n, k, leaf, seed):
+    import hfmm0_amd as hf
+    from hfmm0_amd.learned import DegreeMLP, m2l_flops, predict_p_level, sampled_error
+    print(f"case: learned degrees n={n} k={k} leaf={leaf} seed={seed}", flush=True)
+    x, q, kind = _problem(n, k, seed)
+    net = DegreeMLP.load()
+    pl = predict_p_level(net, x, q, k, n_max=leaf, margin=1.0)
+    tuned = hf.setup(x, k, p=max(pl), N_max=leaf, p_level=pl, precision="mixed")
+    err_t = sampled_error(x, q, k, hf.evaluate(tuned, q), n_samp=1024)
+    vpl = _v_per_level(tuned)
+    f_t = m2l_flops(tuned.p_level, vpl)
+    # the smallest fixed degree meeting the target, from one plan at degree 23
+    fixed = hf.setup(x, k, p=23, N_max=leaf)
+    best = None
+    for p in range(8, 24):
+        fixed.set_degrees([p] * fixed.depth)
+        if sampled_error(x, q, k, hf.evaluate(fixed, q), n_samp=1024) <= TOL:
+            best = p
+            break
+    assert best is not None
+    f_f = m2l_flops([best] * fixed.depth, vpl)
+    print(f"  {kind}: learned {tuned.p_level} err {err_t:.2e} flops {f_t:.3e}; "
+          f"fixed p={best} flops {f_f:.3e} (gain {f_f / f_t:.2f}x)", flush=True)
+    assert err_t <= TOL
+    assert f_t < f_f
+    tuned.close()
+    fixed.close()
+
+
+def test_certified_evaluate_falls_back_to_plain():
+    import hfmm0_amd as hf
+    from hfmm0_amd.learned import evaluate_certified
+    print("case: certified fallback, n=20000 k=20", flush=True)
+    x, q, _ = _problem(20_000, 20.0, 900_003)
+    plain = hf.setup(x, 20.0, p=20, N_max=64)
+    too_low = hf.setup(x, 20.0, p=6, N_max=64, precision="mixed")
+    phi, err, used_plain = evaluate_certified(too_low, plain, x, q, 20.0, tol=TOL, n_samp=512)
+    assert used_plain and err <= TOL
+    ref = hf.evaluate(plain, q)
+    np.testing.assert_allclose(phi, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    phi2, err2, used_plain2 = evaluate_certified(plain, too_low, x, q, 20.0, tol=TOL, n_samp=512)
+    assert not used_plain2 and err2 <= TOL
