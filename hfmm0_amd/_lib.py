@@ -21,13 +21,15 @@ HOST = 0
 DEVICE = 1
 F64 = 0
 C128 = 1
+F32 = 2
+C64 = 3
 MAX_LEVELS = 24
 
 
 class Opts(ctypes.Structure):
     _fields_ = [("p", ctypes.c_int), ("p_level", ctypes.c_int * MAX_LEVELS), ("n_max", ctypes.c_int),
                 ("max_depth", ctypes.c_int), ("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("m2l_kernel", ctypes.c_int)]
+                ("m2l_kernel", ctypes.c_int), ("tree_device", ctypes.c_int)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -56,6 +58,7 @@ def _declare(lib):
         "hfmm_plan_export_tree": (I, [vp, vp, vp, vp, vp, vp, vp, vp]),
         "hfmm_plan_export_list": (I, [vp, ctypes.c_char, vp]),
         "hfmm_evaluate": (I, [vp, vp, I, vp, I, vp]),
+        "hfmm_evaluate_typed": (I, [vp, vp, I, vp, I, I, vp]),
         "hfmm_plan_set_degrees": (I, [vp, vp]),
         "hfmm_benchmark": (I, [vp, vp, I, vp, I, I, ctypes.POINTER(D), vp]),
         "hfmm_benchmark_m2l_levels": (I, [vp, vp, I, vp, I, vp]),
@@ -67,6 +70,7 @@ def _declare(lib):
         "hfmm_batch_destroy": (I, [vp]),
         "hfmm_direct": (I, [vp, L, vp, D, vp, L, vp]),
         "hfmm_tree_create": (I, [vp, L, I, I, ctypes.POINTER(vp)]),
+        "hfmm_tree_create_device": (I, [vp, L, I, I, ctypes.POINTER(vp)]),
         "hfmm_tree_info_get": (I, [vp, ctypes.POINTER(PlanInfo)]),
         "hfmm_tree_export": (I, [vp, vp, vp, vp, vp, vp, vp, vp]),
         "hfmm_tree_export_list": (I, [vp, ctypes.c_char, vp]),
@@ -92,9 +96,9 @@ def _declare(lib):
 
 EXPORTED = [
     "hfmm_opts_default", "hfmm_plan_create", "hfmm_plan_info_get", "hfmm_plan_export_tree", "hfmm_plan_set_degrees",
-    "hfmm_plan_export_list", "hfmm_evaluate", "hfmm_benchmark", "hfmm_benchmark_m2l_levels",
+    "hfmm_plan_export_list", "hfmm_evaluate", "hfmm_evaluate_typed", "hfmm_benchmark", "hfmm_benchmark_m2l_levels",
     "hfmm_plan_destroy", "hfmm_direct",
-    "hfmm_tree_create", "hfmm_tree_info_get", "hfmm_tree_export", "hfmm_tree_export_list",
+    "hfmm_tree_create", "hfmm_tree_create_device", "hfmm_tree_info_get", "hfmm_tree_export", "hfmm_tree_export_list",
     "hfmm_tree_destroy", "hfmm_host_jhat", "hfmm_host_hhat", "hfmm_host_rotation", "hfmm_host_coax",
     "hfmm_mlp_create", "hfmm_mlp_params", "hfmm_mlp_forward", "hfmm_mlp_gradient", "hfmm_mlp_train",
     "hfmm_mlp_destroy", "hfmm_batch_create", "hfmm_batch_plan", "hfmm_batch_evaluate", "hfmm_batch_benchmark",

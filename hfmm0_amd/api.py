@@ -84,7 +84,7 @@ class HFMMState:
 
 
 def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, use_graph=True,
-          m2l_kernel="auto"):
+          m2l_kernel="auto", tree_device=False):
     """Build the octree, lists and operator tables; upload everything to the GPU.
 
     Validation order and messages follow ``HFMM.py:57-64`` and ``HFMM.py:14-16``.
@@ -93,10 +93,11 @@ def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_le
     FP32 M2L on matrix cores).  ``m2l_kernel="generic"`` forces the thread-per-coefficient
     M2L kernel instead of the matrix-core block kernel; an integer 8, 16 or 32 asks the
     block kernel for that many target boxes per workgroup (used where it fits).
+    ``tree_device=True`` builds the octree and lists on the GPU (same arrays).
     """
     x, k = _check_problem(positions, k)
     o = make_opts(p, N_max, max_depth, precision=precision, p_level=p_level, use_graph=use_graph,
-                  m2l_kernel=m2l_kernel, _k=k)
+                  m2l_kernel=m2l_kernel, tree_device=tree_device, _k=k)
     h = ctypes.c_void_p()
     L.check(L.lib().hfmm_plan_create(L.ptr(x), len(x), k, ctypes.byref(o), ctypes.byref(h)))
     return HFMMState(h, len(x), k, o)
@@ -110,7 +111,7 @@ def _check_problem(positions, k):
 
 
 def make_opts(p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, use_graph=True,
-              m2l_kernel="auto", _k=0.0):
+              m2l_kernel="auto", tree_device=False, _k=0.0):
     """Validated plan options (the C ``hfmm_opts``), in the reference's validation order
     after the positions check (``HFMM.py:57-64``, ``HFMM.py:14-16``)."""
     if p < 2 or int(p) != p or N_max < 1 or int(N_max) != N_max:
@@ -130,6 +131,7 @@ def make_opts(p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, 
     if m2l_kernel not in ("auto", "generic", 8, 16, 32):
         raise ValueError("m2l_kernel must be 'auto', 'generic', 8, 16 or 32")
     o.m2l_kernel = {"auto": 0, "generic": 1}.get(m2l_kernel, m2l_kernel)
+    o.tree_device = 1 if tree_device else 0
     if p_level is not None:
         for i, v in enumerate(p_level):
             if i + 1 >= L.MAX_LEVELS:
@@ -138,19 +140,32 @@ def make_opts(p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, 
     return o
 
 
-def evaluate(state, charges):
-    """phi at every point, complex128, input order (``HFMM.py:83-91``)."""
+def _charge_type(q):
+    """(contiguous array, C type code) for real/complex charges of 32 or 64 bits."""
+    if np.iscomplexobj(q):
+        if q.dtype == np.complex64:
+            return np.ascontiguousarray(q), L.C64
+        return np.ascontiguousarray(q, dtype=np.complex128), L.C128
+    if q.dtype == np.float32:
+        return np.ascontiguousarray(q), L.F32
+    return np.ascontiguousarray(q, dtype=np.float64), L.F64
+
+
+def evaluate(state, charges, out_dtype=np.complex128):
+    """phi at every point, input order (``HFMM.py:83-91``).  Real charges give the
+    reference semantics; complex charges are accepted too.  float32 / complex64 charges
+    are widened on the GPU.  ``out_dtype=np.complex64`` opts into a complex64 result
+    (the FP64 result rounded on the GPU)."""
     q = np.asarray(charges)
     if q.shape != (state.n,):
         raise ValueError("charges must be a real array of shape (N,)")
-    if np.iscomplexobj(q):
-        qc = np.ascontiguousarray(q, dtype=np.complex128)
-        qt = L.C128
-    else:
-        qc = np.ascontiguousarray(q, dtype=np.float64)
-        qt = L.F64
-    out = np.empty(state.n, dtype=np.complex128)
-    L.check(L.lib().hfmm_evaluate(state.handle, L.ptr(qc), qt, L.ptr(out), L.HOST, None))
+    qc, qt = _charge_type(q)
+    out_dtype = np.dtype(out_dtype)
+    if out_dtype not in (np.complex128, np.complex64):
+        raise ValueError("out_dtype must be complex128 or complex64")
+    out = np.empty(state.n, dtype=out_dtype)
+    ot = L.C128 if out_dtype == np.complex128 else L.C64
+    L.check(L.lib().hfmm_evaluate_typed(state.handle, L.ptr(qc), qt, L.ptr(out), ot, L.HOST, None))
     return out
 
 
@@ -162,9 +177,9 @@ def evaluate_device(state, q_dev, out_dev, stream=None):
         raise ValueError("charges must be a real array of shape (N,)")
     if not (q_dev.is_cuda and out_dev.is_cuda and q_dev.is_contiguous() and out_dev.is_contiguous()):
         raise ValueError("device evaluate needs contiguous GPU tensors")
-    qt = L.C128 if q_dev.is_complex() else L.F64
-    if qt == L.F64 and q_dev.dtype != torch.float64:
-        raise ValueError("real charges must be float64")
+    qt = {torch.float64: L.F64, torch.complex128: L.C128, torch.float32: L.F32, torch.complex64: L.C64}.get(q_dev.dtype)
+    if qt is None:
+        raise ValueError("charges must be float32/64 or complex64/128")
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
     L.check(L.lib().hfmm_evaluate(state.handle, ctypes.c_void_p(q_dev.data_ptr()), qt,

@@ -1,0 +1,183 @@
+"""Autotuner with a wisdom file, and a tolerance-driven planned evaluator.
+
+Role of the reference's ``jaxfmm/tune.py``: a wisdom store next to the package
+(``tune.py:32-66``), problem-class keys (``tune.py:337-342``), a probe that times
+candidates and keeps the fastest one meeting the accuracy anchor (``tune.py:73-...``),
+and ``plan`` / ``AutoEvaluator`` that certify the first evaluate against a sampled direct
+sum and climb a ladder when it misses the tolerance (``tune.py:375-516``).
+
+Here the knobs are the leaf size ``N_max``, the matrix-core M2L block size and the
+per-level degrees.  The degrees come from the learned network when its weights are
+present (``hfmm0_amd/data/degree_mlp.npz``), else from a fixed-degree ladder; every
+candidate is certified on sampled targets against the GPU FP64 direct sum.
+"""
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import api
+
+WISDOM_VERSION = 1
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def wisdom_path():
+    """The wisdom file: ``.hfmm_wisdom.json`` beside the package (the repository root)."""
+    return os.path.join(ROOT, ".hfmm_wisdom.json")
+
+
+def load_wisdom(path=None):
+    try:
+        with open(path or wisdom_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def save_wisdom(w, path=None):
+    with open(path or wisdom_path(), "w") as f:
+        json.dump(w, f, indent=1, sort_keys=True)
+
+
+def problem_key(positions, k, tol, precision, device="gfx950"):
+    """Problem class: N, aspect ratio and k * extent in powers of two, the tolerance
+    decade, the precision and the device."""
+    x = np.asarray(positions)
+    ext = x.max(0) - x.min(0)
+    nb = int(round(math.log2(max(len(x), 1))))
+    ab = int(np.clip(round(math.log2(max(ext.max() / max(ext.min(), 1e-30), 1.0))), 0, 10))
+    kb = int(round(math.log2(max(abs(float(k)) * float(ext.max()), 1.0))))
+    tb = int(math.floor(math.log10(tol)))
+    return f"v{WISDOM_VERSION}|helmholtz|N2^{nb}|A2^{ab}|kD2^{kb}|tol1e{tb}|{precision}|{device}"
+
+
+def _certify(st, positions, charges, k, n_samp, seed=0):
+    from .learned import sampled_error
+    return sampled_error(positions, charges, k, api.evaluate(st, charges), n_samp=n_samp, seed=seed)
+
+
+def _degrees(positions, charges, k, leaf, net, margin, tol, precision, n_samp, p_max=23):
+    """Certified per-level degrees for one leaf size: the network's prediction (raised by
+    one at every level until it certifies), else the smallest certified fixed degree."""
+    if net is not None:
+        from .learned import predict_p_level
+        for extra in range(4):
+            pl = predict_p_level(net, positions, charges, k, n_max=leaf, margin=margin + extra, p_max=p_max)
+            st = api.setup(positions, k, p=max(pl), N_max=leaf, p_level=pl, precision=precision)
+            err = _certify(st, positions, charges, k, n_samp)
+            if err <= tol:
+                return st, pl, err
+            st.close()
+    st = api.setup(positions, k, p=p_max, N_max=leaf, precision=precision)
+    for p in range(6, p_max + 1):
+        st.set_degrees([p] * st.depth)
+        err = _certify(st, positions, charges, k, n_samp)
+        if err <= tol:
+            st.close()
+            pl = [p] * st.depth
+            return api.setup(positions, k, p=p, N_max=leaf, precision=precision), pl, err
+    st.close()
+    return None, None, None
+
+
+def tune(positions, k, charges, tol=1e-6, precision="mixed", leaves=(32, 64, 128), blocks=("auto", 8, 16, 32),
+         net="default", margin=1.0, n_samp=512, reps=5, verbose=False):
+    """Fastest certified configuration for this problem:
+    ``{"N_max", "m2l_kernel", "p_level", "ms", "err"}``.  ``net="default"`` uses the
+    shipped degree network when present, ``None`` the fixed-degree ladder."""
+    import torch
+    if net == "default":
+        from .learned import DEFAULT_WEIGHTS, DegreeMLP
+        net = DegreeMLP.load() if os.path.exists(DEFAULT_WEIGHTS) else None
+    x = np.ascontiguousarray(positions, dtype=np.float64)
+    q_dev = torch.from_numpy(np.ascontiguousarray(charges)).cuda()
+    out_dev = torch.empty(len(x), dtype=torch.complex128, device="cuda")
+    best = None
+    for leaf in leaves:
+        st, pl, err = _degrees(x, charges, k, leaf, net, margin, tol, precision, n_samp)
+        if st is None:
+            continue
+        ms = api.benchmark(st, q_dev, out_dev, warmup=1, reps=reps)
+        st.close()
+        if verbose:
+            print(f"tune: leaf {leaf} degrees {pl} err {err:.2e} {ms:.3f} ms", flush=True)
+        if best is None or ms < best["ms"]:
+            best = {"N_max": leaf, "m2l_kernel": "auto", "p_level": [int(v) for v in pl], "ms": ms, "err": err}
+    if best is None:
+        raise RuntimeError(f"autotune: no configuration certifies tol={tol:g} (degrees up to 23)")
+    for blk in blocks:
+        if blk == "auto":
+            continue
+        st = api.setup(x, k, p=max(best["p_level"]), N_max=best["N_max"], p_level=best["p_level"],
+                       precision=precision, m2l_kernel=blk)
+        ms = api.benchmark(st, q_dev, out_dev, warmup=1, reps=reps)
+        st.close()
+        if verbose:
+            print(f"tune: m2l block {blk}: {ms:.3f} ms", flush=True)
+        if ms < best["ms"]:
+            best.update(m2l_kernel=blk, ms=ms)
+    if net is not None:
+        net.close()
+    return best
+
+
+class PlannedEvaluator:
+    """``fmm = plan(positions, k, tol=1e-6); phi = fmm(charges)``.
+
+    The first call looks the problem class up in the wisdom file (else tunes on these
+    charges and records the winner), then certifies the result on sampled targets; a
+    miss climbs one degree at every level and certifies again, as the reference's
+    AutoEvaluator climbs its (p, theta) ladder.  ``report`` keeps the history."""
+
+    def __init__(self, positions, k, tol=1e-6, precision="mixed", wisdom=True, n_samp=512, verbose=False):
+        self.x = np.ascontiguousarray(positions, dtype=np.float64)
+        self.k, self.tol, self.precision = float(k), float(tol), precision
+        self.wisdom, self.n_samp, self.verbose = wisdom, n_samp, verbose
+        self.key = problem_key(self.x, self.k, self.tol, precision)
+        self.config = load_wisdom().get(self.key) if wisdom else None
+        self.tuned = False
+        self.state = None
+        self.report = []
+
+    def _build(self):
+        c = self.config
+        self.state = api.setup(self.x, self.k, p=max(c["p_level"]), N_max=c["N_max"], p_level=c["p_level"],
+                               precision=self.precision, m2l_kernel=c["m2l_kernel"])
+
+    def __call__(self, charges):
+        if self.state is None:
+            if self.config is None:
+                t0 = time.perf_counter()
+                self.config = tune(self.x, self.k, charges, tol=self.tol, precision=self.precision,
+                                   n_samp=self.n_samp, verbose=self.verbose)
+                self.config["tune_s"] = time.perf_counter() - t0
+                self.tuned = True
+            self._build()
+            for _ in range(8):
+                err = _certify(self.state, self.x, charges, self.k, self.n_samp)
+                self.report.append((list(self.state.p_level), err))
+                if err <= self.tol:
+                    break
+                self.state.close()
+                self.config = dict(self.config, p_level=[min(p + 1, 40) for p in self.config["p_level"]])
+                self._build()
+            else:
+                raise RuntimeError(f"plan: cannot certify tol={self.tol:g}; history {self.report}")
+            if self.wisdom:
+                w = load_wisdom()
+                w[self.key] = {kk: v for kk, v in self.config.items() if kk in ("N_max", "m2l_kernel", "p_level")}
+                save_wisdom(w)
+        return api.evaluate(self.state, charges)
+
+    def close(self):
+        if self.state is not None:
+            self.state.close()
+            self.state = None
+
+
+def plan(positions, k, tol=1e-6, precision="mixed", wisdom=True, n_samp=512, verbose=False):
+    return PlannedEvaluator(positions, k, tol=tol, precision=precision, wisdom=wisdom, n_samp=n_samp,
+                            verbose=verbose)

@@ -48,6 +48,8 @@ struct Lists {
     std::vector<int> x_tgt, x_src;
 };
 Lists build_lists(const Tree& t);
+// The same tree and lists built on the current GPU (identical arrays).
+void build_tree_device(const double* xyz, int64_t n, int n_max, int max_depth, Tree& t, Lists& ls);
 
 void morton_grid(uint64_t code, int& x, int& y, int& z);
 uint64_t morton3(uint64_t x, uint64_t y, uint64_t z);

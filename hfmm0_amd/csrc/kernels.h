@@ -122,5 +122,7 @@ size_t m2l_block_lds_bytes(int p, int tb, int precision_bytes);
 void prepare_m2l_block(int p, int n_units3, int tb, int fp32);
 void launch_m2l_block(const M2LBlockArgs& a, int tb, int fp32, hipStream_t s);
 void launch_to_float(const double2* in, float2* out, int64_t n, hipStream_t s);
+// the result rounded to complex64 (output opt-in)
+void launch_to_c64(const double2* in, float2* out, int64_t n, hipStream_t s);
 
 }  // namespace hfmm

@@ -107,10 +107,16 @@ __global__ void gather_charges_kernel(const void* q, int q_type, const int* perm
     if (i >= n) return;
     const int o = perm[i];
     double2 v;
-    if (q_type == 0)
+    if (q_type == 0) {
         v = make_double2(((const double*)q)[o], 0.0);
-    else
+    } else if (q_type == 1) {
         v = ((const double2*)q)[o];
+    } else if (q_type == 2) {
+        v = make_double2((double)((const float*)q)[o], 0.0);
+    } else {
+        const float2 f = ((const float2*)q)[o];
+        v = make_double2((double)f.x, (double)f.y);
+    }
     if (conj) v.y = -v.y;
     qs[i] = v;
 }

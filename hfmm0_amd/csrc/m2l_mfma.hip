@@ -346,6 +346,11 @@ __global__ void to_float_kernel(const double2* in, float2* out, int64_t n) {
     for (; i < n; i += stride) out[i] = make_float2((float)in[i].x, (float)in[i].y);
 }
 
+__global__ void to_c64_kernel(const double2* in, float2* out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
+
 template <typename T, int TB, int MAXU>
 void launch_one(const M2LBlockArgs* a, size_t shm, hipStream_t s) {
     if (!a) {   // prepare: allow the dynamic LDS this instantiation may use
@@ -421,6 +426,13 @@ void launch_to_float(const double2* in, float2* out, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(to_float_kernel, dim3((unsigned)blocks), dim3(256), 0, s, in, out, n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) fail(-3, std::string("kernel launch failed (to_float): ") + hipGetErrorString(e));
+}
+
+void launch_to_c64(const double2* in, float2* out, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(to_c64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) fail(-3, std::string("kernel launch failed (to_c64): ") + hipGetErrorString(e));
 }
 
 }  // namespace hfmm

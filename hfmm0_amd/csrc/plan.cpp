@@ -35,6 +35,11 @@ constexpr int kDirM2M = 343;     // 8 child octants, t = c_parent - c_child
 constexpr int kDirL2L = 351;     // 8 child octants, t = c_child - c_parent
 constexpr int kNumDirs = 359;
 
+inline bool real_charges(int q_type) { return q_type == HFMM_F64 || q_type == HFMM_F32; }
+inline size_t charge_bytes(int q_type) {
+    return q_type == HFMM_C128 ? 16 : (q_type == HFMM_F32 ? 4 : 8);
+}
+
 struct M2LBlockLevel {
     bool use = false;
     int tb = 0;
@@ -123,6 +128,8 @@ struct hfmm_plan {
     const void* g_q = nullptr;
     void* g_out = nullptr;
     int g_qtype = -1;
+    int g_outtype = -1;
+    float2* d_out32 = nullptr;      // staging of complex64 output for host pointers
 
     template <typename T>
     T* alloc(size_t count) {
@@ -563,8 +570,12 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
             P.max_lds = (size_t)lds;
     }
-    P.tree = build_tree(xyz, n, o.n_max, o.max_depth);
-    P.lists = build_lists(P.tree);
+    if (o.tree_device) {
+        build_tree_device(xyz, n, o.n_max, o.max_depth, P.tree, P.lists);
+    } else {
+        P.tree = build_tree(xyz, n, o.n_max, o.max_depth);
+        P.lists = build_lists(P.tree);
+    }
     const Tree& t = P.tree;
     P.depth = t.depth;
     P.pmax = 0;
@@ -729,16 +740,27 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
     mark(6);
     launch_p2p(P.precision, P.n_leaves, P.d_leaf_box, bx, P.d_rng_ptr, P.d_rng_start, P.d_rng_pref,
                P.d_leaf_src_total, P.dX, P.dY, P.dZ, P.d_q, P.k, P.d_far, P.d_perm,
-               (P.conj ? 1 : 0) | ((P.k == 0.0 && q_type == HFMM_F64) ? 2 : 0), out, s);
+               (P.conj ? 1 : 0) | ((P.k == 0.0 && real_charges(q_type)) ? 2 : 0), out, s);
     mark(7);
 }
 
-void evaluate_device(hfmm_plan& P, const void* q, int q_type, double2* out, hipStream_t s) {
+// one evaluate into `out` of out_type (HFMM_C64: the FP64 result lands in d_out and is
+// rounded into out)
+void sequence_typed(hfmm_plan& P, const void* q, int q_type, void* out, int out_type, hipStream_t s) {
+    if (out_type == HFMM_C64) {
+        run_sequence(P, q, q_type, P.d_out, s, nullptr);
+        launch_to_c64(P.d_out, (float2*)out, P.n, s);
+    } else {
+        run_sequence(P, q, q_type, (double2*)out, s, nullptr);
+    }
+}
+
+void evaluate_device(hfmm_plan& P, const void* q, int q_type, void* out, hipStream_t s, int out_type = HFMM_C128) {
     if (!P.use_graph) {
-        run_sequence(P, q, q_type, out, s, nullptr);
+        sequence_typed(P, q, q_type, out, out_type, s);
         return;
     }
-    if (!P.gexec || P.g_q != q || P.g_out != out || P.g_qtype != q_type) {
+    if (!P.gexec || P.g_q != q || P.g_out != out || P.g_qtype != q_type || P.g_outtype != out_type) {
         if (P.gexec) {
             HIPCHK(hipGraphExecDestroy(P.gexec));
             P.gexec = nullptr;
@@ -749,7 +771,7 @@ void evaluate_device(hfmm_plan& P, const void* q, int q_type, double2* out, hipS
         }
         HIPCHK(hipStreamBeginCapture(P.stream, hipStreamCaptureModeThreadLocal));
         try {
-            run_sequence(P, q, q_type, out, P.stream, nullptr);
+            sequence_typed(P, q, q_type, out, out_type, P.stream);
         } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(P.stream, &g);
@@ -761,6 +783,7 @@ void evaluate_device(hfmm_plan& P, const void* q, int q_type, double2* out, hipS
         P.g_q = q;
         P.g_out = out;
         P.g_qtype = q_type;
+        P.g_outtype = out_type;
     }
     HIPCHK(hipGraphLaunch(P.gexec, s));
 }
@@ -887,18 +910,25 @@ int hfmm_plan_set_degrees(hfmm_plan* P, const int* p_level) {
     });
 }
 
-int hfmm_evaluate(hfmm_plan* P, const void* q, int q_type, void* out, int ptr_kind, void* stream) {
+int hfmm_evaluate_typed(hfmm_plan* P, const void* q, int q_type, void* out, int out_type, int ptr_kind,
+                        void* stream) {
     return guarded([&] {
         if (!P || !q || !out) fail(HFMM_ERR_ARG, "null argument");
-        if (q_type != HFMM_F64 && q_type != HFMM_C128) fail(HFMM_ERR_ARG, "unknown charge type");
+        if (q_type < HFMM_F64 || q_type > HFMM_C64) fail(HFMM_ERR_ARG, "unknown charge type");
+        if (out_type != HFMM_C128 && out_type != HFMM_C64) fail(HFMM_ERR_ARG, "unknown output type");
         hipStream_t s = stream ? (hipStream_t)stream : P->stream;
         if (ptr_kind == HFMM_HOST) {
-            const size_t qbytes = (size_t)P->n * (q_type == HFMM_F64 ? 8 : 16);
-            HIPCHK(hipMemcpyAsync(P->d_qin, q, qbytes, hipMemcpyHostToDevice, s));
-            evaluate_device(*P, P->d_qin, q_type, P->d_out, s);
-            HIPCHK(hipMemcpyAsync(out, P->d_out, (size_t)P->n * 16, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(P->d_qin, q, (size_t)P->n * charge_bytes(q_type), hipMemcpyHostToDevice, s));
+            if (out_type == HFMM_C64) {
+                if (!P->d_out32) P->d_out32 = P->alloc<float2>(P->n);
+                evaluate_device(*P, P->d_qin, q_type, P->d_out32, s, HFMM_C64);
+                HIPCHK(hipMemcpyAsync(out, P->d_out32, (size_t)P->n * 8, hipMemcpyDeviceToHost, s));
+            } else {
+                evaluate_device(*P, P->d_qin, q_type, P->d_out, s);
+                HIPCHK(hipMemcpyAsync(out, P->d_out, (size_t)P->n * 16, hipMemcpyDeviceToHost, s));
+            }
         } else if (ptr_kind == HFMM_DEVICE) {
-            evaluate_device(*P, q, q_type, (double2*)out, s);
+            evaluate_device(*P, q, q_type, out, s, out_type);
         } else {
             fail(HFMM_ERR_ARG, "unknown pointer kind");
         }
@@ -906,18 +936,22 @@ int hfmm_evaluate(hfmm_plan* P, const void* q, int q_type, void* out, int ptr_ki
     });
 }
 
+int hfmm_evaluate(hfmm_plan* P, const void* q, int q_type, void* out, int ptr_kind, void* stream) {
+    return hfmm_evaluate_typed(P, q, q_type, out, HFMM_C128, ptr_kind, stream);
+}
+
 int hfmm_benchmark(hfmm_plan* P, const void* q_dev, int q_type, void* out_dev, int warmup, int reps,
                    double* median_ms, double* phase_ms) {
     return guarded([&] {
         if (!P || !q_dev || !out_dev || reps < 1) fail(HFMM_ERR_ARG, "bad argument");
         hipStream_t s = P->stream;
-        for (int i = 0; i < warmup; ++i) evaluate_device(*P, q_dev, q_type, (double2*)out_dev, s);
+        for (int i = 0; i < warmup; ++i) evaluate_device(*P, q_dev, q_type, out_dev, s);
         HIPCHK(hipStreamSynchronize(s));
         std::vector<hipEvent_t> ev(2 * reps);
         for (auto& e : ev) HIPCHK(hipEventCreate(&e));
         for (int i = 0; i < reps; ++i) {
             HIPCHK(hipEventRecord(ev[2 * i], s));
-            evaluate_device(*P, q_dev, q_type, (double2*)out_dev, s);
+            evaluate_device(*P, q_dev, q_type, out_dev, s);
             HIPCHK(hipEventRecord(ev[2 * i + 1], s));
         }
         HIPCHK(hipStreamSynchronize(s));
@@ -1030,6 +1064,21 @@ int hfmm_tree_create(const double* xyz, int64_t n, int n_max, int max_depth, hfm
             throw;
         }
         *out = T;
+    });
+}
+
+int hfmm_tree_create_device(const double* xyz, int64_t n, int n_max, int max_depth, hfmm_tree** out) {
+    return guarded([&] {
+        if (!out || !xyz) fail(HFMM_ERR_ARG, "null argument");
+        *out = nullptr;
+        if (n <= 0 || n > (int64_t)1 << 30) fail(HFMM_ERR_ARG, "positions must be a nonempty finite (N,3) array");
+        for (int64_t i = 0; i < 3 * n; ++i)
+            if (!std::isfinite(xyz[i])) fail(HFMM_ERR_ARG, "positions must be a nonempty finite (N,3) array");
+        if (n_max < 1) fail(HFMM_ERR_ARG, "p must be an integer >=2; N_max must be a positive integer");
+        check_device();
+        std::unique_ptr<hfmm_tree> T(new hfmm_tree());
+        build_tree_device(xyz, n, n_max, max_depth, T->tree, T->lists);
+        *out = T.release();
     });
 }
 
@@ -1258,7 +1307,7 @@ int hfmm_batch_evaluate(hfmm_batch* B, const void* const* q, int q_type, void* c
                         void* stream) {
     return guarded([&] {
         if (!B || !q || !out) fail(HFMM_ERR_ARG, "null argument");
-        if (q_type != HFMM_F64 && q_type != HFMM_C128) fail(HFMM_ERR_ARG, "unknown charge type");
+        if (q_type < HFMM_F64 || q_type > HFMM_C64) fail(HFMM_ERR_ARG, "unknown charge type");
         const size_t np = B->plans.size();
         for (size_t i = 0; i < np; ++i)
             if (!q[i] || !out[i]) fail(HFMM_ERR_ARG, "null charges or output of problem " + std::to_string(i));
@@ -1268,8 +1317,7 @@ int hfmm_batch_evaluate(hfmm_batch* B, const void* const* q, int q_type, void* c
             std::vector<void*> od(np);
             for (size_t i = 0; i < np; ++i) {
                 hfmm_plan& P = *B->plans[i];
-                const size_t qbytes = (size_t)P.n * (q_type == HFMM_F64 ? 8 : 16);
-                HIPCHK(hipMemcpyAsync(P.d_qin, q[i], qbytes, hipMemcpyHostToDevice, s));
+                HIPCHK(hipMemcpyAsync(P.d_qin, q[i], (size_t)P.n * charge_bytes(q_type), hipMemcpyHostToDevice, s));
                 qd[i] = P.d_qin;
                 od[i] = P.d_out;
             }

@@ -31,14 +31,16 @@ def _export(handle, info, export_tree, export_list):
     return out
 
 
-def build_tree(positions, n_max=16, max_depth=None):
-    """Tree and lists as NumPy arrays (dict) from the C++ host builder."""
+def build_tree(positions, n_max=16, max_depth=None, device=False):
+    """Tree and lists as NumPy arrays (dict) from the C++ host builder, or (device=True)
+    from the GPU builder, whose arrays are identical."""
     x = np.ascontiguousarray(positions, dtype=np.float64)
     if x.ndim != 2 or x.shape[1] != 3:
         raise ValueError("positions must be a nonempty finite (N,3) array")
     lib = L.lib()
     h = ctypes.c_void_p()
-    L.check(lib.hfmm_tree_create(L.ptr(x), len(x), int(n_max), int(max_depth or 0), ctypes.byref(h)))
+    create = lib.hfmm_tree_create_device if device else lib.hfmm_tree_create
+    L.check(create(L.ptr(x), len(x), int(n_max), int(max_depth or 0), ctypes.byref(h)))
     try:
         info = L.PlanInfo()
         L.check(lib.hfmm_tree_info_get(h, ctypes.byref(info)))

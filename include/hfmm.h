@@ -41,6 +41,8 @@ extern "C" {
 /* charge / output element types */
 #define HFMM_F64 0             /* real float64 charges */
 #define HFMM_C128 1            /* complex128, interleaved (re, im) */
+#define HFMM_F32 2             /* real float32 charges (widened to FP64 on the GPU) */
+#define HFMM_C64 3             /* complex64, interleaved (re, im) */
 
 #define HFMM_MAX_LEVELS 24
 
@@ -54,6 +56,8 @@ typedef struct hfmm_opts {
     int m2l_kernel;                 /* 0: matrix-core block kernel where it fits, chosen block size;
                                        1: generic kernel; 8, 16, 32: block kernel with that many
                                        target boxes per workgroup where it fits */
+    int tree_device;                /* 1: build the octree and lists on the GPU (identical result);
+                                       0: on the host */
 } hfmm_opts;
 
 typedef struct hfmm_plan hfmm_plan;
@@ -92,11 +96,18 @@ int hfmm_plan_export_list(const hfmm_plan* plan, char which, int* pairs);
  * per p in setup, jaxfmm/kifmm/operators.py:572-641). */
 int hfmm_plan_set_degrees(hfmm_plan* plan, const int* p_level /* [HFMM_MAX_LEVELS] */);
 
-/* phi for all N points in input order.  q: N charges of q_type (HFMM_F64 or HFMM_C128);
+/* phi for all N points in input order.  q: N charges of q_type (HFMM_F64, HFMM_C128,
+ * HFMM_F32 or HFMM_C64; real charges give the reference semantics, HFMM.py:83-91);
  * out: N complex128.  q and out are both host or both device pointers (ptr_kind).
  * stream: a hipStream_t or NULL for the plan's own stream; the call returns after the
  * result is complete. */
 int hfmm_evaluate(hfmm_plan* plan, const void* q, int q_type, void* out, int ptr_kind, void* stream);
+
+/* The same with the output element type chosen: out_type HFMM_C128 (as hfmm_evaluate)
+ * or HFMM_C64 (the FP64 result rounded to complex64 on the GPU).  The reference returns
+ * complex128 only (HFMM.py:91); complex64 is an opt-in for memory-bound callers. */
+int hfmm_evaluate_typed(hfmm_plan* plan, const void* q, int q_type, void* out, int out_type, int ptr_kind,
+                        void* stream);
 
 /* Time the device part of evaluate (inputs already resident): runs `reps` evaluates
  * after `warmup`, returns per-evaluate milliseconds from hipEvents (median) and, if
@@ -126,6 +137,9 @@ int hfmm_direct(const double* xyz, int64_t n, const double* q, double k, const i
  * Export functions take the same arguments as the plan variants above. */
 typedef struct hfmm_tree hfmm_tree;
 int hfmm_tree_create(const double* xyz, int64_t n, int n_max, int max_depth, hfmm_tree** out);
+/* The same tree and lists built on the current GPU (Morton sort, coarsening and list
+ * construction in HIP kernels); the exported arrays are identical to hfmm_tree_create's. */
+int hfmm_tree_create_device(const double* xyz, int64_t n, int n_max, int max_depth, hfmm_tree** out);
 int hfmm_tree_info_get(const hfmm_tree* tree, hfmm_plan_info* info);
 int hfmm_tree_export(const hfmm_tree* tree, int* level, uint64_t* code, int* count, int* pstart, int* parent,
                      uint8_t* is_leaf, int64_t* perm);

@@ -221,3 +221,33 @@ def test_set_degrees_generic_kernel_levels(hf):
     assert D.relative_error(a, b) < 1e-13
     with pytest.raises(ValueError):
         st.set_degrees([30] * st.depth)
+
+
+@pytest.mark.parametrize("precision,tb", [("fp64", 8), ("fp64", 16), ("mixed", 8), ("mixed", 16), ("mixed", 32)])
+def test_matrix_core_block_sizes(hf, precision, tb):
+    say(f"case: mfma m2l block size {tb} {precision} N=30000 k=15 p=13 vs generic")
+    x, q = D.uniform_problem(30000)
+    a = hf.evaluate(hf.setup(x, 15.0, p=13, N_max=32, precision=precision, m2l_kernel=tb), q)
+    b = hf.evaluate(hf.setup(x, 15.0, p=13, N_max=32, precision=precision, m2l_kernel="generic"), q)
+    d = D.relative_error(a, b)
+    say(f"  rel diff {d:.3e}")
+    assert d < (1e-13 if precision == "fp64" else 3e-6)
+
+
+def test_charge_and_output_types(hf):
+    say("case: float32/complex64 charges and complex64 output N=12000 k=8 p=10")
+    x, q = D.uniform_problem(12000)
+    st = hf.setup(x, 8.0, p=10, N_max=32)
+    q32 = q.astype(np.complex64)
+    ref = hf.evaluate(st, q32.astype(np.complex128))
+    scale = np.abs(ref).max()
+    np.testing.assert_allclose(hf.evaluate(st, q32), ref, rtol=0, atol=1e-13 * scale)
+    r32 = q32.real.copy()
+    np.testing.assert_allclose(hf.evaluate(st, r32), hf.evaluate(st, r32.astype(np.float64)), rtol=0,
+                               atol=1e-13 * scale)
+    o64 = hf.evaluate(st, q32, out_dtype=np.complex64)
+    assert o64.dtype == np.complex64
+    np.testing.assert_allclose(o64, ref.astype(np.complex64), rtol=0, atol=2e-7 * scale)
+    st0 = hf.setup(x, 0.0, p=10, N_max=32)
+    assert np.all(hf.evaluate(st0, r32).imag == 0)
+    assert np.all(hf.evaluate(st0, r32, out_dtype=np.complex64).imag == 0)
