@@ -2,4 +2,5 @@
 
 ``from hfmm0_amd import setup, evaluate`` mirrors the reference's ``HFMM`` module.
 """
-from .api import HFMMState, benchmark, benchmark_m2l_levels, direct, evaluate, evaluate_device, setup  # noqa: F401
+from .api import (HFMMState, benchmark, benchmark_m2l_levels, direct, evaluate, evaluate_device,  # noqa: F401
+                  evaluate_field, setup)

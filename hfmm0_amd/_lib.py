@@ -59,6 +59,7 @@ def _declare(lib):
         "hfmm_plan_export_list": (I, [vp, ctypes.c_char, vp]),
         "hfmm_evaluate": (I, [vp, vp, I, vp, I, vp]),
         "hfmm_evaluate_typed": (I, [vp, vp, I, vp, I, I, vp]),
+        "hfmm_evaluate_field": (I, [vp, vp, I, vp, vp, I, vp]),
         "hfmm_plan_set_degrees": (I, [vp, vp]),
         "hfmm_benchmark": (I, [vp, vp, I, vp, I, I, ctypes.POINTER(D), vp]),
         "hfmm_benchmark_m2l_levels": (I, [vp, vp, I, vp, I, vp]),
@@ -96,7 +97,7 @@ def _declare(lib):
 
 EXPORTED = [
     "hfmm_opts_default", "hfmm_plan_create", "hfmm_plan_info_get", "hfmm_plan_export_tree", "hfmm_plan_set_degrees",
-    "hfmm_plan_export_list", "hfmm_evaluate", "hfmm_evaluate_typed", "hfmm_benchmark", "hfmm_benchmark_m2l_levels",
+    "hfmm_plan_export_list", "hfmm_evaluate", "hfmm_evaluate_typed", "hfmm_evaluate_field", "hfmm_benchmark", "hfmm_benchmark_m2l_levels",
     "hfmm_plan_destroy", "hfmm_direct",
     "hfmm_tree_create", "hfmm_tree_create_device", "hfmm_tree_info_get", "hfmm_tree_export", "hfmm_tree_export_list",
     "hfmm_tree_destroy", "hfmm_host_jhat", "hfmm_host_hhat", "hfmm_host_rotation", "hfmm_host_coax",

@@ -169,6 +169,20 @@ def evaluate(state, charges, out_dtype=np.complex128):
     return out
 
 
+def evaluate_field(state, charges):
+    """(phi, E) at every point, input order: phi (N,) complex128 and the field
+    E = -grad phi (N, 3) complex128 -- the Helmholtz counterpart of the reference's
+    ``kifmm.evaluate_setup(..., field=True)`` (``jaxfmm/kifmm/setup.py:229-261``)."""
+    q = np.asarray(charges)
+    if q.shape != (state.n,):
+        raise ValueError("charges must be a real array of shape (N,)")
+    qc, qt = _charge_type(q)
+    phi = np.empty(state.n, dtype=np.complex128)
+    field = np.empty((state.n, 3), dtype=np.complex128)
+    L.check(L.lib().hfmm_evaluate_field(state.handle, L.ptr(qc), qt, L.ptr(phi), L.ptr(field), L.HOST, None))
+    return phi, field
+
+
 def evaluate_device(state, q_dev, out_dev, stream=None):
     """Evaluate with device-resident torch tensors: ``q_dev`` (N,) float64 or complex128,
     ``out_dev`` (N,) complex128, both on the GPU."""

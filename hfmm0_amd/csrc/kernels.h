@@ -67,6 +67,17 @@ struct XlateArgs {
 };
 void launch_xlate(const XlateArgs& a, int n_targets, hipStream_t s);
 
+// the field E = -grad phi (3 complex values per point, sorted order for far_e, input
+// order for field) beside the potential
+void launch_l2p_field(const int* leaves, int n_items, DevBoxes bx, DevLevels lv, const double* X, const double* Y,
+                      const double* Z, const double2* L, double k, int pmax, double2* far, double2* far_e,
+                      hipStream_t s);
+void launch_p2p_field(int precision, int n_leaves, const int* leaf_box, DevBoxes bx, const int* rng_ptr,
+                      const int* rng_start, const int* rng_pref, const int* leaf_src_total, const double* X,
+                      const double* Y, const double* Z, const double2* q, double k, const double2* far,
+                      const double2* far_e, const int* perm, int out_mode, double2* out, double2* field,
+                      hipStream_t s);
+
 void launch_p2p(int precision, int n_leaves, const int* leaf_box, DevBoxes bx, const int* rng_ptr,
                 const int* rng_start, const int* rng_pref, const int* leaf_src_total, const double* X,
                 const double* Y, const double* Z, const double2* q, double k, const double2* far, const int* perm,

@@ -298,6 +298,136 @@ __global__ __launch_bounds__(64) void l2p_kernel(const int* leaves, DevBoxes bx,
 }
 
 // ---------------------------------------------------------------------------------
+// L2P with the field.  The gradient of the scaled regular basis
+// B_nm = jhat_n(kr) (r/R)^n Y_n^m stays in the basis, one degree up or down:
+//   d/dz B_nm            = az(n,m) B_{n-1,m}   + gz(n,m) B_{n+1,m}
+//   (d/dx + i d/dy) B_nm = ap(n,m) B_{n-1,m+1} + gp(n,m) B_{n+1,m+1}
+//   (d/dx - i d/dy) B_nm = am(n,m) B_{n-1,m-1} + gm(n,m) B_{n+1,m-1}
+// (the gradient formulas of j_n Y_n^m rescaled by (2n+1)!!/(kR)^n; grad_lo / grad_hi
+// give the coefficients).  A workgroup (one wave) first forms the three derivative
+// expansions (degree p+1) of the leaf's local expansion in LDS, then one thread per
+// point evaluates phi and the derivatives; E = -grad phi.
+// ---------------------------------------------------------------------------------
+// coefficient of B_{n-1,.} in derivative `which` (0: d/dz, 1: d/dx + i d/dy, 2: d/dx - i d/dy) of B_nm
+__device__ inline double grad_lo(int which, int n, int m, double R) {
+    if (n < 1) return 0.0;
+    const double num = which == 0 ? (double)(n + m) * (n - m)
+                                  : (which == 1 ? (double)(n - m - 1) * (n - m) : (double)(n + m - 1) * (n + m));
+    const double v = (2.0 * n + 1.0) / R * sqrt(num / ((2.0 * n - 1.0) * (2.0 * n + 1.0)));
+    return which == 2 ? -v : v;
+}
+// coefficient of B_{n+1,.}
+__device__ inline double grad_hi(int which, int n, int m, double k, double R) {
+    const double num = which == 0 ? (double)(n + 1 + m) * (n + 1 - m)
+                                  : (which == 1 ? (double)(n + m + 1) * (n + m + 2) : (double)(n - m + 1) * (n - m + 2));
+    const double v = k * k * R / (2.0 * n + 3.0) * sqrt(num / ((2.0 * n + 1.0) * (2.0 * n + 3.0)));
+    return which == 1 ? v : -v;
+}
+
+__global__ __launch_bounds__(64) void l2p_field_kernel(const int* leaves, DevBoxes bx, DevLevels lv,
+                                                       const double* X, const double* Y, const double* Z,
+                                                       const double2* Lexp, double k, double2* far,
+                                                       double2* far_e) {
+    extern __shared__ double smem[];
+    const int b = leaves[blockIdx.x];
+    const int L = bx.level[b];
+    const int p = lv.p[L];
+    const double R = lv.R[L];
+    const int P1 = p + 1, P2 = p + 2, nc = P1 * P1, nd = P2 * P2;
+    double2* s_L = (double2*)smem;
+    double2* s_D = s_L + nc;                 // 3 sets of nd: d/dz, d/dx + i d/dy, d/dx - i d/dy
+    double* s_rad = (double*)(s_D + 3 * nd);
+    const double2* src = Lexp + bx.exp_off[b];
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) s_L[c] = src[c];
+    __syncthreads();
+    for (int w = threadIdx.x; w < 3 * nd; w += blockDim.x) {
+        const int which = w / nd, c = w - which * nd;
+        int n, m;
+        nm_decode(c, n, m);
+        const int ms = which == 0 ? m : (which == 1 ? m - 1 : m + 1);   // order of the source terms
+        double2 d = make_double2(0.0, 0.0);
+        const int am = ms < 0 ? -ms : ms;
+        if (n + 1 <= p && am <= n + 1) {
+            const double f = grad_lo(which, n + 1, ms, R);
+            const double2 l = s_L[(n + 1) * (n + 1) + (n + 1) + ms];
+            d.x += f * l.x;
+            d.y += f * l.y;
+        }
+        if (n >= 1 && am <= n - 1) {
+            const double f = grad_hi(which, n - 1, ms, k, R);
+            const double2 l = s_L[(n - 1) * (n - 1) + (n - 1) + ms];
+            d.x += f * l.x;
+            d.y += f * l.y;
+        }
+        s_D[w] = d;
+    }
+    __syncthreads();
+    const double cx = bx.centre[3 * b], cy = bx.centre[3 * b + 1], cz = bx.centre[3 * b + 2];
+    const int ps = bx.pstart[b], cnt = bx.count[b];
+    double* rad = s_rad + threadIdx.x * P2;
+    for (int i0 = 0; i0 < cnt; i0 += blockDim.x) {
+        const int il = i0 + threadIdx.x;
+        if (il >= cnt) break;
+        const int i = ps + il;
+        const double dx = X[i] - cx, dy = Y[i] - cy, dz = Z[i] - cz;
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        const double rxy = sqrt(dx * dx + dy * dy);
+        const double ct = r > 0.0 ? dz / r : 1.0;
+        const double st = r > 0.0 ? rxy / r : 0.0;
+        const double2 eia = rxy > 0.0 ? make_double2(dx / rxy, dy / rxy) : make_double2(1.0, 0.0);
+        jhat_dev(k * r, p + 1, rad, 1);
+        const double rr = r / R;
+        double pw = 1.0;
+        for (int n = 0; n <= p + 1; ++n) {
+            rad[n] *= pw;
+            pw *= rr;
+        }
+        double2 acc[4] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0),
+                          make_double2(0.0, 0.0)};   // phi, d/dz, d/dx + i d/dy, d/dx - i d/dy
+        double pmm = 0.28209479177387814;
+        double2 eim = make_double2(1.0, 0.0);
+        for (int m = 0; m <= p + 1; ++m) {
+            if (m > 0) {
+                pmm *= -c_leg.d[m] * st;
+                eim = cmul(eim, eia);
+            }
+            const double sgn = (m & 1) ? -1.0 : 1.0;
+            double p2 = 0.0, p1 = 0.0;
+            for (int n = m; n <= p + 1; ++n) {
+                double pn;
+                if (n == m)
+                    pn = pmm;
+                else if (n == m + 1)
+                    pn = c_leg.e[m] * ct * pmm;
+                else
+                    pn = c_leg.a[n][m] * (ct * p1 - c_leg.b[n][m] * p2);
+                p2 = p1;
+                p1 = pn;
+                const double g = rad[n] * pn;
+                for (int set = 0; set < 4; ++set) {
+                    if (set == 0 && n > p) continue;
+                    const double2* C = set == 0 ? s_L : s_D + (set - 1) * nd;
+                    double2 term = cmul(C[n * n + n + m], eim);
+                    if (m > 0) {
+                        const double2 t2 = cmul(C[n * n + n - m], conjd(eim));
+                        term.x += sgn * t2.x;
+                        term.y += sgn * t2.y;
+                    }
+                    acc[set].x = fma(g, term.x, acc[set].x);
+                    acc[set].y = fma(g, term.y, acc[set].y);
+                }
+            }
+        }
+        far[i] = acc[0];
+        // d/dx = (D+ + D-)/2, d/dy = (D+ - D-)/(2i), E = -grad phi
+        const double2 sp = acc[2], sm = acc[3];
+        far_e[3 * (int64_t)i + 0] = make_double2(-0.5 * (sp.x + sm.x), -0.5 * (sp.y + sm.y));
+        far_e[3 * (int64_t)i + 1] = make_double2(-0.5 * (sp.y - sm.y), 0.5 * (sp.x - sm.x));
+        far_e[3 * (int64_t)i + 2] = make_double2(-acc[1].x, -acc[1].y);
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // Translation (M2M, M2L, L2L): rotate the source expansion so the translation axis
 // is +z, apply the coaxial operator (block diagonal in m), rotate back, accumulate.
 // One workgroup per target box; thread per coefficient; the three stages exchange
@@ -464,6 +594,17 @@ struct P2PMath<double> {
     }
 };
 
+// e^{ikr}/r and the field weight (1 - ikr) e^{ikr}/r^3, so that
+// -grad_x e^{ik|x-y|}/|x-y| = (x - y) * field weight (HFMM.py:29-37 in complex form)
+template <typename T>
+__device__ inline void field_weights(T r2, T k, T& wr, T& wi, T& fr, T& fi) {
+    P2PMath<T>::kernel(r2, k, wr, wi);
+    const T rinv2 = (T)1 / r2;
+    const T kr = k * sqrt(r2);
+    fr = (wr + kr * wi) * rinv2;
+    fi = (wi - kr * wr) * rinv2;
+}
+
 // One wave per target leaf.  Targets of a chunk of up to 64 points are spread over
 // lane groups: with T' = next power of two >= chunk size, lane l holds target l % T'
 // and scans the sources j == l / T' (mod 64/T') of each 64-point source batch staged
@@ -540,6 +681,100 @@ __global__ __launch_bounds__(64) void p2p_kernel(const int* leaf_box, DevBoxes b
             if (out_mode & 1) f.y = -f.y;   // k < 0: conjugate of the |k| result
             if (out_mode & 2) f.y = 0.0;    // k == 0 with real charges: the result is real
             out[perm[it]] = f;
+        }
+    }
+}
+
+// P2P with the field: the potential and E = -grad phi of the near field (same
+// decomposition and source ranges as p2p_kernel), added to the far-field values.
+// field[3 i + a] holds component a of point i (input order).
+template <typename T>
+__global__ __launch_bounds__(64) void p2p_field_kernel(const int* leaf_box, DevBoxes bx, const int* rng_ptr,
+                                                       const int* rng_start, const int* rng_pref,
+                                                       const int* leaf_src_total, const double* X, const double* Y,
+                                                       const double* Z, const double2* q, double k,
+                                                       const double2* far, const double2* far_e, const int* perm,
+                                                       int out_mode, double2* out, double2* field) {
+    __shared__ T sx[64], sy[64], sz[64], sqr[64], sqi[64];
+    const int leaf = blockIdx.x;
+    const int b = leaf_box[leaf];
+    const int tstart = bx.pstart[b], tcnt = bx.count[b];
+    const double cx = bx.centre[3 * b], cy = bx.centre[3 * b + 1], cz = bx.centre[3 * b + 2];
+    const int r0 = rng_ptr[leaf], r1 = rng_ptr[leaf + 1];
+    const int S = leaf_src_total[leaf];
+    const int lane = threadIdx.x;
+    const T kk = (T)k;
+    for (int t0 = 0; t0 < tcnt; t0 += 64) {
+        const int tc = min(64, tcnt - t0);
+        int Tp = 1;
+        while (Tp < tc) Tp <<= 1;
+        const int G = 64 / Tp;
+        const int ti = lane & (Tp - 1), g = lane / Tp;
+        const bool has_t = ti < tc;
+        const int it = tstart + t0 + (has_t ? ti : 0);
+        const T xt = (T)(X[it] - cx), yt = (T)(Y[it] - cy), zt = (T)(Z[it] - cz);
+        T acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // phi, Ex, Ey, Ez (re, im)
+        for (int base = 0; base < S; base += 64) {
+            const int v = base + lane;
+            if (v < S) {
+                int lo = r0, hi = r1 - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (rng_pref[mid] <= v)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                const int idx = rng_start[lo] + (v - rng_pref[lo]);
+                sx[lane] = (T)(X[idx] - cx);
+                sy[lane] = (T)(Y[idx] - cy);
+                sz[lane] = (T)(Z[idx] - cz);
+                const double2 qq = q[idx];
+                sqr[lane] = (T)qq.x;
+                sqi[lane] = (T)qq.y;
+            }
+            __syncthreads();
+            const int nb = min(64, S - base);
+            for (int j = g; j < nb; j += G) {
+                const T dx = xt - sx[j], dy = yt - sy[j], dz = zt - sz[j];
+                const T r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > (T)0) {
+                    T wr, wi, fr, fi;
+                    field_weights<T>(r2, kk, wr, wi, fr, fi);
+                    const T qr = sqr[j], qi = sqi[j];
+                    acc[0] += wr * qr - wi * qi;
+                    acc[1] += wr * qi + wi * qr;
+                    const T gr = fr * qr - fi * qi, gi = fr * qi + fi * qr;
+                    acc[2] += dx * gr;
+                    acc[3] += dx * gi;
+                    acc[4] += dy * gr;
+                    acc[5] += dy * gi;
+                    acc[6] += dz * gr;
+                    acc[7] += dz * gi;
+                }
+            }
+            __syncthreads();
+        }
+        for (int off = Tp; off < 64; off <<= 1)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) acc[c] += __shfl_xor(acc[c], off);
+        if (g == 0 && has_t) {
+            const int o = perm[it];
+            double2 f = far[it];
+            f.x += (double)acc[0];
+            f.y += (double)acc[1];
+            if (out_mode & 1) f.y = -f.y;   // k < 0: conjugate of the |k| result
+            if (out_mode & 2) f.y = 0.0;    // k == 0 with real charges: the result is real
+            out[o] = f;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                double2 e = far_e[3 * (int64_t)it + a];
+                e.x += (double)acc[2 + 2 * a];
+                e.y += (double)acc[3 + 2 * a];
+                if (out_mode & 1) e.y = -e.y;
+                if (out_mode & 2) e.y = 0.0;
+                field[3 * (int64_t)o + a] = e;
+            }
         }
     }
 }
@@ -653,6 +888,44 @@ void launch_l2p(const int* leaves, int n_items, DevBoxes bx, DevLevels lv, const
     const size_t shm = (size_t)P1 * P1 * sizeof(double2) + 64 * P1 * sizeof(double);
     hipLaunchKernelGGL(l2p_kernel, dim3(n_items), dim3(64), shm, s, leaves, bx, lv, X, Y, Z, L, k, far);
     check_launch("l2p");
+}
+
+size_t l2p_field_lds_bytes(int p) {
+    const size_t P1 = p + 1, P2 = p + 2;
+    return (P1 * P1 + 3 * P2 * P2) * sizeof(double2) + 64 * P2 * sizeof(double);
+}
+
+void launch_l2p_field(const int* leaves, int n_items, DevBoxes bx, DevLevels lv, const double* X, const double* Y,
+                      const double* Z, const double2* L, double k, int pmax, double2* far, double2* far_e,
+                      hipStream_t s) {
+    if (n_items <= 0) return;
+    upload_legendre_coefficients();
+    if (pmax + 1 > kMaxP) fail(-2, "the field needs leaf degrees of at most 39");
+    const size_t shm = l2p_field_lds_bytes(pmax);
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void*)l2p_field_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)l2p_field_lds_bytes(kMaxP - 1));
+        if (e != hipSuccess) fail(-3, std::string("hipFuncSetAttribute(l2p_field) failed: ") + hipGetErrorString(e));
+        attr = true;
+    }
+    hipLaunchKernelGGL(l2p_field_kernel, dim3(n_items), dim3(64), shm, s, leaves, bx, lv, X, Y, Z, L, k, far, far_e);
+    check_launch("l2p_field");
+}
+
+void launch_p2p_field(int precision, int n_leaves, const int* leaf_box, DevBoxes bx, const int* rng_ptr,
+                      const int* rng_start, const int* rng_pref, const int* leaf_src_total, const double* X,
+                      const double* Y, const double* Z, const double2* q, double k, const double2* far,
+                      const double2* far_e, const int* perm, int out_mode, double2* out, double2* field,
+                      hipStream_t s) {
+    if (n_leaves <= 0) return;
+    if (precision == 0)
+        hipLaunchKernelGGL(p2p_field_kernel<double>, dim3(n_leaves), dim3(64), 0, s, leaf_box, bx, rng_ptr, rng_start,
+                           rng_pref, leaf_src_total, X, Y, Z, q, k, far, far_e, perm, out_mode, out, field);
+    else
+        hipLaunchKernelGGL(p2p_field_kernel<float>, dim3(n_leaves), dim3(64), 0, s, leaf_box, bx, rng_ptr, rng_start,
+                           rng_pref, leaf_src_total, X, Y, Z, q, k, far, far_e, perm, out_mode, out, field);
+    check_launch("p2p_field");
 }
 
 void launch_xlate(const XlateArgs& a, int n_targets, hipStream_t s) {

@@ -130,6 +130,8 @@ struct hfmm_plan {
     int g_qtype = -1;
     int g_outtype = -1;
     float2* d_out32 = nullptr;      // staging of complex64 output for host pointers
+    double2* d_far_e = nullptr;     // far field E (sorted order), allocated at the first field call
+    double2* d_field_out = nullptr; // staging of the field for host pointers
 
     template <typename T>
     T* alloc(size_t count) {
@@ -669,8 +671,9 @@ XlateArgs xargs(const hfmm_plan& P, const XlateLaunch& X, const double2* srcbuf,
 
 // The device sequence of one evaluate.  ev (optional, 8 entries) brackets the phases;
 // lev (optional, 2 per level) brackets the M2L launch of every level.
+// field (optional): E = -grad phi, 3 complex values per point in input order.
 void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStream_t s, hipEvent_t* ev,
-                  hipEvent_t* lev = nullptr) {
+                  hipEvent_t* lev = nullptr, double2* field = nullptr) {
     auto mark = [&](int i) {
         if (ev) HIPCHK(hipEventRecord(ev[i], s));
     };
@@ -736,11 +739,21 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
     for (int L = 2; L < P.depth; ++L)
         if (P.l2l[L].n_targets) launch_xlate(xargs(P, P.l2l[L], P.dL, P.dL, 1), P.l2l[L].n_targets, s);
     mark(5);
-    launch_l2p(P.d_far_leaves, P.n_far_leaves, bx, lv, P.dX, P.dY, P.dZ, P.dL, P.k, P.pmax, P.d_far, s);
-    mark(6);
-    launch_p2p(P.precision, P.n_leaves, P.d_leaf_box, bx, P.d_rng_ptr, P.d_rng_start, P.d_rng_pref,
-               P.d_leaf_src_total, P.dX, P.dY, P.dZ, P.d_q, P.k, P.d_far, P.d_perm,
-               (P.conj ? 1 : 0) | ((P.k == 0.0 && real_charges(q_type)) ? 2 : 0), out, s);
+    const int out_mode = (P.conj ? 1 : 0) | ((P.k == 0.0 && real_charges(q_type)) ? 2 : 0);
+    if (field) {
+        launch_fill_zero(P.d_far_e, 3 * P.n, s);
+        launch_l2p_field(P.d_far_leaves, P.n_far_leaves, bx, lv, P.dX, P.dY, P.dZ, P.dL, P.k, P.p_leafmax, P.d_far,
+                         P.d_far_e, s);
+        mark(6);
+        launch_p2p_field(P.precision, P.n_leaves, P.d_leaf_box, bx, P.d_rng_ptr, P.d_rng_start, P.d_rng_pref,
+                         P.d_leaf_src_total, P.dX, P.dY, P.dZ, P.d_q, P.k, P.d_far, P.d_far_e, P.d_perm, out_mode,
+                         out, field, s);
+    } else {
+        launch_l2p(P.d_far_leaves, P.n_far_leaves, bx, lv, P.dX, P.dY, P.dZ, P.dL, P.k, P.pmax, P.d_far, s);
+        mark(6);
+        launch_p2p(P.precision, P.n_leaves, P.d_leaf_box, bx, P.d_rng_ptr, P.d_rng_start, P.d_rng_pref,
+                   P.d_leaf_src_total, P.dX, P.dY, P.dZ, P.d_q, P.k, P.d_far, P.d_perm, out_mode, out, s);
+    }
     mark(7);
 }
 
@@ -929,6 +942,29 @@ int hfmm_evaluate_typed(hfmm_plan* P, const void* q, int q_type, void* out, int 
             }
         } else if (ptr_kind == HFMM_DEVICE) {
             evaluate_device(*P, q, q_type, out, s, out_type);
+        } else {
+            fail(HFMM_ERR_ARG, "unknown pointer kind");
+        }
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
+int hfmm_evaluate_field(hfmm_plan* P, const void* q, int q_type, void* phi, void* field, int ptr_kind,
+                        void* stream) {
+    return guarded([&] {
+        if (!P || !q || !phi || !field) fail(HFMM_ERR_ARG, "null argument");
+        if (q_type < HFMM_F64 || q_type > HFMM_C64) fail(HFMM_ERR_ARG, "unknown charge type");
+        if (P->p_leafmax + 1 > kMaxP) fail(HFMM_ERR_ARG, "the field needs leaf degrees of at most 39");
+        hipStream_t s = stream ? (hipStream_t)stream : P->stream;
+        if (!P->d_far_e) P->d_far_e = P->alloc<double2>(3 * P->n);
+        if (ptr_kind == HFMM_HOST) {
+            if (!P->d_field_out) P->d_field_out = P->alloc<double2>(3 * P->n);
+            HIPCHK(hipMemcpyAsync(P->d_qin, q, (size_t)P->n * charge_bytes(q_type), hipMemcpyHostToDevice, s));
+            run_sequence(*P, P->d_qin, q_type, P->d_out, s, nullptr, nullptr, P->d_field_out);
+            HIPCHK(hipMemcpyAsync(phi, P->d_out, (size_t)P->n * 16, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(field, P->d_field_out, (size_t)P->n * 48, hipMemcpyDeviceToHost, s));
+        } else if (ptr_kind == HFMM_DEVICE) {
+            run_sequence(*P, q, q_type, (double2*)phi, s, nullptr, nullptr, (double2*)field);
         } else {
             fail(HFMM_ERR_ARG, "unknown pointer kind");
         }

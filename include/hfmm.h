@@ -109,6 +109,14 @@ int hfmm_evaluate(hfmm_plan* plan, const void* q, int q_type, void* out, int ptr
 int hfmm_evaluate_typed(hfmm_plan* plan, const void* q, int q_type, void* out, int out_type, int ptr_kind,
                         void* stream);
 
+/* phi and the field E = -grad phi for all N points (input order): phi N complex128,
+ * field N x 3 complex128 (x, y, z per point).  Replaces kifmm.evaluate_setup(...,
+ * field=True) (jaxfmm/kifmm/setup.py:229-261; near-field weight HFMM.py:29-37) for the
+ * Helmholtz kernel, which HFMM.evaluate does not expose.  Leaf degrees up to 39.  Runs
+ * eagerly on the stream (not graph-replayed). */
+int hfmm_evaluate_field(hfmm_plan* plan, const void* q, int q_type, void* phi, void* field, int ptr_kind,
+                        void* stream);
+
 /* Time the device part of evaluate (inputs already resident): runs `reps` evaluates
  * after `warmup`, returns per-evaluate milliseconds from hipEvents (median) and, if
  * phase_ms is non-NULL, per-phase milliseconds averaged over `reps` further

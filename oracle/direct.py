@@ -7,13 +7,17 @@ leg may import this module, and only as the checker or the timed CPU baseline.
   ``exp(ikr)/r`` with the r = 0 term excluded, no 1/(4 pi)).
 * :func:`direct_c` is the same sum in C with OpenMP (``oracle/direct.c``), used for
   larger sample sets.
+* :func:`direct_field_numpy` / :func:`direct_field_c`: the field E = -grad phi of the
+  same sum, with the reference kernel bundle's field weight (``HFMM.py:29-37``:
+  (cos kr + kr sin kr)/r^3 and (sin kr - kr cos kr)/r^3 times (x_t - x_s)).
 * :func:`relative_error` restates ``test.py:20-22``.
 * :func:`sample_targets` is the sampled-target set: ``default_rng(0).choice(N, n,
   replace=False)`` (seed as ``jaxfmm/tune.py:90``).
 
 Pinned by ``tests/test_oracle_direct.py``: closed-form two- and three-point values,
 the k=0 Coulomb sum of a regular polygon, symmetry and linearity identities, and
-agreement between the NumPy and C implementations.
+agreement between the NumPy and C implementations; the field by central differences
+of the potential, the k=0 Coulomb field in closed form and Newton's third law.
 """
 import ctypes
 import os
@@ -39,6 +43,23 @@ def direct_numpy(positions, charges, k, targets=None):
     return result
 
 
+def direct_field_numpy(positions, charges, k, targets=None):
+    """E = -grad phi at ``positions[targets]`` from all sources, (n, 3) complex128."""
+    positions = np.asarray(positions, dtype=np.float64)
+    charges = np.asarray(charges).astype(np.complex128)
+    tgt = positions if targets is None else positions[np.asarray(targets)]
+    result = np.empty((len(tgt), 3), dtype=np.complex128)
+    for start in range(0, len(tgt), 256):
+        x = tgt[start:start + 256]
+        d = x[:, None, :] - positions[None, :, :]
+        r = np.linalg.norm(d, axis=-1)
+        ok = r > 0
+        rs = np.where(ok, r, 1.0)
+        w = np.where(ok, (1 - 1j * k * rs) * np.exp(1j * k * rs) / rs ** 3, 0.0)
+        result[start:start + len(x)] = np.einsum("tsa,ts,s->ta", d, w, charges)
+    return result
+
+
 def build(force=False):
     """Compile ``liboracle_direct.so`` with the committed Makefile."""
     so = os.path.join(_HERE, "liboracle_direct.so")
@@ -57,6 +78,8 @@ def _lib():
         lib.oracle_direct.restype = None
         lib.oracle_direct.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_double,
                                       ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+        lib.oracle_direct_field.restype = None
+        lib.oracle_direct_field.argtypes = lib.oracle_direct.argtypes
         _LIB = lib
     return _LIB
 
@@ -69,6 +92,17 @@ def direct_c(positions, charges, k, targets=None):
     out = np.empty(len(tgt), dtype=np.complex128)
     _lib().oracle_direct(positions.ctypes.data, len(positions), q.ctypes.data, float(k),
                          tgt.ctypes.data, len(tgt), out.ctypes.data)
+    return out
+
+
+def direct_field_c(positions, charges, k, targets=None):
+    """Same field as :func:`direct_field_numpy`, in C/OpenMP."""
+    positions = np.ascontiguousarray(positions, dtype=np.float64)
+    q = np.ascontiguousarray(np.asarray(charges).astype(np.complex128))
+    tgt = positions if targets is None else np.ascontiguousarray(positions[np.asarray(targets)])
+    out = np.empty((len(tgt), 3), dtype=np.complex128)
+    _lib().oracle_direct_field(positions.ctypes.data, len(positions), q.ctypes.data, float(k),
+                               tgt.ctypes.data, len(tgt), out.ctypes.data)
     return out
 
 

@@ -3,6 +3,7 @@ import numpy as np
 import pytest
 
 from oracle import direct
+from oracle import direct as D
 
 
 def test_two_points_closed_form():
@@ -75,3 +76,38 @@ def test_negative_k_is_conjugate():
     a = direct.direct_c(x, q, 5.0)
     b = direct.direct_c(x, np.conj(q), -5.0)
     np.testing.assert_allclose(b, np.conj(a), rtol=1e-13, atol=1e-13)
+
+
+def test_field_matches_central_differences_of_phi():
+    rng = np.random.default_rng(11)
+    x = rng.uniform(size=(60, 3))
+    q = rng.normal(size=60) + 1j * rng.normal(size=60)
+    k, h = 7.0, 1e-6
+    E = D.direct_field_numpy(x, q, k, np.arange(5))
+    for t in range(5):
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = h
+            tp, tm = x[t] + e, x[t] - e
+
+            def phi_at(p):
+                r = np.linalg.norm(p[None, :] - np.delete(x, t, 0), axis=1)
+                return np.sum(np.delete(q, t) * np.exp(1j * k * r) / r)
+            fd = -(phi_at(tp) - phi_at(tm)) / (2 * h)
+            assert abs(E[t, a] - fd) <= 1e-6 * abs(fd) + 1e-8
+
+
+def test_field_coulomb_closed_form_and_newton_third_law():
+    x = np.array([[0.0, 0.0, 0.0], [0.3, -0.4, 1.2]])
+    q = np.array([2.0, -0.5])
+    E = D.direct_field_numpy(x, q, 0.0)
+    d = x[0] - x[1]
+    r = np.linalg.norm(d)
+    np.testing.assert_allclose(E[0], q[1] * d / r ** 3, rtol=1e-14)
+    # forces q_t E_t on the two points are equal and opposite (real kernel at k = 0)
+    np.testing.assert_allclose(q[0] * E[0], -q[1] * E[1], rtol=1e-14)
+    rng = np.random.default_rng(12)
+    y = rng.uniform(size=(300, 3))
+    qq = rng.normal(size=300) + 1j * rng.normal(size=300)
+    np.testing.assert_allclose(D.direct_field_c(y, qq, 5.0, np.arange(40)),
+                               D.direct_field_numpy(y, qq, 5.0, np.arange(40)), rtol=1e-12, atol=1e-12)
