@@ -22,6 +22,7 @@ sys.path.insert(0, ROOT)
 PLAIN_P = 17                         # smallest fixed degree of the plain FP64 FMM with rel err <= 1e-6
 TUNED_P = "learned"                  # per-level degrees of the tuned form: the shipped network's
                                      # prediction, or a list for levels 2, 3, ... (last one repeats)
+FALLBACK_P = "19,15,12,11"           # the list used when no trained network is shipped
 TOL = 1e-6
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
 
@@ -197,8 +198,13 @@ def main():
     # depth of the tree (auto rule) to expand the per-level degree list
     from hfmm0_amd.tree import build_tree
     depth = build_tree(x, args.leaf)["depth"]
+    from hfmm0_amd.learned import DEFAULT_WEIGHTS
+    if args.tuned_p == "learned" and not os.path.exists(DEFAULT_WEIGHTS):
+        print(f"bench: no trained degree network at {DEFAULT_WEIGHTS}; using the per-level list "
+              f"{FALLBACK_P}", file=sys.stderr)
+        args.tuned_p = FALLBACK_P
     if args.tuned_p == "learned":
-        from hfmm0_amd.learned import DEFAULT_WEIGHTS, DegreeMLP, predict_p_level
+        from hfmm0_amd.learned import DegreeMLP, predict_p_level
         t0 = time.perf_counter()
         net = DegreeMLP.load()
         tuned_levels = predict_p_level(net, x, q, args.k, n_max=args.leaf, margin=args.margin)

@@ -97,9 +97,10 @@ __host__ __device__ constexpr int m2l_tiles(int len) { return (len + 15) / 16; }
 __host__ __device__ constexpr int m2l_acc_regs(int maxu, int tb, int elem_bytes) {
     return maxu * (2 * tb / 16) * 4 * (elem_bytes / 4);
 }
-// LDS elements of the expansion block ((p+1)^2 rows x 2 TB + 1), padded to 16 bytes
+// LDS elements of the expansion block ((p+1)^2 rows x 2 TB + 1, plus one scratch row
+// that receives the stores of rows outside a block), padded to 16 bytes
 __host__ __device__ constexpr int m2l_lds_x_elems(int p, int tb) {
-    return ((p + 1) * (p + 1) * (2 * tb + 1) + 3) & ~3;
+    return (((p + 1) * (p + 1) + 1) * (2 * tb + 1) + 3) & ~3;
 }
 struct M2LBlockArgs {
     int p;                           // effective degree of this level (<= p_tab)

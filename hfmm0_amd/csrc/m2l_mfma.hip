@@ -25,7 +25,7 @@
 
 namespace hfmm {
 
-// LDS: the expansion block, then p+1 complex phases
+// LDS: the expansion block (with its scratch row), then p+1 complex phases
 size_t m2l_block_lds_bytes(int p, int tb, int precision_bytes) {
     return ((size_t)m2l_lds_x_elems(p, tb) + 2 * (size_t)(p + 1)) * precision_bytes;
 }
@@ -89,12 +89,15 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     constexpr int NCOL = 2 * TB;            // (pair, re/im) columns
     constexpr int LDX = NCOL + 1;           // odd row stride in LDS
     constexpr int NCT = NCOL / 16;          // 16-wide column tiles
-    constexpr int NG = 16;                  // gathered values per thread and batch
+    constexpr int NG = 8;                   // gathered values per thread and batch
     extern __shared__ char smem_raw[];
     const int p = a.p, P1 = p + 1, nc = P1 * P1;
+    const int spare = (a.p_tab + 1) * (a.p_tab + 1);   // scratch row for out-of-block stores
     T* X = (T*)smem_raw;
     C2* Ps = (C2*)(X + m2l_lds_x_elems(a.p_tab, TB));
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index is uniform across a wavefront: keep it (and all unit bookkeeping
+    // derived from it) in scalar registers so that the unit branches are scalar
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int li = lane & 15, lk = lane >> 4;
     const int4 item = a.items[blockIdx.x];  // (block, e0, e1, atomic)
     const int* tg = a.blk_targets + (size_t)item.x * TB;
@@ -188,9 +191,9 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int row = mt * 16 + Mma<T>::row(lk, r);
-                        if (row < w)
+                        const int dst = (row < w ? base + row : spare) * LDX;
 #pragma unroll
-                            for (int ct = 0; ct < NCT; ++ct) X[(base + row) * LDX + ct * 16 + li] = c4[mt][ct][r];
+                        for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[mt][ct][r];
                     }
         }
         __syncthreads();
@@ -253,11 +256,10 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int row = t * 16 + Mma<T>::row(lk, r);
-                        if (row < len) {
-                            const int nn = am + row;
+                        const int nn = am + row;
+                        const int dst = (row < len ? nn * nn + nn + m : spare) * LDX;
 #pragma unroll
-                            for (int ct = 0; ct < NCT; ++ct) X[(nn * nn + nn + m) * LDX + ct * 16 + li] = c4[t][ct][r];
-                        }
+                        for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[t][ct][r];
                     }
         }
         __syncthreads();
