@@ -188,18 +188,19 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     for (int n = 0; n <= p; ++n) n_units3 += m2l_tiles(2 * n + 1);
     const int maxu = (n_units3 + 3) / 4;
     // block size: a requested one if it fits, else the largest whose accumulators fit the
-    // register budget and whose LDS buffer admits two workgroups per CU (else one)
-    auto fits = [&](int c, int per_cu) {
+    // register budget and whose LDS buffer fits a CU, with 32 only at levels of at least
+    // 16384 boxes (measured at N = 1M: 32 wins at the finest level, 16 at coarser ones,
+    // where fewer, longer work items amortise the per-item accumulator set-up better)
+    auto fits = [&](int c) {
         if (fp32 ? (c != 8 && c != 16 && c != 32) : (c != 8 && c != 16)) return false;
         if (m2l_acc_regs((maxu + 3) / 4 * 4, c, eb) > kM2LMaxAccRegs) return false;
-        return per_cu * m2l_block_lds_bytes(p, c, eb) <= P.max_lds;
+        return m2l_block_lds_bytes(p, c, eb) <= P.max_lds;
     };
+    const int n_level = (int)targets.size();
     int tb = 0;
-    if (P.m2l_kernel > 1 && fits(P.m2l_kernel, 1)) tb = P.m2l_kernel;
+    if (P.m2l_kernel > 1 && fits(P.m2l_kernel)) tb = P.m2l_kernel;
     for (int c : {32, 16, 8})
-        if (!tb && fits(c, 2)) tb = c;
-    for (int c : {32, 16, 8})
-        if (!tb && fits(c, 1)) tb = c;
+        if (!tb && fits(c) && (c < 32 || n_level >= 16384)) tb = c;
     if (!tb) return;
     // units of work, costliest first (dealt round-robin to the 4 waves)
     std::vector<std::pair<int, int>> u1, u2, u3;   // (cost, code)
