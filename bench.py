@@ -159,7 +159,7 @@ def main():
     ap.add_argument("--leaf", type=int, default=128)
     ap.add_argument("--plain-p", type=int, default=PLAIN_P)
     ap.add_argument("--tuned-p", type=str, default=TUNED_P)
-    ap.add_argument("--margin", type=float, default=1.0, help="degrees added to the network's prediction")
+    ap.add_argument("--margin", type=float, default=0.0, help="degrees added to the network's prediction")
     ap.add_argument("--tuned-precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--samples", type=int, default=4096)
     ap.add_argument("--no-plain", action="store_true")

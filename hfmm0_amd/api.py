@@ -91,8 +91,9 @@ def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_le
     ``p_level`` optionally gives the truncation degree per tree level (index 0 is
     level 1); ``precision`` is ``"fp64"`` (plain FMM) or ``"mixed"`` (FP32 near field and
     FP32 M2L on matrix cores).  ``m2l_kernel="generic"`` forces the thread-per-coefficient
-    M2L kernel instead of the matrix-core block kernel; an integer 8, 16 or 32 asks the
-    block kernel for that many target boxes per workgroup (used where it fits).
+    M2L kernel instead of the matrix-core block kernel; an integer 8, 16, 32 or 64 asks the
+    block kernel for that many pairs per operator application (target boxes times
+    directions of equal theta and |d|; used where it fits).
     ``tree_device=True`` builds the octree and lists on the GPU (same arrays).
     """
     x, k = _check_problem(positions, k)
@@ -128,8 +129,8 @@ def make_opts(p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, 
     o.max_depth = int(max_depth) if max_depth is not None else 0
     o.precision = PRECISIONS[precision]
     o.use_graph = 1 if use_graph else 0
-    if m2l_kernel not in ("auto", "generic", 8, 16, 32):
-        raise ValueError("m2l_kernel must be 'auto', 'generic', 8, 16 or 32")
+    if m2l_kernel not in ("auto", "generic", 8, 16, 32, 64):
+        raise ValueError("m2l_kernel must be 'auto', 'generic', 8, 16, 32 or 64")
     o.m2l_kernel = {"auto": 0, "generic": 1}.get(m2l_kernel, m2l_kernel)
     o.tree_device = 1 if tree_device else 0
     if p_level is not None:

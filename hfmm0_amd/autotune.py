@@ -86,8 +86,10 @@ def _degrees(positions, charges, k, leaf, net, margin, tol, precision, n_samp, p
 def tune(positions, k, charges, tol=1e-6, precision="mixed", leaves=(32, 64, 128), blocks=("auto", 8, 16, 32),
          net="default", margin=1.0, n_samp=512, reps=5, verbose=False):
     """Fastest certified configuration for this problem:
-    ``{"N_max", "m2l_kernel", "p_level", "ms", "err"}``.  ``net="default"`` uses the
-    shipped degree network when present, ``None`` the fixed-degree ladder."""
+    ``{"N_max", "m2l_kernel", "p_level", "precision", "ms", "err"}``.  ``net="default"``
+    uses the shipped degree network when present, ``None`` the fixed-degree ladder.  When
+    no mixed-precision configuration certifies (the FP32 floor of the problem lies above
+    the tolerance), the search is repeated in FP64."""
     import torch
     if net == "default":
         from .learned import DEFAULT_WEIGHTS, DegreeMLP
@@ -107,7 +109,11 @@ def tune(positions, k, charges, tol=1e-6, precision="mixed", leaves=(32, 64, 128
         if best is None or ms < best["ms"]:
             best = {"N_max": leaf, "m2l_kernel": "auto", "p_level": [int(v) for v in pl], "ms": ms, "err": err}
     if best is None:
+        if precision == "mixed":   # below the FP32 floor of this problem: certify in FP64
+            return tune(positions, k, charges, tol=tol, precision="fp64", leaves=leaves, blocks=blocks, net=net,
+                        margin=margin, n_samp=n_samp, reps=reps, verbose=verbose)
         raise RuntimeError(f"autotune: no configuration certifies tol={tol:g} (degrees up to 23)")
+    best["precision"] = precision
     for blk in blocks:
         if blk == "auto":
             continue
@@ -145,7 +151,7 @@ class PlannedEvaluator:
     def _build(self):
         c = self.config
         self.state = api.setup(self.x, self.k, p=max(c["p_level"]), N_max=c["N_max"], p_level=c["p_level"],
-                               precision=self.precision, m2l_kernel=c["m2l_kernel"])
+                               precision=c.get("precision", self.precision), m2l_kernel=c["m2l_kernel"])
 
     def __call__(self, charges):
         if self.state is None:
@@ -168,7 +174,8 @@ class PlannedEvaluator:
                 raise RuntimeError(f"plan: cannot certify tol={self.tol:g}; history {self.report}")
             if self.wisdom:
                 w = load_wisdom()
-                w[self.key] = {kk: v for kk, v in self.config.items() if kk in ("N_max", "m2l_kernel", "p_level")}
+                w[self.key] = {kk: v for kk, v in self.config.items()
+                               if kk in ("N_max", "m2l_kernel", "p_level", "precision")}
                 save_wisdom(w)
         return api.evaluate(self.state, charges)
 

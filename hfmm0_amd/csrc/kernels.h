@@ -91,25 +91,25 @@ void launch_direct(const double* X, const double* Y, const double* Z, const doub
 // k-steps) lane l holds, as four consecutive values j = 0..3,
 // A[16 mt + l % 16][16 kg + 4 j + l / 16], zero outside the block.
 constexpr int kM2LBlockMaxP = 23;  // largest degree handled by the block kernel
-constexpr int kM2LMaxAccRegs = 128; // VGPRs per lane for the stage-3 accumulators
+constexpr int kM2LMaxAccRegs = 128; // VGPRs per lane for the stage-3 accumulators (block-size choice)
 __host__ __device__ constexpr int m2l_tiles(int len) { return (len + 15) / 16; }
 // accumulator VGPRs of a (MAXU units per wave, TB targets, element size) configuration
 __host__ __device__ constexpr int m2l_acc_regs(int maxu, int tb, int elem_bytes) {
     return maxu * (2 * tb / 16) * 4 * (elem_bytes / 4);
 }
-// LDS elements of the expansion block ((p+1)^2 rows x 2 TB + 1, plus one scratch row
-// that receives the stores of rows outside a block), padded to 16 bytes
-__host__ __device__ constexpr int m2l_lds_x_elems(int p, int tb) {
-    return (((p + 1) * (p + 1) + 1) * (2 * tb + 1) + 3) & ~3;
+// LDS elements of the expansion block ((p+1)^2 rows x ncol + 1 columns, plus one scratch
+// row that receives the stores of rows outside a block), padded to 16 bytes
+__host__ __device__ constexpr int m2l_lds_x_elems(int p, int ncol) {
+    return (((p + 1) * (p + 1) + 1) * (ncol + 1) + 3) & ~3;
 }
 struct M2LBlockArgs {
     int p;                           // effective degree of this level (<= p_tab)
     int p_tab;                       // degree the tables, units and LDS were sized for
     int n_items;
-    const int4* items;               // (block, first entry, end entry, atomic add)
+    const int4* items;               // (block, first group, end group, atomic add)
     const int* blk_targets;          // blocks x TB target boxes (-1: empty slot)
-    const unsigned short* ent_dir;   // per entry: direction index
-    const int* ent_off;              // per entry: TB source expansion offsets (-1: none)
+    const unsigned short* grp_dir;   // per group: G directions of equal theta and |d| (0xFFFF: none)
+    const int* grp_off;              // per group: G x TB source expansion offsets (-1: none)
     const int* units1;               // forward rotation units: degree n
     int n_units1;
     const int* units2;               // coaxial units: m + p_tab
@@ -130,9 +130,9 @@ struct M2LBlockArgs {
     double2* L;                      // local expansions (FP64), accumulated into
     const int64_t* exp_off;
 };
-size_t m2l_block_lds_bytes(int p, int tb, int precision_bytes);
-void prepare_m2l_block(int p, int n_units3, int tb, int fp32);
-void launch_m2l_block(const M2LBlockArgs& a, int tb, int fp32, hipStream_t s);
+size_t m2l_block_lds_bytes(int p, int tb, int g, int precision_bytes);
+void prepare_m2l_block(int p, int tb, int g, int fp32);
+void launch_m2l_block(const M2LBlockArgs& a, int tb, int g, int fp32, hipStream_t s);
 void launch_to_float(const double2* in, float2* out, int64_t n, hipStream_t s);
 // the result rounded to complex64 (output opt-in)
 void launch_to_c64(const double2* in, float2* out, int64_t n, hipStream_t s);

@@ -3,21 +3,26 @@
 //
 // A workgroup (4 waves) owns TB target boxes of one level that have the same position
 // (octant) inside their parents, so their V lists use the same set of displacements.
-// For each displacement d present in the block it
-//   0. gathers the TB source multipoles into LDS, multiplied by e^{i m phi_d}
-//      (columns = (pair, re/im), rows = coefficient n*n+n+m);
-//   1. rotates each degree block in place: X_n <- E_n(theta_d)^T X_n   (real x complex)
-//   2. applies the coaxial operator per order m in place: X^m <- T^{|m|}(|d|) X^m, as
+// Displacements with the same polar angle theta and length |d| (they differ only in the
+// azimuth phi) share the rotation E_n(theta) and the coaxial operator T(|d|), so they are
+// processed together as one group of up to G directions: G * TB pairs, 2 G TB columns.
+// For each group the workgroup
+//   0. gathers the G * TB source multipoles into LDS, each multiplied by e^{i m phi} of its
+//      direction (rows = coefficient n*n+n+m, columns = (direction, pair, re/im));
+//   1. rotates each degree block in place: X_n <- E_n^T X_n   (real x complex);
+//   2. applies the coaxial operator per order m in place: X^m <- T^{|m|} X^m, as
 //      Re(T) * X + Im(T) * (X with re/im columns swapped and signed);
-//   3. rotates back, W_n = E_n X_n, multiplies by e^{-i m phi_d} (the partner lane holds
-//      the other half of the complex value) and accumulates in registers.
+//   3. rotates back, W_n = E_n X_n, multiplies by e^{-i m phi} of each column's direction
+//      (the partner lane holds the other half of the complex value) and accumulates per
+//      target in registers.
 // Every product is a 16x16x4 MFMA tile.  The A operands come from operator tables that
 // the host stores in fragment order, zero-padded to whole 16x16 tiles, so a lane loads
 // the A values of four k-steps with one 16-byte (FP32) or two 16-byte (FP64) loads and
-// no bounds test.  B operands are read from LDS with row indices clamped into the block
-// (their products with the zero padding of A vanish), and each A fragment is applied to
-// every 16-column tile of the block.  The stage-3 accumulators stay in registers across
-// all displacements and are added to the FP64 local expansions once at the end.
+// no bounds test; each A fragment is applied to every 16-column tile of the group.  B
+// operands are read from LDS with row indices clamped into the block (their products
+// with the zero padding of A vanish); rows outside a block are stored to a scratch row.  The stage-3
+// accumulators stay in registers across all groups and are added to the FP64 local
+// expansions once at the end.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -25,9 +30,9 @@
 
 namespace hfmm {
 
-// LDS: the expansion block (with its scratch row), then p+1 complex phases
-size_t m2l_block_lds_bytes(int p, int tb, int precision_bytes) {
-    return ((size_t)m2l_lds_x_elems(p, tb) + 2 * (size_t)(p + 1)) * precision_bytes;
+// LDS: the expansion block (with its scratch row), then G * (p+1) complex phases
+size_t m2l_block_lds_bytes(int p, int tb, int g, int precision_bytes) {
+    return ((size_t)m2l_lds_x_elems(p, 2 * tb * g) + 2 * (size_t)g * (p + 1)) * precision_bytes;
 }
 
 namespace {
@@ -77,80 +82,94 @@ __device__ inline void decode_nm(int c, int& n, int& m) {
     m = c - n * n - n;
 }
 
-constexpr int kMaxTiles = (2 * kM2LBlockMaxP + 1 + 15) / 16;   // row tiles of a degree block
-constexpr int kMaxTiles2 = (kM2LBlockMaxP + 1 + 15) / 16;      // row tiles of an order block
-
-// Work item: entries [e0, e1) of one target block.  When a block is split over several
-// items (coarse levels, to fill the GPU) the results are added atomically.
-template <typename T, int TB, int MAXU>
+// Work item: groups [g0, g1) of one target block.  When a block is split over several
+// items (coarse levels, to fill the GPU) the results are added atomically.  KT is the
+// number of 16-row tiles of the largest degree block, m2l_tiles(2 p_tab + 1) (1 up to
+// p = 7, 2 up to 15, 3 up to 23); it bounds the A fragments a unit keeps in registers.
+template <typename T, int TB, int G, int KT, int MAXU>
 __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     using V4 = typename Mma<T>::v4;
     using C2 = typename Mma<T>::c2;
-    constexpr int NCOL = 2 * TB;            // (pair, re/im) columns
-    constexpr int LDX = NCOL + 1;           // odd row stride in LDS
-    constexpr int NCT = NCOL / 16;          // 16-wide column tiles
-    constexpr int NG = 8;                   // gathered values per thread and batch
+    constexpr int kMaxTiles = KT;                    // row tiles of a degree block
+    constexpr int kMaxTiles2 = KT == 3 ? 2 : 1;      // row tiles of an order block
+    constexpr int NCOLD = 2 * TB;           // columns of one direction: (pair, re/im)
+    constexpr int NCTD = NCOLD / 16;        // 16-wide column tiles of one direction
+    constexpr int NCT = G * NCTD;           // column tiles of the group
+    constexpr int LDX = G * NCOLD + 1;      // odd row stride in LDS
     extern __shared__ char smem_raw[];
     const int p = a.p, P1 = p + 1, nc = P1 * P1;
     const int spare = (a.p_tab + 1) * (a.p_tab + 1);   // scratch row for out-of-block stores
     T* X = (T*)smem_raw;
-    C2* Ps = (C2*)(X + m2l_lds_x_elems(a.p_tab, TB));
+    C2* Ps = (C2*)(X + m2l_lds_x_elems(a.p_tab, G * NCOLD));   // G x (p+1) phases
     // the wave index is uniform across a wavefront: keep it (and all unit bookkeeping
     // derived from it) in scalar registers so that the unit branches are scalar
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int li = lane & 15, lk = lane >> 4;
-    const int4 item = a.items[blockIdx.x];  // (block, e0, e1, atomic)
+    const int4 item = a.items[blockIdx.x];  // (block, g0, g1, atomic)
     const int* tg = a.blk_targets + (size_t)item.x * TB;
     const C2* Mexp = (const C2*)a.M;
 
-    V4 acc[MAXU][NCT];
+    V4 acc[MAXU][NCTD];
 #pragma unroll
     for (int u = 0; u < MAXU; ++u)
 #pragma unroll
-        for (int ct = 0; ct < NCT; ++ct) acc[u][ct] = V4{0, 0, 0, 0};
+        for (int cd = 0; cd < NCTD; ++cd) acc[u][cd] = V4{0, 0, 0, 0};
 
     for (int e = item.y; e < item.z; ++e) {
-        const int d = a.ent_dir[e];
-        const DispInfo di = a.dinfo[d];
+        const unsigned short* gd = a.grp_dir + (size_t)e * G;
+        const DispInfo di = a.dinfo[gd[0]];   // shared by the directions of the group
         const T* R1 = (const T*)a.rot1 + (size_t)di.theta * a.rot_stride;
         const T* R3 = (const T*)a.rot3 + (size_t)di.theta * a.rot_stride;
         const T* Tz = (const T*)a.coax + (size_t)di.dist * a.coax_stride;
-        const C2* phg = (const C2*)a.phase + (size_t)d * a.phase_stride;
-        const int* soff = a.ent_off + (size_t)e * TB;
+        const int* soff = a.grp_off + (size_t)e * G * TB;   // -1: no source (or no direction)
 
-        // stage 0: gather + e^{i m phi}; the loads of a batch are issued before any is used
-        if (threadIdx.x < P1) Ps[threadIdx.x] = phg[threadIdx.x];
-        for (int base = 0; base < TB * nc; base += 256 * NG) {
-            C2 raw[NG];
+        // stage 0: gather + e^{i m phi}
+        for (int i = threadIdx.x; i < G * P1; i += 256) {
+            const int g = i / P1, m = i - g * P1;
+            const unsigned short d = gd[g];
+            C2 v;
+            v.x = 1;
+            v.y = 0;
+            if (d != 0xFFFF) v = ((const C2*)a.phase)[(size_t)d * a.phase_stride + m];
+            Ps[i] = v;
+        }
+        // every wave gathers for one direction (uniform source offsets, scalar loads),
+        // lanes over coefficients (consecutive lanes read consecutive words of one
+        // expansion); the coefficient's (n, m) is decoded once per pass, not per source
+        {
+            constexpr int WPD = 4 / G;   // waves per direction
+            const int g = wave / WPD;
+            const unsigned short d = gd[g];
+            const int* so = soff + g * TB;
+            for (int c = (wave % WPD) * 64 + lane; c < nc; c += 64 * WPD) {
+                int n, m;
+                decode_nm(c, n, m);
+                C2 pm;
+                pm.x = 1;
+                pm.y = 0;
+                if (d != 0xFFFF) pm = ((const C2*)a.phase)[(size_t)d * a.phase_stride + (m < 0 ? -m : m)];
+                if (m < 0) pm.y = -pm.y;
+                C2 raw[TB];
 #pragma unroll
-            for (int g = 0; g < NG; ++g) {
-                const int idx = base + threadIdx.x + 256 * g;
-                raw[g].x = 0;
-                raw[g].y = 0;
-                if (idx < TB * nc) {
-                    const int j = idx / nc, c = idx - j * nc;
-                    const int o = soff[j];
-                    if (o >= 0) raw[g] = Mexp[(int64_t)o + c];
+                for (int j = 0; j < TB; ++j) {
+                    const int o = so[j];
+                    raw[j].x = 0;
+                    raw[j].y = 0;
+                    if (o >= 0) raw[j] = Mexp[(int64_t)o + c];
                 }
-            }
+                T* xr = X + c * LDX + g * NCOLD;
 #pragma unroll
-            for (int g = 0; g < NG; ++g) {
-                const int idx = base + threadIdx.x + 256 * g;
-                if (idx < TB * nc) {
-                    const int j = idx / nc, c = idx - j * nc;
-                    int n, m;
-                    decode_nm(c, n, m);
-                    C2 pm = phg[m < 0 ? -m : m];
-                    if (m < 0) pm.y = -pm.y;
-                    X[c * LDX + 2 * j] = raw[g].x * pm.x - raw[g].y * pm.y;
-                    X[c * LDX + 2 * j + 1] = raw[g].x * pm.y + raw[g].y * pm.x;
+                for (int j = 0; j < TB; ++j) {
+                    xr[2 * j] = raw[j].x * pm.x - raw[j].y * pm.y;
+                    xr[2 * j + 1] = raw[j].x * pm.y + raw[j].y * pm.x;
                 }
             }
         }
         __syncthreads();
 
         // stage 1: X_n <- E_n^T X_n in place.  A unit is one degree n with all its row and
-        // column tiles: the wave writes back only after all products are formed, and no
+        // column tiles: the A fragments of a k group are loaded once and applied to every
+        // column tile; the wave writes back only after all products are formed, and no
         // other wave touches the rows of degree n.
         for (int u = wave; u < a.n_units1; u += 4) {
             const int n = a.units1[u];
@@ -171,7 +190,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                 T bv[4][NCT];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const int row = base + min(16 * kg + 4 * j + lk, w - 1);
+                    const int row = base + min(16 * kg + 4 * j + lk, w - 1);   // A is 0 past w
 #pragma unroll
                     for (int ct = 0; ct < NCT; ++ct) bv[j][ct] = X[row * LDX + ct * 16 + li];
                 }
@@ -264,7 +283,8 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
         }
         __syncthreads();
 
-        // stage 3: W_n = E_n X_n, times e^{-i m phi}, accumulated; unit = (n, row tile)
+        // stage 3: W_n = E_n X_n, times e^{-i m phi} of each column tile's direction, summed
+        // over the directions into the accumulators of the targets; unit = (n, row tile)
 #pragma unroll
         for (int uu = 0; uu < MAXU; ++uu) {
             const int u = wave + 4 * uu;
@@ -297,14 +317,17 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int m = mt * 16 + Mma<T>::row(lk, r) - n;
-                    const int am = m < 0 ? -m : m;
-                    const C2 pm = Ps[min(am, p)];
-                    const T cs = pm.x, sn = m < 0 ? -pm.y : pm.y;
+                    const int am = min(m < 0 ? -m : m, p);
 #pragma unroll
-                    for (int ct = 0; ct < NCT; ++ct) {
-                        const T v = c4[ct][r];
-                        const T partner = Mma<T>::swap_pair(v);
-                        acc[uu][ct][r] += im_part ? (v * cs - partner * sn) : (v * cs + partner * sn);
+                    for (int g = 0; g < G; ++g) {
+                        const C2 pm = Ps[g * P1 + am];
+                        const T cs = pm.x, sn = m < 0 ? -pm.y : pm.y;
+#pragma unroll
+                        for (int cd = 0; cd < NCTD; ++cd) {
+                            const T v = c4[g * NCTD + cd][r];
+                            const T partner = Mma<T>::swap_pair(v);
+                            acc[uu][cd][r] += im_part ? (v * cs - partner * sn) : (v * cs + partner * sn);
+                        }
                     }
                 }
             }
@@ -320,8 +343,8 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
             const int code = a.units3[u];
             const int n = code & 255, mt = code >> 8;
 #pragma unroll
-            for (int ct = 0; ct < NCT; ++ct) {
-                const int col = ct * 16 + li;
+            for (int cd = 0; cd < NCTD; ++cd) {
+                const int col = cd * 16 + li;
                 const int t = tg[col >> 1];
                 if (t >= 0) {
                     double* dst = (double*)(a.L + a.exp_off[t]) + (col & 1);
@@ -331,9 +354,9 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                         if (row < 2 * n + 1) {
                             double* x = dst + 2 * (n * n + row);
                             if (item.w)
-                                atomicAdd(x, (double)acc[uu][ct][r]);
+                                atomicAdd(x, (double)acc[uu][cd][r]);
                             else
-                                *x += (double)acc[uu][ct][r];
+                                *x += (double)acc[uu][cd][r];
                         }
                     }
                 }
@@ -353,59 +376,64 @@ __global__ void to_c64_kernel(const double2* in, float2* out, int64_t n) {
     if (i < n) out[i] = make_float2((float)in[i].x, (float)in[i].y);
 }
 
-template <typename T, int TB, int MAXU>
+template <typename T, int TB, int G, int KT, int MAXU>
 void launch_one(const M2LBlockArgs* a, size_t shm, hipStream_t s) {
     if (!a) {   // prepare: allow the dynamic LDS this instantiation may use
         int dev = 0, lds = 0;
         (void)hipGetDevice(&dev);
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) lds = 65536;
-        hipError_t e = hipFuncSetAttribute((const void*)m2l_block_kernel<T, TB, MAXU>,
+        hipError_t e = hipFuncSetAttribute((const void*)m2l_block_kernel<T, TB, G, KT, MAXU>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) fail(-3, std::string("hipFuncSetAttribute(m2l) failed: ") + hipGetErrorString(e));
         return;
     }
-    hipLaunchKernelGGL((m2l_block_kernel<T, TB, MAXU>), dim3(a->n_items), dim3(256), shm, s, *a);
+    hipLaunchKernelGGL((m2l_block_kernel<T, TB, G, KT, MAXU>), dim3(a->n_items), dim3(256), shm, s, *a);
 }
 
-template <typename T, int TB, int MAXU>
-void launch_checked(const M2LBlockArgs* a, size_t shm, hipStream_t s) {
-    if constexpr (m2l_acc_regs(MAXU, TB, sizeof(T)) <= kM2LMaxAccRegs)
-        launch_one<T, TB, MAXU>(a, shm, s);
+// (KT, MAXU) follow from the table degree: p <= 7: (1, 4); p <= 11: (2, 4);
+// p <= 15: (2, 8); p <= 17: (3, 8); p <= 23: (3, 12)
+template <typename T, int TB, int G>
+void launch_p(const M2LBlockArgs* a, size_t shm, int p, hipStream_t s) {
+    if (p <= 7)
+        launch_one<T, TB, G, 1, 4>(a, shm, s);
+    else if (p <= 11)
+        launch_one<T, TB, G, 2, 4>(a, shm, s);
+    else if (p <= 15)
+        launch_one<T, TB, G, 2, 8>(a, shm, s);
+    else if (p <= 17)
+        launch_one<T, TB, G, 3, 8>(a, shm, s);
+    else if (p <= kM2LBlockMaxP)
+        launch_one<T, TB, G, 3, 12>(a, shm, s);
     else
-        fail(-9, "internal: M2L block configuration exceeds the register budget");
+        fail(-9, "internal: degree above the block kernel's range");
 }
 
 template <typename T, int TB>
-void launch_tb(const M2LBlockArgs* a, size_t shm, int maxu, hipStream_t s) {
-    if (maxu <= 4)
-        launch_checked<T, TB, 4>(a, shm, s);
-    else if (maxu <= 8)
-        launch_checked<T, TB, 8>(a, shm, s);
-    else if (maxu <= 12)
-        launch_checked<T, TB, 12>(a, shm, s);
-    else if (maxu <= 16)
-        launch_checked<T, TB, 16>(a, shm, s);
+void launch_g(const M2LBlockArgs* a, size_t shm, int g, int p, hipStream_t s) {
+    if (g == 1)
+        launch_p<T, TB, 1>(a, shm, p, s);
+    else if (g == 2)
+        launch_p<T, TB, 2>(a, shm, p, s);
+    else if (g == 4)
+        launch_p<T, TB, 4>(a, shm, p, s);
     else
-        fail(-9, "internal: too many M2L units per wave");
+        fail(-9, "internal: unsupported M2L group size");
 }
 
-void dispatch(const M2LBlockArgs* a, int p, int n_units3, int tb, int fp32, hipStream_t s) {
-    const int maxu = (n_units3 + 3) / 4;
-    const size_t shm = m2l_block_lds_bytes(p, tb, fp32 ? 4 : 8);
+void dispatch(const M2LBlockArgs* a, int p, int tb, int g, int fp32, hipStream_t s) {
+    const size_t shm = m2l_block_lds_bytes(p, tb, g, fp32 ? 4 : 8);
     if (fp32) {
-        if (tb == 32)
-            launch_tb<float, 32>(a, shm, maxu, s);
-        else if (tb == 16)
-            launch_tb<float, 16>(a, shm, maxu, s);
+        if (tb == 16)
+            launch_g<float, 16>(a, shm, g, p, s);
         else if (tb == 8)
-            launch_tb<float, 8>(a, shm, maxu, s);
+            launch_g<float, 8>(a, shm, g, p, s);
         else
             fail(-9, "internal: unsupported M2L block size");
     } else {
-        if (tb == 16)
-            launch_tb<double, 16>(a, shm, maxu, s);
-        else if (tb == 8)
-            launch_tb<double, 8>(a, shm, maxu, s);
+        if (tb == 8)
+            launch_g<double, 8>(a, shm, g, p, s);
+        else if (tb == 16 && g == 1)
+            launch_p<double, 16, 1>(a, shm, p, s);
         else
             fail(-9, "internal: unsupported M2L block size");
     }
@@ -413,11 +441,11 @@ void dispatch(const M2LBlockArgs* a, int p, int n_units3, int tb, int fp32, hipS
 
 }  // namespace
 
-void prepare_m2l_block(int p, int n_units3, int tb, int fp32) { dispatch(nullptr, p, n_units3, tb, fp32, nullptr); }
+void prepare_m2l_block(int p, int tb, int g, int fp32) { dispatch(nullptr, p, tb, g, fp32, nullptr); }
 
-void launch_m2l_block(const M2LBlockArgs& a, int tb, int fp32, hipStream_t s) {
+void launch_m2l_block(const M2LBlockArgs& a, int tb, int g, int fp32, hipStream_t s) {
     if (a.n_items <= 0) return;
-    dispatch(&a, a.p_tab, a.n_units3, tb, fp32, s);
+    dispatch(&a, a.p_tab, tb, g, fp32, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) fail(-3, std::string("kernel launch failed (m2l_block): ") + hipGetErrorString(e));
 }

@@ -42,7 +42,8 @@ inline size_t charge_bytes(int q_type) {
 
 struct M2LBlockLevel {
     bool use = false;
-    int tb = 0;
+    int tb = 0;                    // target boxes per workgroup
+    int grp = 1;                   // directions per group
     int n_items = 0;
     int4* items = nullptr;
     int* targets = nullptr;
@@ -187,20 +188,28 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     int n_units3 = 0;
     for (int n = 0; n <= p; ++n) n_units3 += m2l_tiles(2 * n + 1);
     const int maxu = (n_units3 + 3) / 4;
-    // block size: a requested one if it fits, else the largest whose accumulators fit the
-    // register budget and whose LDS buffer fits a CU, with 32 only at levels of at least
-    // 16384 boxes (measured at N = 1M: 32 wins at the finest level, 16 at coarser ones,
-    // where fewer, longer work items amortise the per-item accumulator set-up better)
-    auto fits = [&](int c) {
-        if (fp32 ? (c != 8 && c != 16 && c != 32) : (c != 8 && c != 16)) return false;
+    // shape of one operator application: TB target boxes x G directions (of equal theta
+    // and |d|).  A requested pair count TB*G (m2l_kernel = 8, 16, 32, 64) is used if it
+    // fits.  Otherwise, from measurements at N = 1M, k = 40 (finest level, 5.4M pairs):
+    // FP32 takes 8 x 2 (7.4 ms at p = 11 vs 7.6 for 8 x 4 and 8.1 for 16 x 4; 8.8 ms at
+    // p = 12 vs 9.2 for 8 x 4), FP64 takes 8 x 4 (50.8 ms at p = 17 vs 55.4 for 8 x 2);
+    // the next smaller shape when LDS is short.
+    auto fits = [&](int c, int g) {
+        if (!fp32 && !(c == 8 || (c == 16 && g == 1))) return false;   // FP64 instantiations
         if (m2l_acc_regs((maxu + 3) / 4 * 4, c, eb) > kM2LMaxAccRegs) return false;
-        return m2l_block_lds_bytes(p, c, eb) <= P.max_lds;
+        return m2l_block_lds_bytes(p, c, g, eb) <= P.max_lds;
     };
-    const int n_level = (int)targets.size();
-    int tb = 0;
-    if (P.m2l_kernel > 1 && fits(P.m2l_kernel)) tb = P.m2l_kernel;
-    for (int c : {32, 16, 8})
-        if (!tb && fits(c) && (c < 32 || n_level >= 16384)) tb = c;
+    int tb = 0, grp = 0;
+    const int shapes[6][2] = {{16, 4}, {8, 4}, {16, 2}, {8, 2}, {16, 1}, {8, 1}};
+    if (P.m2l_kernel > 1)
+        for (auto& sh : shapes)
+            if (!tb && sh[0] * sh[1] == P.m2l_kernel && fits(sh[0], sh[1])) tb = sh[0], grp = sh[1];
+    const int pref[2][2][2] = {{{8, 2}, {8, 1}}, {{8, 4}, {8, 2}}};
+    const int* pr = &pref[fp32 ? 0 : 1][0][0];
+    for (int i = 0; i < 2 && !tb; ++i)
+        if (fits(pr[2 * i], pr[2 * i + 1])) tb = pr[2 * i], grp = pr[2 * i + 1];
+    for (auto& sh : shapes)
+        if (!tb && fits(sh[0], sh[1])) tb = sh[0], grp = sh[1];
     if (!tb) return;
     // units of work, costliest first (dealt round-robin to the 4 waves)
     std::vector<std::pair<int, int>> u1, u2, u3;   // (cost, code)
@@ -225,14 +234,15 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     const Tree& t = P.tree;
     std::vector<int> tpos(t.n_boxes(), -1);     // target ordinal in `targets`
     for (size_t i = 0; i < targets.size(); ++i) tpos[targets[i]] = (int)i;
-    std::vector<int> blk_targets, blk_ptr{0}, ent_off;
-    std::vector<unsigned short> ent_dir;
+    std::vector<int> blk_targets, blk_ptr{0}, grp_off;
+    std::vector<unsigned short> grp_dir;
     for (int o = 0; o < 8; ++o) {
         std::vector<int> boxes;
         for (int b = t.level_start[L]; b < t.level_start[L + 1]; ++b)
             if ((int)(t.code[b] & 7) == o && tpos[b] >= 0) boxes.push_back(b);
         for (size_t b0 = 0; b0 < boxes.size(); b0 += tb) {
-            std::map<std::tuple<int, int, int>, std::vector<int>> ents;   // (theta, dist, dir) -> sources
+            // (theta, dist) -> direction -> sources of the TB targets
+            std::map<std::pair<int, int>, std::map<int, std::vector<int>>> ents;
             for (int j = 0; j < tb; ++j) {
                 const int b = b0 + j < boxes.size() ? boxes[b0 + j] : -1;
                 blk_targets.push_back(b);
@@ -240,17 +250,23 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
                 const int ti = tpos[b];
                 for (int e = ptr[ti]; e < ptr[ti + 1]; ++e) {
                     const int d = dir[e];
-                    auto key = std::make_tuple(dinfo[d].theta, dinfo[d].dist, d);
-                    auto it = ents.find(key);
-                    if (it == ents.end()) it = ents.emplace(key, std::vector<int>(tb, -1)).first;
+                    auto& byd = ents[std::make_pair(dinfo[d].theta, dinfo[d].dist)];
+                    auto it = byd.find(d);
+                    if (it == byd.end()) it = byd.emplace(d, std::vector<int>(tb, -1)).first;
                     it->second[j] = (int)P.exp_off[src[e]];
                 }
             }
             for (auto& kv : ents) {
-                ent_dir.push_back((unsigned short)std::get<2>(kv.first));
-                for (int j = 0; j < tb; ++j) ent_off.push_back(kv.second[j]);
+                std::vector<std::pair<int, const std::vector<int>*>> ds;
+                for (auto& dv : kv.second) ds.push_back({dv.first, &dv.second});
+                for (size_t g0 = 0; g0 < ds.size(); g0 += grp)
+                    for (int g = 0; g < grp; ++g) {
+                        const bool has = g0 + g < ds.size();
+                        grp_dir.push_back(has ? (unsigned short)ds[g0 + g].first : (unsigned short)0xFFFF);
+                        for (int j = 0; j < tb; ++j) grp_off.push_back(has ? (*ds[g0 + g].second)[j] : -1);
+                    }
             }
-            blk_ptr.push_back((int)ent_dir.size());
+            blk_ptr.push_back((int)(grp_dir.size() / grp));
         }
     }
     // work items: split the entry lists of the blocks when there are few blocks
@@ -266,11 +282,12 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     M2LBlockLevel& B = P.m2lb[L];
     B.use = true;
     B.tb = tb;
+    B.grp = grp;
     B.n_items = (int)items.size();
     B.items = P.upload(items);
     B.targets = P.upload(blk_targets);
-    B.dir = P.upload(ent_dir);
-    B.off = P.upload(ent_off);
+    B.dir = P.upload(grp_dir);
+    B.off = P.upload(grp_off);
     B.units1 = P.upload(units1);
     B.n_units1 = (int)units1.size();
     B.units2 = P.upload(units2);
@@ -311,7 +328,7 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
         B.coax = P.upload(cf);
     }
     B.coax_off = P.upload(coff);
-    prepare_m2l_block(p, B.n_units3, tb, fp32);
+    prepare_m2l_block(p, tb, grp, fp32);
 }
 
 void build_tables(hfmm_plan& P) {
@@ -557,8 +574,9 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
         if (o.p_level[L] != 0 && (o.p_level[L] < 2 || o.p_level[L] > kMaxP))
             fail(HFMM_ERR_ARG, "p_level entries must be 0 or in [2, 40]");
     if (o.precision != HFMM_PREC_FP64 && o.precision != HFMM_PREC_MIXED) fail(HFMM_ERR_ARG, "unknown precision");
-    if (o.m2l_kernel != 0 && o.m2l_kernel != 1 && o.m2l_kernel != 8 && o.m2l_kernel != 16 && o.m2l_kernel != 32)
-        fail(HFMM_ERR_ARG, "m2l_kernel must be 0, 1, 8, 16 or 32");
+    if (o.m2l_kernel != 0 && o.m2l_kernel != 1 && o.m2l_kernel != 8 && o.m2l_kernel != 16 && o.m2l_kernel != 32 &&
+        o.m2l_kernel != 64)
+        fail(HFMM_ERR_ARG, "m2l_kernel must be 0, 1, 8, 16, 32 or 64");
     check_device();
     upload_legendre_coefficients();
     P.n = n;
@@ -711,8 +729,8 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
         a.n_items = B.n_items;
         a.items = B.items;
         a.blk_targets = B.targets;
-        a.ent_dir = B.dir;
-        a.ent_off = B.off;
+        a.grp_dir = B.dir;
+        a.grp_off = B.off;
         a.units1 = B.units1;
         a.n_units1 = B.n_units1;
         a.units2 = B.units2;
@@ -733,7 +751,7 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
         a.L = P.dL;
         a.exp_off = P.d_expoff;
         if (lev) HIPCHK(hipEventRecord(lev[2 * L], s));
-        launch_m2l_block(a, B.tb, fp32, s);
+        launch_m2l_block(a, B.tb, B.grp, fp32, s);
         if (lev) HIPCHK(hipEventRecord(lev[2 * L + 1], s));
     }
     mark(4);

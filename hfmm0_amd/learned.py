@@ -12,9 +12,12 @@ Features of level l (normalised to O(1)):
   2 log2(mean points per box of the level) / 10      (source count)
   3 log2(max points per box of the level) / 10       (target count)
   4 log10(||q||_2 / sqrt(N))                          (charge norm)
-  5 mean V-list size per box / 189                    (neighbour count)
+  5 the level's share of all V-list pairs            (neighbour count)
 Target: (p_min - 6) / 26, p_min the smallest degree whose truncation error at that
-level, with every other level at the table degree, stays below the level budget.
+level, with every other level at the table degree, stays below the level budget.  The
+budget of a level is the total budget times its share of the V pairs, so the error is
+spent where the work is (the finest levels) and the coarse, cheap levels get high
+degrees -- the allocation that minimises the M2L work for a total error.
 """
 import ctypes
 import math
@@ -137,6 +140,7 @@ def level_features(tree, charges, k):
     level, count = np.asarray(tree["level"]), np.asarray(tree["count"])
     V = np.asarray(tree["V"])
     vt_level = level[V[:, 0]] if len(V) else np.zeros(0, int)
+    n_v = max(len(V), 1)
     rows = []
     for lv in range(2, tree["depth"] + 1):
         sel = level == lv
@@ -147,16 +151,25 @@ def level_features(tree, charges, k):
         diam = 2.0 * math.sqrt(3.0) * tree["extent"] * 0.5 ** (lv + 1)
         mean_c = float(count[sel].mean())
         max_c = float(count[sel].max())
-        nv = float((vt_level == lv).sum()) / nb
+        share = float((vt_level == lv).sum()) / n_v
         rows.append([lv / 10.0, k * diam / 50.0, math.log2(max(mean_c, 1.0)) / 10.0,
-                     math.log2(max(max_c, 1.0)) / 10.0, qn, nv / 189.0])
+                     math.log2(max(max_c, 1.0)) / 10.0, qn, share])
     return np.array(rows, dtype=np.float32)
 
 
-def label_levels(state, q, budget=5e-7, p_lo=4):
+def v_shares(tree):
+    """Share of all V-list pairs per level 2..depth."""
+    level = np.asarray(tree["level"])
+    V = np.asarray(tree["V"])
+    vt = level[V[:, 0]] if len(V) else np.zeros(0, int)
+    return np.array([float((vt == lv).sum()) / max(len(V), 1) for lv in range(2, tree["depth"] + 1)])
+
+
+def label_levels(state, q, budget=5e-7, p_lo=4, shares=None):
     """Smallest degree per level (2..depth) keeping that level's truncation error, with
-    the other levels at their table degree, at or below `budget` (relative to the result
-    at the table degrees).  Uses evaluate-time degrees: no new setup per trial."""
+    the other levels at their table degree, at or below its budget (relative to the
+    result at the table degrees): `budget` times the level's V-pair share when `shares`
+    is given, else `budget`.  Uses evaluate-time degrees: no new setup per trial."""
     from .api import evaluate
     table = state.p_table
     state.set_degrees([0] * state.depth)
@@ -164,20 +177,67 @@ def label_levels(state, q, budget=5e-7, p_lo=4):
     nref = np.linalg.norm(ref)
     labels = []
     for lv in range(2, state.depth + 1):
+        b = budget if shares is None else budget * max(float(shares[lv - 2]), 1e-4)
         lo, hi = p_lo, table[lv - 1]
-        while lo < hi:                      # smallest p with error <= budget
+        while lo < hi:                      # smallest p with error <= b
             mid = (lo + hi) // 2
             degs = [0] * state.depth
             degs[lv - 1] = mid
             state.set_degrees(degs)
             err = np.linalg.norm(evaluate(state, q) - ref) / nref
-            if err <= budget:
+            if err <= b:
                 hi = mid
             else:
                 lo = mid + 1
         labels.append(lo)
     state.set_degrees([0] * state.depth)
     return np.array(labels)
+
+
+def label_levels_joint(state, q, budget=6e-7, p_lo=4):
+    """Cost-optimal degrees per level (2..depth) for a total error budget: starting from
+    the table degrees, lower one level's degree at a time -- the level whose step saves
+    the most M2L flops (its V pairs times the drop of F_M2L) per unit of added error --
+    while the error of the whole evaluate (relative to the result at the table degrees)
+    stays within `budget`.  Every trial is one evaluate with evaluate-time degrees."""
+    from .api import evaluate
+    from .workloads import f_m2l
+    table = state.p_table
+    depth = state.depth
+    tree = state.tree()
+    level = np.asarray(tree["level"])
+    V = np.asarray(tree["V"])
+    vt = level[V[:, 0]] if len(V) else np.zeros(0, int)
+    pairs = {lv: int((vt == lv).sum()) for lv in range(2, depth + 1)}
+    state.set_degrees([0] * depth)
+    ref = evaluate(state, q)
+    nref = np.linalg.norm(ref)
+    cur = {lv: table[lv - 1] for lv in range(2, depth + 1)}
+
+    def err_of(degs):
+        state.set_degrees([degs.get(lv, table[lv - 1]) for lv in range(1, depth + 1)])
+        return float(np.linalg.norm(evaluate(state, q) - ref) / nref)
+
+    e_cur = 0.0
+    while True:
+        best = None
+        for lv in cur:
+            if cur[lv] <= p_lo or pairs[lv] == 0:
+                continue
+            trial = dict(cur)
+            trial[lv] -= 1
+            e = err_of(trial)
+            if e > budget:
+                continue
+            gain = pairs[lv] * (f_m2l(cur[lv]) - f_m2l(cur[lv] - 1)) / max(e - e_cur, 1e-15)
+            if best is None or gain > best[0]:
+                best = (gain, lv, e)
+        if best is None:
+            break
+        cur[best[1]] -= 1
+        e_cur = best[2]
+    state.set_degrees([0] * depth)
+    return np.array([cur[lv] for lv in range(2, depth + 1)])
 
 
 def sampled_error(positions, charges, k, phi, n_samp=256, seed=0):

@@ -3,8 +3,9 @@ the GPU.  Run on a GPU:  python -m hfmm0_amd.train_degrees --problems 200 --out 
 
 For every seeded problem (hfmm0_amd.workloads.mixed_problem, N log-uniform, leaf size
 drawn from 32/64/128) it builds one plan at the table degree, labels every level by the
-smallest degree whose truncation error stays within the per-level budget
-(hfmm0_amd.learned.label_levels), and records the level features.  The network is then
+cost-optimal degree vector for the total budget (hfmm0_amd.learned.label_levels_joint:
+greedy descent from the table degrees, the level saving the most M2L flops per unit of
+added error first) and records the level features.  The network is then
 trained by the HIP kernels (hfmm_mlp_train) on 80% of the rows; the rest is held out.
 """
 import argparse
@@ -17,13 +18,13 @@ import time
 import numpy as np
 
 from . import api
-from .learned import P_LO, P_SPAN, DegreeMLP, label_levels, level_features
+from .learned import P_LO, P_SPAN, DegreeMLP, label_levels_joint, level_features
 from .tree import build_tree
 from .workloads import mixed_problem
 
 
 def generate(n_problems, seed0=1000, n_range=(10_000, 200_000), k_range=(1.0, 60.0), leaves=(32, 64, 128),
-             p_tab=23, budget=2e-7, log=None):
+             p_tab=23, budget=6e-7, log=None):
     feats, labels, meta = [], [], []
     for i in range(n_problems):
         rng = np.random.default_rng(seed0 + i)
@@ -32,8 +33,8 @@ def generate(n_problems, seed0=1000, n_range=(10_000, 200_000), k_range=(1.0, 60
         x, q, k, kind = mixed_problem(n, k_range, seed=seed0 + i)
         t0 = time.time()
         st = api.setup(x, k, p=p_tab, N_max=leaf)
-        lab = label_levels(st, q, budget=budget)
         tree = st.tree()
+        lab = label_levels_joint(st, q, budget=budget)
         f = level_features(tree, q, k)
         st.close()
         feats.append(f)
@@ -51,7 +52,7 @@ def main():
     ap.add_argument("--out", default="gpurun_out")
     ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--budget", type=float, default=2e-7)
+    ap.add_argument("--budget", type=float, default=6e-7, help="total error budget of the labels")
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
     X, p, meta = generate(args.problems, budget=args.budget, log=sys.stdout)
@@ -68,7 +69,7 @@ def main():
     net.save(os.path.join(args.out, "degree_mlp.npz"))
     with open(os.path.join(args.out, "degree_mlp.json"), "w") as f:
         json.dump({"problems": args.problems, "rows": int(len(X)), "train_rows": n_train, "epochs": args.epochs,
-                   "budget_per_level": args.budget, "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+                   "budget_total": args.budget, "labels": "cost-optimal joint degrees (greedy)", "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
                    "heldout_mean_resid": float(resid.mean()), "heldout_under": float(np.mean(resid < 0)),
                    "heldout_within1": float(np.mean(np.abs(resid) <= 1))}, f, indent=1)
 

@@ -53,9 +53,10 @@ typedef struct hfmm_opts {
     int max_depth;                  /* <= 0: automatic depth (reference rule) */
     int precision;                  /* HFMM_PREC_* */
     int use_graph;                  /* 1: replay evaluate as a captured hipGraph */
-    int m2l_kernel;                 /* 0: matrix-core block kernel where it fits, chosen block size;
-                                       1: generic kernel; 8, 16, 32: block kernel with that many
-                                       target boxes per workgroup where it fits */
+    int m2l_kernel;                 /* 0: matrix-core block kernel where it fits, chosen shape;
+                                       1: generic kernel; 8, 16, 32, 64: block kernel with that
+                                       many pairs per operator application (target boxes x
+                                       directions of equal theta and |d|) where it fits */
     int tree_device;                /* 1: build the octree and lists on the GPU (identical result);
                                        0: on the host */
 } hfmm_opts;

@@ -223,9 +223,10 @@ def test_set_degrees_generic_kernel_levels(hf):
         st.set_degrees([30] * st.depth)
 
 
-@pytest.mark.parametrize("precision,tb", [("fp64", 8), ("fp64", 16), ("mixed", 8), ("mixed", 16), ("mixed", 32)])
+@pytest.mark.parametrize("precision,tb", [("fp64", 8), ("fp64", 16), ("fp64", 32), ("mixed", 8), ("mixed", 16),
+                                         ("mixed", 32), ("mixed", 64)])
 def test_matrix_core_block_sizes(hf, precision, tb):
-    say(f"case: mfma m2l block size {tb} {precision} N=30000 k=15 p=13 vs generic")
+    say(f"case: mfma m2l {tb} pairs per operator {precision} N=30000 k=15 p=13 vs generic")
     x, q = D.uniform_problem(30000)
     a = hf.evaluate(hf.setup(x, 15.0, p=13, N_max=32, precision=precision, m2l_kernel=tb), q)
     b = hf.evaluate(hf.setup(x, 15.0, p=13, N_max=32, precision=precision, m2l_kernel="generic"), q)

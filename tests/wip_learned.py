@@ -29,8 +29,9 @@ def test_learned_degrees_beat_best_fixed_degree(n, k, leaf, seed):
     print(f"case: learned degrees n={n} k={k} leaf={leaf} seed={seed}", flush=True)
     x, q, kind = _problem(n, k, seed)
     net = DegreeMLP.load()
-    pl = predict_p_level(net, x, q, k, n_max=leaf, margin=1.0)
-    tuned = hf.setup(x, k, p=max(pl), N_max=leaf, p_level=pl, precision="mixed")
+    pl = predict_p_level(net, x, q, k, n_max=leaf, margin=0.0)
+    # degree selection alone: both forms in FP64, against the best fixed degree
+    tuned = hf.setup(x, k, p=max(pl), N_max=leaf, p_level=pl)
     err_t = sampled_error(x, q, k, hf.evaluate(tuned, q), n_samp=1024)
     vpl = _v_per_level(tuned)
     f_t = m2l_flops(tuned.p_level, vpl)
