@@ -91,16 +91,19 @@ void launch_direct(const double* X, const double* Y, const double* Z, const doub
 // k-steps) lane l holds, as four consecutive values j = 0..3,
 // A[16 mt + l % 16][16 kg + 4 j + l / 16], zero outside the block.
 constexpr int kM2LBlockMaxP = 23;  // largest degree handled by the block kernel
+constexpr int kXcds = 8;            // accelerator dies; workgroups are dealt to them round-robin
 constexpr int kM2LMaxAccRegs = 128; // VGPRs per lane for the stage-3 accumulators (block-size choice)
 __host__ __device__ constexpr int m2l_tiles(int len) { return (len + 15) / 16; }
 // accumulator VGPRs of a (MAXU units per wave, TB targets, element size) configuration
 __host__ __device__ constexpr int m2l_acc_regs(int maxu, int tb, int elem_bytes) {
     return maxu * (2 * tb / 16) * 4 * (elem_bytes / 4);
 }
-// LDS elements of the expansion block ((p+1)^2 rows x ncol + 1 columns, plus one scratch
-// row that receives the stores of rows outside a block), padded to 16 bytes
+// LDS elements of the expansion block: (p+1)^2 rows x ncol + 1 columns, one scratch row
+// that receives the stores of rows outside a block, and 16 zero rows that the reads of
+// the last block's padding run into; padded to 16 bytes
+__host__ __device__ constexpr int m2l_lds_x_rows(int p) { return (p + 1) * (p + 1) + 17; }
 __host__ __device__ constexpr int m2l_lds_x_elems(int p, int ncol) {
-    return (((p + 1) * (p + 1) + 1) * (ncol + 1) + 3) & ~3;
+    return (m2l_lds_x_rows(p) * (ncol + 1) + 3) & ~3;
 }
 struct M2LBlockArgs {
     int p;                           // effective degree of this level (<= p_tab)
@@ -112,7 +115,7 @@ struct M2LBlockArgs {
     const int* grp_off;              // per group: G x TB source expansion offsets (-1: none)
     const int* units1;               // forward rotation units: degree n
     int n_units1;
-    const int* units2;               // coaxial units: m + p_tab
+    const int* units2;               // coaxial units: |m| (orders +|m| and -|m|)
     int n_units2;
     const int* units3;               // back rotation units: n | row tile << 8
     int n_units3;

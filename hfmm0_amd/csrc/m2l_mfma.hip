@@ -105,7 +105,11 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     // derived from it) in scalar registers so that the unit branches are scalar
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int li = lane & 15, lk = lane >> 4;
-    const int4 item = a.items[blockIdx.x];  // (block, g0, g1, atomic)
+    // workgroup i runs on XCD i % 8: give XCD x the contiguous item range [x n/8, (x+1) n/8)
+    // (the host pads the item list to a multiple of 8 and orders it in space)
+    const int per_xcd = a.n_items / kXcds;
+    const int4 item = a.items[(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds];   // (block, g0, g1, atomic)
+    if (item.y >= item.z) return;   // padding item
     const int* tg = a.blk_targets + (size_t)item.x * TB;
     const C2* Mexp = (const C2*)a.M;
 
@@ -114,6 +118,12 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     for (int u = 0; u < MAXU; ++u)
 #pragma unroll
         for (int cd = 0; cd < NCTD; ++cd) acc[u][cd] = V4{0, 0, 0, 0};
+    // zero the whole block once: rows that the padded tiles read beyond the gathered
+    // degrees are then finite (stale results of earlier groups) or zero, and their
+    // products with the zero padding of A vanish
+    const int zero_row = spare + 1;   // the 16 rows after the scratch row stay zero
+    for (int i = threadIdx.x; i < m2l_lds_x_rows(a.p_tab) * LDX; i += 256) X[i] = (T)0;
+    __syncthreads();
 
     for (int e = item.y; e < item.z; ++e) {
         const unsigned short* gd = a.grp_dir + (size_t)e * G;
@@ -190,7 +200,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                 T bv[4][NCT];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const int row = base + min(16 * kg + 4 * j + lk, w - 1);   // A is 0 past w
+                    const int row = base + 16 * kg + 4 * j + lk;   // past w: A is 0, X is finite
 #pragma unroll
                     for (int ct = 0; ct < NCT; ++ct) bv[j][ct] = X[row * LDX + ct * 16 + li];
                 }
@@ -217,22 +227,26 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
         }
         __syncthreads();
 
-        // stage 2: X^m <- T^{|m|} X^m in place, one order m (all tiles) per unit.  The
-        // table blocks are built for p_tab; source degrees beyond the evaluated p are
-        // read as zero.
+        // stage 2: X^m <- T^{|m|} X^m in place.  A unit is one |m| with the orders +|m| and
+        // -|m| (same operator): its A fragments of a k group are loaded once and applied to
+        // both orders and every column tile.  The table blocks are built for p_tab; source
+        // degrees beyond the evaluated p read a zero row.
+        const T sgn1 = (li & 1) ? (T)1 : (T)-1;   // Im(T) x X: (-Im X, Re X) per column pair
         for (int u = wave; u < a.n_units2; u += 4) {
-            const int m = a.units2[u] - a.p_tab;
-            const int am = m < 0 ? -m : m;
+            const int am = a.units2[u];
             if (am > p) continue;
+            const int ns = am > 0 ? 2 : 1;                  // orders +am and -am
             const int len = P1 - am;
             const int ntab = m2l_tiles(a.p_tab + 1 - am);   // tiles of the table block
             const int nt = m2l_tiles(len), nks = (len + 3) >> 2;
             const T* A = Tz + a.coax_off[am];
-            V4 c4[kMaxTiles2][NCT];
+            V4 c4[2][kMaxTiles2][NCT];
 #pragma unroll
-            for (int t = 0; t < kMaxTiles2; ++t)
+            for (int sg = 0; sg < 2; ++sg)
 #pragma unroll
-                for (int ct = 0; ct < NCT; ++ct) c4[t][ct] = V4{0, 0, 0, 0};
+                for (int t = 0; t < kMaxTiles2; ++t)
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct) c4[sg][t][ct] = V4{0, 0, 0, 0};
             for (int kg = 0; kg < nt; ++kg) {
                 V4 are[kMaxTiles2], aim[kMaxTiles2];
 #pragma unroll
@@ -242,44 +256,53 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                         are[t] = Mma<T>::load4(At);
                         aim[t] = Mma<T>::load4(At + 4);
                     }
-                T b0[4][NCT], b1[4][NCT];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int kin = 16 * kg + 4 * j + lk;
-                    const bool kv = kin < len;
-                    const int nn = am + min(kin, len - 1);
-                    const int cin = nn * nn + nn + m;
+                for (int sg = 0; sg < 2; ++sg) {
+                    if (sg < ns) {
+                        const int m = sg ? -am : am;
+                        T b0[4][NCT], b1[4][NCT];
 #pragma unroll
-                    for (int ct = 0; ct < NCT; ++ct) {
-                        const int col = ct * 16 + li;
-                        const T v0 = X[cin * LDX + col], v1 = X[cin * LDX + (col ^ 1)];
-                        b0[j][ct] = kv ? v0 : (T)0;
-                        b1[j][ct] = kv ? ((col & 1) ? v1 : -v1) : (T)0;
+                        for (int j = 0; j < 4; ++j) {
+                            const int kin = 16 * kg + 4 * j + lk;
+                            const int nn = am + kin;
+                            const int cin = kin < len ? nn * nn + nn + m : zero_row;
+#pragma unroll
+                            for (int ct = 0; ct < NCT; ++ct) {
+                                const int col = ct * 16 + li;
+                                b0[j][ct] = X[cin * LDX + col];
+                                b1[j][ct] = sgn1 * X[cin * LDX + (col ^ 1)];
+                            }
+                        }
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (4 * kg + j < nks)
+#pragma unroll
+                                for (int t = 0; t < kMaxTiles2; ++t)
+                                    if (t < nt)
+#pragma unroll
+                                        for (int ct = 0; ct < NCT; ++ct) {
+                                            c4[sg][t][ct] = Mma<T>::mma(are[t][j], b0[j][ct], c4[sg][t][ct]);
+                                            c4[sg][t][ct] = Mma<T>::mma(aim[t][j], b1[j][ct], c4[sg][t][ct]);
+                                        }
                     }
                 }
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (4 * kg + j < nks)
-#pragma unroll
-                        for (int t = 0; t < kMaxTiles2; ++t)
-                            if (t < nt)
-#pragma unroll
-                                for (int ct = 0; ct < NCT; ++ct) {
-                                    c4[t][ct] = Mma<T>::mma(are[t][j], b0[j][ct], c4[t][ct]);
-                                    c4[t][ct] = Mma<T>::mma(aim[t][j], b1[j][ct], c4[t][ct]);
-                                }
             }
 #pragma unroll
-            for (int t = 0; t < kMaxTiles2; ++t)
-                if (t < nt)
+            for (int sg = 0; sg < 2; ++sg)
+                if (sg < ns) {
+                    const int m = sg ? -am : am;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = t * 16 + Mma<T>::row(lk, r);
-                        const int nn = am + row;
-                        const int dst = (row < len ? nn * nn + nn + m : spare) * LDX;
+                    for (int t = 0; t < kMaxTiles2; ++t)
+                        if (t < nt)
 #pragma unroll
-                        for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[t][ct][r];
-                    }
+                            for (int r = 0; r < 4; ++r) {
+                                const int row = t * 16 + Mma<T>::row(lk, r);
+                                const int nn = am + row;
+                                const int dst = (row < len ? nn * nn + nn + m : spare) * LDX;
+#pragma unroll
+                                for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[sg][t][ct][r];
+                            }
+                }
         }
         __syncthreads();
 
@@ -302,7 +325,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                     T bv[4][NCT];
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
-                        const int row = base + min(16 * kg + 4 * j + lk, w - 1);
+                        const int row = base + 16 * kg + 4 * j + lk;
 #pragma unroll
                         for (int ct = 0; ct < NCT; ++ct) bv[j][ct] = X[row * LDX + ct * 16 + li];
                     }
@@ -312,21 +335,22 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
                             for (int ct = 0; ct < NCT; ++ct) c4[ct] = Mma<T>::mma(av[j], bv[j][ct], c4[ct]);
                 }
-                // rows past the block are zero (A is zero-padded); their phase is irrelevant
-                const bool im_part = (li & 1) != 0;
+                // rows past the block are zero (A is zero-padded); their phase is irrelevant.
+                // Re += v cos + v' sin, Im += v cos - v' sin with v' the partner column and
+                // sin = sin(m phi): the sign is folded into one coefficient per row.
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int m = mt * 16 + Mma<T>::row(lk, r) - n;
                     const int am = min(m < 0 ? -m : m, p);
+                    const bool flip = (m < 0) != ((li & 1) != 0);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
                         const C2 pm = Ps[g * P1 + am];
-                        const T cs = pm.x, sn = m < 0 ? -pm.y : pm.y;
+                        const T cs = pm.x, sn = flip ? -pm.y : pm.y;
 #pragma unroll
                         for (int cd = 0; cd < NCTD; ++cd) {
                             const T v = c4[g * NCTD + cd][r];
-                            const T partner = Mma<T>::swap_pair(v);
-                            acc[uu][cd][r] += im_part ? (v * cs - partner * sn) : (v * cs + partner * sn);
+                            acc[uu][cd][r] = fma(v, cs, fma(Mma<T>::swap_pair(v), sn, acc[uu][cd][r]));
                         }
                     }
                 }

@@ -191,9 +191,9 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     // shape of one operator application: TB target boxes x G directions (of equal theta
     // and |d|).  A requested pair count TB*G (m2l_kernel = 8, 16, 32, 64) is used if it
     // fits.  Otherwise, from measurements at N = 1M, k = 40 (finest level, 5.4M pairs):
-    // FP32 takes 8 x 2 (7.4 ms at p = 11 vs 7.6 for 8 x 4 and 8.1 for 16 x 4; 8.8 ms at
-    // p = 12 vs 9.2 for 8 x 4), FP64 takes 8 x 4 (50.8 ms at p = 17 vs 55.4 for 8 x 2);
-    // the next smaller shape when LDS is short.
+    // FP32 takes 8 x 2 (6.0 ms at p = 11 vs 6.6 for 8 x 4 and 9.9 for 16 x 4; 7.5 ms at
+    // p = 12 vs 7.8 for 8 x 4), FP64 takes 8 x 1 (38.9 ms at p = 17 vs 48.4 for 8 x 2 and
+    // 48.3 for 8 x 4); the next smaller shape when LDS is short.
     auto fits = [&](int c, int g) {
         if (!fp32 && !(c == 8 || (c == 16 && g == 1))) return false;   // FP64 instantiations
         if (m2l_acc_regs((maxu + 3) / 4 * 4, c, eb) > kM2LMaxAccRegs) return false;
@@ -204,7 +204,7 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     if (P.m2l_kernel > 1)
         for (auto& sh : shapes)
             if (!tb && sh[0] * sh[1] == P.m2l_kernel && fits(sh[0], sh[1])) tb = sh[0], grp = sh[1];
-    const int pref[2][2][2] = {{{8, 2}, {8, 1}}, {{8, 4}, {8, 2}}};
+    const int pref[2][2][2] = {{{8, 2}, {8, 1}}, {{8, 1}, {8, 2}}};
     const int* pr = &pref[fp32 ? 0 : 1][0][0];
     for (int i = 0; i < 2 && !tb; ++i)
         if (fits(pr[2 * i], pr[2 * i + 1])) tb = pr[2 * i], grp = pr[2 * i + 1];
@@ -218,9 +218,9 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
         u1.push_back({nt * ks, n});
         for (int mt = 0; mt < nt; ++mt) u3.push_back({ks, n | (mt << 8)});
     }
-    for (int m = -p; m <= p; ++m) {
-        const int len = p + 1 - std::abs(m);
-        u2.push_back({m2l_tiles(len) * ((len + 3) / 4), m + p});
+    for (int m = 0; m <= p; ++m) {
+        const int len = p + 1 - m;
+        u2.push_back({(m ? 2 : 1) * m2l_tiles(len) * ((len + 3) / 4), m});
     }
     auto by_cost = [](const std::pair<int, int>& a, const std::pair<int, int>& b) { return a.first > b.first; };
     std::stable_sort(u1.begin(), u1.end(), by_cost);
@@ -269,16 +269,31 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
             blk_ptr.push_back((int)(grp_dir.size() / grp));
         }
     }
-    // work items: split the entry lists of the blocks when there are few blocks
+    // work items: split the entry lists of the blocks when there are few blocks.  Blocks
+    // are visited in space order (the parent region of their first target, then octant),
+    // so that the kernel's XCD-aware item mapping gives each XCD a compact region whose
+    // source expansions stay in its L2; the list is padded to a multiple of the XCD
+    // count with empty items.
     const int nblk = (int)blk_ptr.size() - 1;
+    std::vector<int> border(nblk);
+    for (int b = 0; b < nblk; ++b) border[b] = b;
+    auto region = [&](int b) {
+        const int t0 = blk_targets[(size_t)b * tb];
+        return t0 < 0 ? ~0ull : (t.code[t0] >> 3) * 8 + (t.code[t0] & 7);
+    };
+    std::stable_sort(border.begin(), border.end(), [&](int x, int y) {
+        const uint64_t rx = region(x) >> 3, ry = region(y) >> 3;
+        return rx != ry ? rx < ry : (region(x) & 7) < (region(y) & 7);
+    });
     const int nsplit = std::max(1, std::min(16, (2048 + nblk - 1) / std::max(nblk, 1)));
     std::vector<int4> items;
-    for (int b = 0; b < nblk; ++b) {
+    for (int b : border) {
         const int e0 = blk_ptr[b], e1 = blk_ptr[b + 1];
         const int chunk = (e1 - e0 + nsplit - 1) / nsplit;
         for (int s = e0; s < e1; s += std::max(chunk, 1))
             items.push_back(make_int4(b, s, std::min(e1, s + std::max(chunk, 1)), nsplit > 1 ? 1 : 0));
     }
+    while (items.size() % kXcds) items.push_back(make_int4(0, 0, 0, 0));
     M2LBlockLevel& B = P.m2lb[L];
     B.use = true;
     B.tb = tb;
