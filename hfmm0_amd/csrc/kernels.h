@@ -106,8 +106,11 @@ __host__ __device__ constexpr int m2l_lds_x_elems(int p, int ncol) {
     return (m2l_lds_x_rows(p) * (ncol + 1) + 3) & ~3;
 }
 struct M2LBlockArgs {
-    int p;                           // effective degree of this level (<= p_tab)
-    int p_tab;                       // degree the tables, units and LDS were sized for
+    int p;                           // effective source degree (<= p_tab)
+    int p_tab;                       // source degree the tables were built for
+    int pt;                          // effective target degree (<= pt_tab); = p for M2L
+    int pt_tab;                      // target degree the tables were built for
+    int overwrite;                   // store instead of add (every block in one item)
     int n_items;
     const int4* items;               // (block, first group, end group, atomic add)
     const int* blk_targets;          // blocks x TB target boxes (-1: empty slot)

@@ -1,5 +1,7 @@
 // M2L on matrix cores: all pairs of a block of target boxes that share a translation
-// operator are contracted together.
+// operator are contracted together.  The same kernel runs M2M (targets: parents, sources:
+// their children, source degree = child level's) and L2L (targets: children of one octant,
+// source: their parents) with source and target degrees that differ.
 //
 // A workgroup (4 waves) owns TB target boxes of one level that have the same position
 // (octant) inside their parents, so their V lists use the same set of displacements.
@@ -34,6 +36,7 @@ namespace hfmm {
 size_t m2l_block_lds_bytes(int p, int tb, int g, int precision_bytes) {
     return ((size_t)m2l_lds_x_elems(p, 2 * tb * g) + 2 * (size_t)g * (p + 1)) * precision_bytes;
 }
+// p: the larger of the source and target table degrees
 
 namespace {
 
@@ -97,10 +100,14 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     constexpr int NCT = G * NCTD;           // column tiles of the group
     constexpr int LDX = G * NCOLD + 1;      // odd row stride in LDS
     extern __shared__ char smem_raw[];
+    // source degree p (multipoles or parent locals) and target degree pt; equal for M2L
     const int p = a.p, P1 = p + 1, nc = P1 * P1;
-    const int spare = (a.p_tab + 1) * (a.p_tab + 1);   // scratch row for out-of-block stores
+    const int pt = a.pt, PT1 = pt + 1, nct = PT1 * PT1;
+    const int ptab = max(a.p_tab, a.pt_tab);           // LDS rows are sized for this degree
+    const int PH = max(P1, PT1);                        // phases per direction
+    const int spare = (ptab + 1) * (ptab + 1);          // scratch row for out-of-block stores
     T* X = (T*)smem_raw;
-    C2* Ps = (C2*)(X + m2l_lds_x_elems(a.p_tab, G * NCOLD));   // G x (p+1) phases
+    C2* Ps = (C2*)(X + m2l_lds_x_elems(ptab, G * NCOLD));   // G x PH phases
     // the wave index is uniform across a wavefront: keep it (and all unit bookkeeping
     // derived from it) in scalar registers so that the unit branches are scalar
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -122,7 +129,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     // degrees are then finite (stale results of earlier groups) or zero, and their
     // products with the zero padding of A vanish
     const int zero_row = spare + 1;   // the 16 rows after the scratch row stay zero
-    for (int i = threadIdx.x; i < m2l_lds_x_rows(a.p_tab) * LDX; i += 256) X[i] = (T)0;
+    for (int i = threadIdx.x; i < m2l_lds_x_rows(ptab) * LDX; i += 256) X[i] = (T)0;
     __syncthreads();
 
     for (int e = item.y; e < item.z; ++e) {
@@ -134,8 +141,8 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
         const int* soff = a.grp_off + (size_t)e * G * TB;   // -1: no source (or no direction)
 
         // stage 0: gather + e^{i m phi}
-        for (int i = threadIdx.x; i < G * P1; i += 256) {
-            const int g = i / P1, m = i - g * P1;
+        for (int i = threadIdx.x; i < G * PH; i += 256) {
+            const int g = i / PH, m = i - g * PH;
             const unsigned short d = gd[g];
             C2 v;
             v.x = 1;
@@ -173,6 +180,13 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                     xr[2 * j] = raw[j].x * pm.x - raw[j].y * pm.y;
                     xr[2 * j + 1] = raw[j].x * pm.y + raw[j].y * pm.x;
                 }
+            }
+            // target degrees above the source degree start from zero (orders |m| > p
+            // receive nothing in stage 2)
+            for (int c = nc + (wave % WPD) * 64 + lane; c < nct; c += 64 * WPD) {
+                T* xr = X + c * LDX + g * NCOLD;
+#pragma unroll
+                for (int j = 0; j < NCOLD; ++j) xr[j] = (T)0;
             }
         }
         __syncthreads();
@@ -234,11 +248,11 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
         const T sgn1 = (li & 1) ? (T)1 : (T)-1;   // Im(T) x X: (-Im X, Re X) per column pair
         for (int u = wave; u < a.n_units2; u += 4) {
             const int am = a.units2[u];
-            if (am > p) continue;
+            if (am > p || am > pt) continue;
             const int ns = am > 0 ? 2 : 1;                  // orders +am and -am
-            const int len = P1 - am;
-            const int ntab = m2l_tiles(a.p_tab + 1 - am);   // tiles of the table block
-            const int nt = m2l_tiles(len), nks = (len + 3) >> 2;
+            const int len = P1 - am, lent = PT1 - am;       // source and target block sizes
+            const int ntab = m2l_tiles(a.p_tab + 1 - am);   // k groups of the table block
+            const int nt = m2l_tiles(lent), nkg = m2l_tiles(len), nks = (len + 3) >> 2;
             const T* A = Tz + a.coax_off[am];
             V4 c4[2][kMaxTiles2][NCT];
 #pragma unroll
@@ -247,7 +261,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                 for (int t = 0; t < kMaxTiles2; ++t)
 #pragma unroll
                     for (int ct = 0; ct < NCT; ++ct) c4[sg][t][ct] = V4{0, 0, 0, 0};
-            for (int kg = 0; kg < nt; ++kg) {
+            for (int kg = 0; kg < nkg; ++kg) {
                 V4 are[kMaxTiles2], aim[kMaxTiles2];
 #pragma unroll
                 for (int t = 0; t < kMaxTiles2; ++t)
@@ -298,7 +312,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                             for (int r = 0; r < 4; ++r) {
                                 const int row = t * 16 + Mma<T>::row(lk, r);
                                 const int nn = am + row;
-                                const int dst = (row < len ? nn * nn + nn + m : spare) * LDX;
+                                const int dst = (row < lent ? nn * nn + nn + m : spare) * LDX;
 #pragma unroll
                                 for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[sg][t][ct][r];
                             }
@@ -311,7 +325,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
         for (int uu = 0; uu < MAXU; ++uu) {
             const int u = wave + 4 * uu;
-            if (u < a.n_units3 && (a.units3[u] & 255) <= p) {
+            if (u < a.n_units3 && (a.units3[u] & 255) <= pt) {
                 const int code = a.units3[u];
                 const int n = code & 255, mt = code >> 8;
                 const int w = 2 * n + 1, base = n * n;
@@ -341,11 +355,11 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int m = mt * 16 + Mma<T>::row(lk, r) - n;
-                    const int am = min(m < 0 ? -m : m, p);
+                    const int am = min(m < 0 ? -m : m, PH - 1);
                     const bool flip = (m < 0) != ((li & 1) != 0);
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
-                        const C2 pm = Ps[g * P1 + am];
+                        const C2 pm = Ps[g * PH + am];
                         const T cs = pm.x, sn = flip ? -pm.y : pm.y;
 #pragma unroll
                         for (int cd = 0; cd < NCTD; ++cd) {
@@ -363,7 +377,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
     for (int uu = 0; uu < MAXU; ++uu) {
         const int u = wave + 4 * uu;
-        if (u < a.n_units3 && (a.units3[u] & 255) <= p) {
+        if (u < a.n_units3 && (a.units3[u] & 255) <= pt) {
             const int code = a.units3[u];
             const int n = code & 255, mt = code >> 8;
 #pragma unroll
@@ -379,6 +393,8 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                             double* x = dst + 2 * (n * n + row);
                             if (item.w)
                                 atomicAdd(x, (double)acc[uu][cd][r]);
+                            else if (a.overwrite)
+                                *x = (double)acc[uu][cd][r];
                             else
                                 *x += (double)acc[uu][cd][r];
                         }
@@ -469,7 +485,7 @@ void prepare_m2l_block(int p, int tb, int g, int fp32) { dispatch(nullptr, p, tb
 
 void launch_m2l_block(const M2LBlockArgs& a, int tb, int g, int fp32, hipStream_t s) {
     if (a.n_items <= 0) return;
-    dispatch(&a, a.p_tab, tb, g, fp32, s);
+    dispatch(&a, std::max(a.p_tab, a.pt_tab), tb, g, fp32, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) fail(-3, std::string("kernel launch failed (m2l_block): ") + hipGetErrorString(e));
 }

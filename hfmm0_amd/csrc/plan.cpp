@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -44,6 +45,8 @@ struct M2LBlockLevel {
     bool use = false;
     int tb = 0;                    // target boxes per workgroup
     int grp = 1;                   // directions per group
+    int ps = 0, pt = 0;            // source and target table degrees
+    int src_level = 0, tgt_level = 0;
     int n_items = 0;
     int4* items = nullptr;
     int* targets = nullptr;
@@ -110,6 +113,7 @@ struct hfmm_plan {
     DispInfo* d_dinfo = nullptr;
     XlateLaunch m2l[kMaxLevels], m2m[kMaxLevels], l2l[kMaxLevels];
     M2LBlockLevel m2lb[kMaxLevels];
+    M2LBlockLevel m2mb[kMaxLevels], l2lb[kMaxLevels];   // M2M / L2L on matrix cores (FP64)
     int m2l_kernel = 0;
     size_t max_lds = 65536;
     // rotation blocks as MFMA A fragments (E^T for the forward, E for the back rotation)
@@ -174,6 +178,99 @@ void check_device() {
 
 int octant(uint64_t code) { return (int)(code & 7); }
 
+// Units of work of a translation from degree ps to degree pt, costliest first (dealt
+// round-robin to the 4 waves): forward rotations n <= ps, coaxial orders |m| <= min,
+// back-rotation row tiles (n, mt) with n <= pt.
+void make_units(int ps, int pt, std::vector<int>& units1, std::vector<int>& units2, std::vector<int>& units3) {
+    std::vector<std::pair<int, int>> u1, u2, u3;   // (cost, code)
+    for (int n = 0; n <= ps; ++n) {
+        const int w = 2 * n + 1;
+        u1.push_back({m2l_tiles(w) * ((w + 3) / 4), n});
+    }
+    for (int n = 0; n <= pt; ++n) {
+        const int w = 2 * n + 1, nt = m2l_tiles(w), ks = (w + 3) / 4;
+        for (int mt = 0; mt < nt; ++mt) u3.push_back({ks, n | (mt << 8)});
+    }
+    for (int m = 0; m <= std::min(ps, pt); ++m) {
+        const int lens = ps + 1 - m, lent = pt + 1 - m;
+        u2.push_back({(m ? 2 : 1) * m2l_tiles(lent) * ((lens + 3) / 4), m});
+    }
+    auto by_cost = [](const std::pair<int, int>& a, const std::pair<int, int>& b) { return a.first > b.first; };
+    std::stable_sort(u1.begin(), u1.end(), by_cost);
+    std::stable_sort(u2.begin(), u2.end(), by_cost);
+    std::stable_sort(u3.begin(), u3.end(), by_cost);
+    units1.clear();
+    units2.clear();
+    units3.clear();
+    for (auto& x : u1) units1.push_back(x.second);
+    for (auto& x : u2) units2.push_back(x.second);
+    for (auto& x : u3) units3.push_back(x.second);
+}
+
+// Coaxial tables as A fragments: per distance, per order m = 0..min(ps, pt) the packed
+// (pt+1-m) x (ps+1-m) block T[i][k] (row i: target degree m+i, column k: source degree
+// m+k), padded to 16x16 tiles; per (row tile, k group, lane) eight values: Re then Im
+// of j = 0..3.  coff[m] is the offset of order m within one distance.
+int coax_fragments(const std::vector<double>& tabs, int n_dist, int ps, int pt, std::vector<double>& cf,
+                   std::vector<int>& coff) {
+    const int mm = std::min(ps, pt);
+    coff.assign(mm + 2, 0);
+    for (int m = 0; m <= mm; ++m) coff[m + 1] = coff[m] + m2l_tiles(pt + 1 - m) * m2l_tiles(ps + 1 - m) * 64 * 8;
+    const int stride = coff[mm + 1];
+    cf.assign((size_t)n_dist * stride, 0.0);
+    const int csz = coax_size(pt, ps);
+    for (int di = 0; di < n_dist; ++di) {
+        const double* tab = &tabs[(size_t)di * csz * 2];
+        double* dst = &cf[(size_t)di * stride];
+        int off = 0;
+        for (int m = 0; m <= mm; ++m) {
+            const int lent = pt + 1 - m, lens = ps + 1 - m;
+            const int nr = m2l_tiles(lent), nk = m2l_tiles(lens);
+            for (int rt = 0; rt < nr; ++rt)
+                for (int kg = 0; kg < nk; ++kg)
+                    for (int lane = 0; lane < 64; ++lane)
+                        for (int j = 0; j < 4; ++j) {
+                            const int i = 16 * rt + (lane & 15), k = 16 * kg + 4 * j + (lane >> 4);
+                            if (i >= lent || k >= lens) continue;
+                            double* f = dst + coff[m] + ((size_t)(rt * nk + kg) * 64 + lane) * 8;
+                            f[j] = tab[2 * (off + i * lens + k)];
+                            f[4 + j] = tab[2 * (off + i * lens + k) + 1];
+                        }
+            off += lent * lens;
+        }
+    }
+    return stride;
+}
+
+// Work items of target blocks: blocks in space order (the parent region of their first
+// target, then octant), so that the kernel's XCD-aware item mapping gives each XCD a
+// compact region whose source expansions stay in its L2; entry lists split when there
+// are few blocks (if allowed); padded to a multiple of the XCD count with empty items.
+std::vector<int4> make_items(const Tree& t, const std::vector<int>& blk_targets, const std::vector<int>& blk_ptr,
+                             int tb, bool allow_split) {
+    const int nblk = (int)blk_ptr.size() - 1;
+    std::vector<int> border(nblk);
+    for (int b = 0; b < nblk; ++b) border[b] = b;
+    auto region = [&](int b) {
+        const int t0 = blk_targets[(size_t)b * tb];
+        return t0 < 0 ? ~0ull : t.code[t0];
+    };
+    std::stable_sort(border.begin(), border.end(), [&](int x, int y) {
+        const uint64_t rx = region(x) >> 3, ry = region(y) >> 3;
+        return rx != ry ? rx < ry : (region(x) & 7) < (region(y) & 7);
+    });
+    const int nsplit = allow_split ? std::max(1, std::min(16, (2048 + nblk - 1) / std::max(nblk, 1))) : 1;
+    std::vector<int4> items;
+    for (int b : border) {
+        const int e0 = blk_ptr[b], e1 = blk_ptr[b + 1];
+        const int chunk = std::max(1, (e1 - e0 + nsplit - 1) / nsplit);
+        for (int s = e0; s < e1; s += chunk)
+            items.push_back(make_int4(b, s, std::min(e1, s + chunk), nsplit > 1 ? 1 : 0));
+    }
+    while (items.size() % kXcds) items.push_back(make_int4(0, 0, 0, 0));
+    return items;
+}
+
 // Target blocks of the matrix-core M2L: boxes of level L with the same octant, TB at a
 // time in Morton order; per block the displacements present (sorted by rotation and
 // distance) with the source box of every target (-1 if that target lacks it).
@@ -211,25 +308,8 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     for (auto& sh : shapes)
         if (!tb && fits(sh[0], sh[1])) tb = sh[0], grp = sh[1];
     if (!tb) return;
-    // units of work, costliest first (dealt round-robin to the 4 waves)
-    std::vector<std::pair<int, int>> u1, u2, u3;   // (cost, code)
-    for (int n = 0; n <= p; ++n) {
-        const int w = 2 * n + 1, nt = m2l_tiles(w), ks = (w + 3) / 4;
-        u1.push_back({nt * ks, n});
-        for (int mt = 0; mt < nt; ++mt) u3.push_back({ks, n | (mt << 8)});
-    }
-    for (int m = 0; m <= p; ++m) {
-        const int len = p + 1 - m;
-        u2.push_back({(m ? 2 : 1) * m2l_tiles(len) * ((len + 3) / 4), m});
-    }
-    auto by_cost = [](const std::pair<int, int>& a, const std::pair<int, int>& b) { return a.first > b.first; };
-    std::stable_sort(u1.begin(), u1.end(), by_cost);
-    std::stable_sort(u2.begin(), u2.end(), by_cost);
-    std::stable_sort(u3.begin(), u3.end(), by_cost);
     std::vector<int> units1, units2, units3;
-    for (auto& x : u1) units1.push_back(x.second);
-    for (auto& x : u2) units2.push_back(x.second);
-    for (auto& x : u3) units3.push_back(x.second);
+    make_units(p, p, units1, units2, units3);
 
     const Tree& t = P.tree;
     std::vector<int> tpos(t.n_boxes(), -1);     // target ordinal in `targets`
@@ -269,35 +349,13 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
             blk_ptr.push_back((int)(grp_dir.size() / grp));
         }
     }
-    // work items: split the entry lists of the blocks when there are few blocks.  Blocks
-    // are visited in space order (the parent region of their first target, then octant),
-    // so that the kernel's XCD-aware item mapping gives each XCD a compact region whose
-    // source expansions stay in its L2; the list is padded to a multiple of the XCD
-    // count with empty items.
-    const int nblk = (int)blk_ptr.size() - 1;
-    std::vector<int> border(nblk);
-    for (int b = 0; b < nblk; ++b) border[b] = b;
-    auto region = [&](int b) {
-        const int t0 = blk_targets[(size_t)b * tb];
-        return t0 < 0 ? ~0ull : (t.code[t0] >> 3) * 8 + (t.code[t0] & 7);
-    };
-    std::stable_sort(border.begin(), border.end(), [&](int x, int y) {
-        const uint64_t rx = region(x) >> 3, ry = region(y) >> 3;
-        return rx != ry ? rx < ry : (region(x) & 7) < (region(y) & 7);
-    });
-    const int nsplit = std::max(1, std::min(16, (2048 + nblk - 1) / std::max(nblk, 1)));
-    std::vector<int4> items;
-    for (int b : border) {
-        const int e0 = blk_ptr[b], e1 = blk_ptr[b + 1];
-        const int chunk = (e1 - e0 + nsplit - 1) / nsplit;
-        for (int s = e0; s < e1; s += std::max(chunk, 1))
-            items.push_back(make_int4(b, s, std::min(e1, s + std::max(chunk, 1)), nsplit > 1 ? 1 : 0));
-    }
-    while (items.size() % kXcds) items.push_back(make_int4(0, 0, 0, 0));
+    const std::vector<int4> items = make_items(t, blk_targets, blk_ptr, tb, true);
     M2LBlockLevel& B = P.m2lb[L];
     B.use = true;
     B.tb = tb;
     B.grp = grp;
+    B.ps = B.pt = p;
+    B.src_level = B.tgt_level = L;
     B.n_items = (int)items.size();
     B.items = P.upload(items);
     B.targets = P.upload(blk_targets);
@@ -309,33 +367,9 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     B.n_units2 = (int)units2.size();
     B.units3 = P.upload(units3);
     B.n_units3 = (int)units3.size();
-    // coaxial tables as A fragments: per distance, per order m = 0..p the (p+1-m)^2 block
-    // T[i][k] (row i: target degree m+i, column k: source degree m+k), padded to 16x16
-    // tiles; per (row tile, k group, lane) eight values: Re then Im of j = 0..3
-    std::vector<int> coff(p + 2, 0);
-    for (int m = 0; m <= p; ++m) coff[m + 1] = coff[m] + m2l_tiles(p + 1 - m) * m2l_tiles(p + 1 - m) * 64 * 8;
-    B.coax_stride = coff[p + 1];
-    std::vector<double> cf((size_t)n_dist * B.coax_stride, 0.0);
-    const int csz = coax_size(p, p);
-    for (int di = 0; di < n_dist; ++di) {
-        const double* tab = &coax_d[(size_t)di * csz * 2];
-        double* dst = &cf[(size_t)di * B.coax_stride];
-        int off = 0;
-        for (int m = 0; m <= p; ++m) {
-            const int len = p + 1 - m, nt = m2l_tiles(len);
-            for (int rt = 0; rt < nt; ++rt)
-                for (int kg = 0; kg < nt; ++kg)
-                    for (int lane = 0; lane < 64; ++lane)
-                        for (int j = 0; j < 4; ++j) {
-                            const int i = 16 * rt + (lane & 15), k = 16 * kg + 4 * j + (lane >> 4);
-                            if (i >= len || k >= len) continue;
-                            double* f = dst + coff[m] + ((size_t)(rt * nt + kg) * 64 + lane) * 8;
-                            f[j] = tab[2 * (off + i * len + k)];
-                            f[4 + j] = tab[2 * (off + i * len + k) + 1];
-                        }
-            off += len * len;
-        }
-    }
+    std::vector<double> cf;
+    std::vector<int> coff;
+    B.coax_stride = coax_fragments(coax_d, n_dist, p, p, cf, coff);
     if (fp32) {
         std::vector<float> c32(cf.begin(), cf.end());
         B.coax = P.upload(c32);
@@ -344,6 +378,98 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     }
     B.coax_off = P.upload(coff);
     prepare_m2l_block(p, tb, grp, fp32);
+}
+
+// M2M (kind 1: the parents of level L from their children) and L2L (kind 2: the children
+// at level L+1 from their parents) on the matrix-core block kernel, in FP64 at every
+// precision (these expansions feed every coarser or finer level).  Blocks of 8 targets,
+// one direction per group: an M2M block runs one group per child octant present among
+// its parents, every target receiving all of its children within the one work item, so
+// the result is stored (no clear of the parent multipoles is needed); an L2L block holds
+// 8 children of one octant and a single group (their parents' locals), added to what
+// M2L left there.  tab: the coaxial table (pt+1-m) x (ps+1-m) per m of this level pair.
+void build_tree_blocks(hfmm_plan& P, int L, int kind, const std::vector<double>& tab) {
+    if (P.m2l_kernel == 1 || !P.d_rf1) return;
+    const Tree& t = P.tree;
+    const int ps = kind == 1 ? P.p_level[L + 1] : P.p_level[L];
+    const int pt = kind == 1 ? P.p_level[L] : P.p_level[L + 1];
+    const int pm = std::max(ps, pt);
+    if (pm > kM2LBlockMaxP) return;
+    const int tb = 8, grp = 1;
+    if (m2l_block_lds_bytes(pm, tb, grp, 8) > P.max_lds) return;
+    std::vector<int> blk_targets, blk_ptr{0}, grp_off;
+    std::vector<unsigned short> grp_dir;
+    if (kind == 1) {
+        // parents with children, in level order, and their child per octant
+        std::vector<int> parents;
+        std::vector<std::array<int, 8>> child;
+        for (int c = t.level_start[L + 1]; c < t.level_start[L + 2]; ++c) {
+            const int par = t.parent[c];
+            if (parents.empty() || parents.back() != par) {
+                parents.push_back(par);
+                child.push_back({-1, -1, -1, -1, -1, -1, -1, -1});
+            }
+            child.back()[octant(t.code[c])] = c;
+        }
+        for (size_t b0 = 0; b0 < parents.size(); b0 += tb) {
+            for (int j = 0; j < tb; ++j) blk_targets.push_back(b0 + j < parents.size() ? parents[b0 + j] : -1);
+            for (int o = 0; o < 8; ++o) {
+                bool any = false;
+                for (int j = 0; j < tb && b0 + j < parents.size(); ++j) any |= child[b0 + j][o] >= 0;
+                if (!any) continue;
+                grp_dir.push_back((unsigned short)(kDirM2M + o));
+                for (int j = 0; j < tb; ++j) {
+                    const int c = b0 + j < parents.size() ? child[b0 + j][o] : -1;
+                    grp_off.push_back(c >= 0 ? (int)P.exp_off[c] : -1);
+                }
+            }
+            blk_ptr.push_back((int)grp_dir.size());
+        }
+    } else {
+        for (int o = 0; o < 8; ++o) {
+            std::vector<int> kids;
+            for (int c = t.level_start[L + 1]; c < t.level_start[L + 2]; ++c)
+                if (octant(t.code[c]) == o) kids.push_back(c);
+            for (size_t b0 = 0; b0 < kids.size(); b0 += tb) {
+                grp_dir.push_back((unsigned short)(kDirL2L + o));
+                for (int j = 0; j < tb; ++j) {
+                    const int c = b0 + j < kids.size() ? kids[b0 + j] : -1;
+                    blk_targets.push_back(c);
+                    grp_off.push_back(c >= 0 ? (int)P.exp_off[t.parent[c]] : -1);
+                }
+                blk_ptr.push_back((int)grp_dir.size());
+            }
+        }
+    }
+    if (blk_ptr.size() < 2) return;
+    const std::vector<int4> items = make_items(t, blk_targets, blk_ptr, tb, false);
+    std::vector<int> units1, units2, units3;
+    make_units(ps, pt, units1, units2, units3);
+    M2LBlockLevel& B = kind == 1 ? P.m2mb[L] : P.l2lb[L];
+    B.use = true;
+    B.tb = tb;
+    B.grp = grp;
+    B.ps = ps;
+    B.pt = pt;
+    B.src_level = kind == 1 ? L + 1 : L;
+    B.tgt_level = kind == 1 ? L : L + 1;
+    B.n_items = (int)items.size();
+    B.items = P.upload(items);
+    B.targets = P.upload(blk_targets);
+    B.dir = P.upload(grp_dir);
+    B.off = P.upload(grp_off);
+    B.units1 = P.upload(units1);
+    B.n_units1 = (int)units1.size();
+    B.units2 = P.upload(units2);
+    B.n_units2 = (int)units2.size();
+    B.units3 = P.upload(units3);
+    B.n_units3 = (int)units3.size();
+    std::vector<double> cf;
+    std::vector<int> coff;
+    B.coax_stride = coax_fragments(tab, 1, ps, pt, cf, coff);
+    B.coax = P.upload(cf);
+    B.coax_off = P.upload(coff);
+    prepare_m2l_block(pm, tb, grp, 0);
 }
 
 void build_tables(hfmm_plan& P) {
@@ -417,10 +543,9 @@ void build_tables(hfmm_plan& P) {
         if (P.precision == HFMM_PREC_MIXED) {
             P.d_rf1_32 = P.upload(std::vector<float>(f1.begin(), f1.end()));
             P.d_rf3_32 = P.upload(std::vector<float>(f3.begin(), f3.end()));
-        } else {
-            P.d_rf1 = P.upload(f1);
-            P.d_rf3 = P.upload(f3);
         }
+        P.d_rf1 = P.upload(f1);   // FP64: M2L at FP64, M2M and L2L at every precision
+        P.d_rf3 = P.upload(f3);
     }
     std::vector<double2> phase((size_t)kNumDirs * (pmax + 1));
     for (int d = 0; d < kNumDirs; ++d)
@@ -512,6 +637,7 @@ void build_tables(hfmm_plan& P) {
         std::vector<double> cx((size_t)A.coax_stride * 2);
         coax_table(1, G, P.k, P.R[L + 1], P.R[L + 1], P.R[L], A.ps, A.pt, cx.data());
         A.coax = (double2*)P.upload(cx);
+        build_tree_blocks(P, L, 1, cx);
         XlateLaunch& B = P.l2l[L];
         B.n_targets = (int)ctargets.size();
         B.targets = P.upload(ctargets);
@@ -526,6 +652,7 @@ void build_tables(hfmm_plan& P) {
         std::vector<double> cy((size_t)B.coax_stride * 2);
         coax_table(2, G, P.k, P.R[L + 1], P.R[L], P.R[L + 1], B.ps, B.pt, cy.data());
         B.coax = (double2*)P.upload(cy);
+        build_tree_blocks(P, L, 2, cy);
     }
 }
 
@@ -703,6 +830,41 @@ XlateArgs xargs(const hfmm_plan& P, const XlateLaunch& X, const double2* srcbuf,
     return a;
 }
 
+M2LBlockArgs block_args(const hfmm_plan& P, const M2LBlockLevel& B, bool fp32, const void* src, double2* dst,
+                        int overwrite) {
+    M2LBlockArgs a;
+    a.p = P.p_eff[B.src_level];
+    a.p_tab = B.ps;
+    a.pt = P.p_eff[B.tgt_level];
+    a.pt_tab = B.pt;
+    a.overwrite = overwrite;
+    a.n_items = B.n_items;
+    a.items = B.items;
+    a.blk_targets = B.targets;
+    a.grp_dir = B.dir;
+    a.grp_off = B.off;
+    a.units1 = B.units1;
+    a.n_units1 = B.n_units1;
+    a.units2 = B.units2;
+    a.n_units2 = B.n_units2;
+    a.units3 = B.units3;
+    a.n_units3 = B.n_units3;
+    a.rot1 = fp32 ? (const void*)P.d_rf1_32 : (const void*)P.d_rf1;
+    a.rot3 = fp32 ? (const void*)P.d_rf3_32 : (const void*)P.d_rf3;
+    a.rot_off = P.d_rf_off;
+    a.rot_stride = P.rf_stride;
+    a.phase = fp32 ? (const void*)P.d_phase32 : (const void*)P.d_phase;
+    a.phase_stride = P.pmax + 1;
+    a.dinfo = P.d_dinfo;
+    a.coax = B.coax;
+    a.coax_off = B.coax_off;
+    a.coax_stride = B.coax_stride;
+    a.M = src;
+    a.L = dst;
+    a.exp_off = P.d_expoff;
+    return a;
+}
+
 // The device sequence of one evaluate.  ev (optional, 8 entries) brackets the phases;
 // lev (optional, 2 per level) brackets the M2L launch of every level.
 // field (optional): E = -grad phi, 3 complex values per point in input order.
@@ -720,8 +882,12 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
     const DevLevels lv = P.levels();
     launch_p2m(P.d_far_leaves, P.n_far_leaves, P.p2m_chunk, bx, lv, P.dX, P.dY, P.dZ, P.d_q, P.k, P.p_leafmax, P.dM, s);
     mark(2);
-    for (int L = P.depth - 1; L >= 2; --L)
-        if (P.m2m[L].n_targets) launch_xlate(xargs(P, P.m2m[L], P.dM, P.dM, 0), P.m2m[L].n_targets, s);
+    for (int L = P.depth - 1; L >= 2; --L) {
+        if (P.m2mb[L].use)
+            launch_m2l_block(block_args(P, P.m2mb[L], false, P.dM, P.dM, 1), P.m2mb[L].tb, P.m2mb[L].grp, 0, s);
+        else if (P.m2m[L].n_targets)
+            launch_xlate(xargs(P, P.m2m[L], P.dM, P.dM, 0), P.m2m[L].n_targets, s);
+    }
     mark(3);
     const bool fp32 = P.precision == HFMM_PREC_MIXED;
     bool converted = false;
@@ -738,40 +904,18 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
             launch_to_float(P.dM, P.dM32, P.n_coef, s);
             converted = true;
         }
-        M2LBlockArgs a;
-        a.p = P.p_eff[L];
-        a.p_tab = P.p_level[L];
-        a.n_items = B.n_items;
-        a.items = B.items;
-        a.blk_targets = B.targets;
-        a.grp_dir = B.dir;
-        a.grp_off = B.off;
-        a.units1 = B.units1;
-        a.n_units1 = B.n_units1;
-        a.units2 = B.units2;
-        a.n_units2 = B.n_units2;
-        a.units3 = B.units3;
-        a.n_units3 = B.n_units3;
-        a.rot1 = fp32 ? (const void*)P.d_rf1_32 : (const void*)P.d_rf1;
-        a.rot3 = fp32 ? (const void*)P.d_rf3_32 : (const void*)P.d_rf3;
-        a.rot_off = P.d_rf_off;
-        a.rot_stride = P.rf_stride;
-        a.phase = fp32 ? (const void*)P.d_phase32 : (const void*)P.d_phase;
-        a.phase_stride = P.pmax + 1;
-        a.dinfo = P.d_dinfo;
-        a.coax = B.coax;
-        a.coax_off = B.coax_off;
-        a.coax_stride = B.coax_stride;
-        a.M = fp32 ? (const void*)P.dM32 : (const void*)P.dM;
-        a.L = P.dL;
-        a.exp_off = P.d_expoff;
+        const M2LBlockArgs a = block_args(P, B, fp32, fp32 ? (const void*)P.dM32 : (const void*)P.dM, P.dL, 0);
         if (lev) HIPCHK(hipEventRecord(lev[2 * L], s));
         launch_m2l_block(a, B.tb, B.grp, fp32, s);
         if (lev) HIPCHK(hipEventRecord(lev[2 * L + 1], s));
     }
     mark(4);
-    for (int L = 2; L < P.depth; ++L)
-        if (P.l2l[L].n_targets) launch_xlate(xargs(P, P.l2l[L], P.dL, P.dL, 1), P.l2l[L].n_targets, s);
+    for (int L = 2; L < P.depth; ++L) {
+        if (P.l2lb[L].use)
+            launch_m2l_block(block_args(P, P.l2lb[L], false, P.dL, P.dL, 0), P.l2lb[L].tb, P.l2lb[L].grp, 0, s);
+        else if (P.l2l[L].n_targets)
+            launch_xlate(xargs(P, P.l2l[L], P.dL, P.dL, 1), P.l2l[L].n_targets, s);
+    }
     mark(5);
     const int out_mode = (P.conj ? 1 : 0) | ((P.k == 0.0 && real_charges(q_type)) ? 2 : 0);
     if (field) {
