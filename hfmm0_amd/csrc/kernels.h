@@ -102,6 +102,9 @@ __host__ __device__ constexpr int m2l_acc_regs(int maxu, int tb, int elem_bytes)
 // that receives the stores of rows outside a block, and 16 zero rows that the reads of
 // the last block's padding run into; padded to 16 bytes
 __host__ __device__ constexpr int m2l_lds_x_rows(int p) { return (p + 1) * (p + 1) + 17; }
+// phase table of the M2L block kernel: entries per (lane parity, direction), indexed by
+// the signed order m + 8 KT (KT = m2l_tiles(2 p + 1)); covers -p <= m <= p + 15
+__host__ __device__ constexpr int m2l_phase_len(int p) { return 16 * m2l_tiles(2 * p + 1) + 16; }
 __host__ __device__ constexpr int m2l_lds_x_elems(int p, int ncol) {
     return (m2l_lds_x_rows(p) * (ncol + 1) + 3) & ~3;
 }

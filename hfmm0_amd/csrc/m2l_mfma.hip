@@ -21,10 +21,12 @@
 // the host stores in fragment order, zero-padded to whole 16x16 tiles, so a lane loads
 // the A values of four k-steps with one 16-byte (FP32) or two 16-byte (FP64) loads and
 // no bounds test; each A fragment is applied to every 16-column tile of the group.  B
-// operands are read from LDS with row indices clamped into the block (their products
-// with the zero padding of A vanish); rows outside a block are stored to a scratch row.  The stage-3
+// rows read past a block hold finite values (zero, or results of earlier groups) whose
+// products with the zero padding of A vanish; source degrees above the evaluated one read
+// a zero row; results of rows outside a block are stored to a scratch row.  The stage-3
 // accumulators stay in registers across all groups and are added to the FP64 local
-// expansions once at the end.
+// expansions once at the end.  Work units (degrees, orders, row tiles) are dealt to the
+// four waves by a host-built list that balances their MFMA counts.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -32,9 +34,10 @@
 
 namespace hfmm {
 
-// LDS: the expansion block (with its scratch row), then G * (p+1) complex phases
+// LDS: the expansion block (with its scratch row), then the phase table: 2 lane parities
+// x G directions x m2l_phase_len(p) complex entries
 size_t m2l_block_lds_bytes(int p, int tb, int g, int precision_bytes) {
-    return ((size_t)m2l_lds_x_elems(p, 2 * tb * g) + 2 * (size_t)g * (p + 1)) * precision_bytes;
+    return ((size_t)m2l_lds_x_elems(p, 2 * tb * g) + 4 * (size_t)g * m2l_phase_len(p)) * precision_bytes;
 }
 // p: the larger of the source and target table degrees
 
@@ -50,7 +53,7 @@ struct Mma<float> {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
     // output element r of lane group lk (= lane / 16) is row 4 lk + r of the 16x16 tile
-    __device__ static inline int row(int lk, int r) { return 4 * lk + r; }
+    __host__ __device__ static constexpr int row(int lk, int r) { return 4 * lk + r; }
     __device__ static inline v4 load4(const float* p) { return *(const v4*)p; }
     // value of the neighbouring lane (lane ^ 1): quad permutation (1, 0, 3, 2)
     __device__ static inline float swap_pair(float v) {
@@ -65,7 +68,7 @@ struct Mma<double> {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
     // output element r of lane group lk (= lane / 16) is row lk + 4 r of the 16x16 tile
-    __device__ static inline int row(int lk, int r) { return lk + 4 * r; }
+    __host__ __device__ static constexpr int row(int lk, int r) { return lk + 4 * r; }
     __device__ static inline v4 load4(const double* p) {
         const double2 a = *(const double2*)p, b = *(const double2*)(p + 2);
         return v4{a.x, a.y, b.x, b.y};
@@ -78,11 +81,11 @@ struct Mma<double> {
     }
 };
 
+// coefficient index c = n^2 + n + m -> (n, m).  sqrt(c + 1/2) lies at least 1/(4(n+1)) away
+// from an integer for 0 <= c < 2^16, so the hardware square root (1 ulp) truncates to n.
 __device__ inline void decode_nm(int c, int& n, int& m) {
-    n = (int)sqrtf((float)c + 0.5f);
-    if ((n + 1) * (n + 1) <= c) ++n;
-    if (n * n > c) --n;
-    m = c - n * n - n;
+    n = (int)__builtin_amdgcn_sqrtf((float)c + 0.5f);
+    m = c - __umul24(n, n + 1);
 }
 
 // Work item: groups [g0, g1) of one target block.  When a block is split over several
@@ -99,6 +102,11 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     constexpr int NCTD = NCOLD / 16;        // 16-wide column tiles of one direction
     constexpr int NCT = G * NCTD;           // column tiles of the group
     constexpr int LDX = G * NCOLD + 1;      // odd row stride in LDS
+    // phase table: per lane parity and direction, entries for the signed order
+    // m = -8 KT .. 8 KT + 15 at index m + MOFF: (cos m phi, +-sin m phi), the sine negated
+    // for odd (imaginary) columns; orders |m| >= PH hold zero
+    constexpr int TL = 16 * KT + 16, MOFF = 8 * KT;
+    constexpr int RSTEP = Mma<T>::row(0, 1) - Mma<T>::row(0, 0);   // rows between output registers
     extern __shared__ char smem_raw[];
     // source degree p (multipoles or parent locals) and target degree pt; equal for M2L
     const int p = a.p, P1 = p + 1, nc = P1 * P1;
@@ -107,11 +115,14 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     const int PH = max(P1, PT1);                        // phases per direction
     const int spare = (ptab + 1) * (ptab + 1);          // scratch row for out-of-block stores
     T* X = (T*)smem_raw;
-    C2* Ps = (C2*)(X + m2l_lds_x_elems(ptab, G * NCOLD));   // G x PH phases
+    C2* Ps = (C2*)(X + m2l_lds_x_elems(ptab, G * NCOLD));   // [parity][G][TL]
     // the wave index is uniform across a wavefront: keep it (and all unit bookkeeping
     // derived from it) in scalar registers so that the unit branches are scalar
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int li = lane & 15, lk = lane >> 4;
+    // this lane's phase entries of stage 3: add (16 mt - n) for a unit (n, mt) and
+    // g TL + r RSTEP for direction g and output register r
+    const C2* ps_lane = Ps + (li & 1) * G * TL + Mma<T>::row(lk, 0) + MOFF;
     // workgroup i runs on XCD i % 8: give XCD x the contiguous item range [x n/8, (x+1) n/8)
     // (the host pads the item list to a multiple of 8 and orders it in space)
     const int per_xcd = a.n_items / kXcds;
@@ -140,14 +151,19 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
         const T* Tz = (const T*)a.coax + (size_t)di.dist * a.coax_stride;
         const int* soff = a.grp_off + (size_t)e * G * TB;   // -1: no source (or no direction)
 
-        // stage 0: gather + e^{i m phi}
-        for (int i = threadIdx.x; i < G * PH; i += 256) {
-            const int g = i / PH, m = i - g * PH;
+        // stage 0: phase table, gather + e^{i m phi}
+        for (int i = threadIdx.x; i < 2 * G * TL; i += 256) {
+            const int par = i / (G * TL), g = (i / TL) % G, m = i % TL - MOFF;
+            const int am = m < 0 ? -m : m;
             const unsigned short d = gd[g];
             C2 v;
-            v.x = 1;
+            v.x = 0;
             v.y = 0;
-            if (d != 0xFFFF) v = ((const C2*)a.phase)[(size_t)d * a.phase_stride + m];
+            if (am < PH) {
+                v.x = 1;
+                if (d != 0xFFFF) v = ((const C2*)a.phase)[(size_t)d * a.phase_stride + am];
+                if ((m < 0) != (par != 0)) v.y = -v.y;
+            }
             Ps[i] = v;
         }
         // every wave gathers for one direction (uniform source offsets, scalar loads),
@@ -189,28 +205,27 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                 for (int j = 0; j < NCOLD; ++j) xr[j] = (T)0;
             }
         }
-        __syncthreads();
-
         // stage 1: X_n <- E_n^T X_n in place.  A unit is one degree n with all its row and
-        // column tiles: the A fragments of a k group are loaded once and applied to every
-        // column tile; the wave writes back only after all products are formed, and no
-        // other wave touches the rows of degree n.
-        for (int u = wave; u < a.n_units1; u += 4) {
-            const int n = a.units1[u];
-            if (n > p) continue;
-            const int w = 2 * n + 1, base = n * n;
-            const int nt = m2l_tiles(w), nks = (w + 3) >> 2;
-            const T* A = R1 + a.rot_off[n];
-            V4 c4[kMaxTiles][NCT];
-#pragma unroll
-            for (int mt = 0; mt < kMaxTiles; ++mt)
-#pragma unroll
-                for (int ct = 0; ct < NCT; ++ct) c4[mt][ct] = V4{0, 0, 0, 0};
-            for (int kg = 0; kg < nt; ++kg) {
-                V4 av[kMaxTiles];
+        // column tiles: the A fragments of a k group are applied to every column tile; the
+        // wave writes back only after all products of the unit are formed, and no other
+        // wave touches the rows of degree n.  Steps: (unit, k group).
+        {
+            V4 a1[kMaxTiles], c4[kMaxTiles][NCT];
+            auto load1 = [&](int n, int kg) {
+                const int nt = m2l_tiles(2 * n + 1);
+                const T* A = R1 + a.rot_off[n];
 #pragma unroll
                 for (int mt = 0; mt < kMaxTiles; ++mt)
-                    if (mt < nt) av[mt] = Mma<T>::load4(A + ((size_t)(mt * nt + kg) * 64 + lane) * 4);
+                    if (mt < nt) a1[mt] = Mma<T>::load4(A + ((size_t)(mt * nt + kg) * 64 + lane) * 4);
+            };
+            auto zero1 = [&]() {
+#pragma unroll
+                for (int mt = 0; mt < kMaxTiles; ++mt)
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct) c4[mt][ct] = V4{0, 0, 0, 0};
+            };
+            auto mma1 = [&](int n, int kg, const V4 (&av)[kMaxTiles]) {
+                const int w = 2 * n + 1, base = n * n, nt = m2l_tiles(w), nks = (w + 3) >> 2;
                 T bv[4][NCT];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
@@ -227,64 +242,87 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
                                 for (int ct = 0; ct < NCT; ++ct)
                                     c4[mt][ct] = Mma<T>::mma(av[mt][j], bv[j][ct], c4[mt][ct]);
+            };
+            auto store1 = [&](int n) {
+                const int w = 2 * n + 1, base = n * n, nt = m2l_tiles(w);
+#pragma unroll
+                for (int mt = 0; mt < kMaxTiles; ++mt)
+                    if (mt < nt)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = mt * 16 + Mma<T>::row(lk, r);
+                            const int dst = __umul24(row < w ? base + row : spare, LDX);
+#pragma unroll
+                            for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[mt][ct][r];
+                        }
+            };
+            __syncthreads();
+            for (int u = wave; u < a.n_units1; u += 4) {
+                const int n = a.units1[u];
+                if (n > p) continue;
+                const int nt = m2l_tiles(2 * n + 1);
+                zero1();
+                for (int kg = 0; kg < nt; ++kg) {
+                    load1(n, kg);
+                    mma1(n, kg, a1);
+                }
+                store1(n);
             }
-#pragma unroll
-            for (int mt = 0; mt < kMaxTiles; ++mt)
-                if (mt < nt)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = mt * 16 + Mma<T>::row(lk, r);
-                        const int dst = (row < w ? base + row : spare) * LDX;
-#pragma unroll
-                        for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[mt][ct][r];
-                    }
         }
-        __syncthreads();
 
         // stage 2: X^m <- T^{|m|} X^m in place.  A unit is one |m| with the orders +|m| and
-        // -|m| (same operator): its A fragments of a k group are loaded once and applied to
-        // both orders and every column tile.  The table blocks are built for p_tab; source
-        // degrees beyond the evaluated p read a zero row.
-        const T sgn1 = (li & 1) ? (T)1 : (T)-1;   // Im(T) x X: (-Im X, Re X) per column pair
-        for (int u = wave; u < a.n_units2; u += 4) {
-            const int am = a.units2[u];
-            if (am > p || am > pt) continue;
-            const int ns = am > 0 ? 2 : 1;                  // orders +am and -am
-            const int len = P1 - am, lent = PT1 - am;       // source and target block sizes
-            const int ntab = m2l_tiles(a.p_tab + 1 - am);   // k groups of the table block
-            const int nt = m2l_tiles(lent), nkg = m2l_tiles(len), nks = (len + 3) >> 2;
-            const T* A = Tz + a.coax_off[am];
-            V4 c4[2][kMaxTiles2][NCT];
-#pragma unroll
-            for (int sg = 0; sg < 2; ++sg)
-#pragma unroll
-                for (int t = 0; t < kMaxTiles2; ++t)
-#pragma unroll
-                    for (int ct = 0; ct < NCT; ++ct) c4[sg][t][ct] = V4{0, 0, 0, 0};
-            for (int kg = 0; kg < nkg; ++kg) {
-                V4 are[kMaxTiles2], aim[kMaxTiles2];
+        // -|m| (same operator): its A fragments of a k group are applied to both orders and
+        // every column tile.  The table blocks are built for p_tab; source degrees beyond
+        // the evaluated p read a zero row.  Steps: (unit, k group).
+        {
+            const T sgn1 = (li & 1) ? (T)1 : (T)-1;   // Im(T) x X: (-Im X, Re X) per column pair
+            V4 a2r[kMaxTiles2], a2i[kMaxTiles2], c4[2][kMaxTiles2][NCT];
+            auto load2 = [&](int am, int kg) {
+                const int ntab = m2l_tiles(a.p_tab + 1 - am), nt = m2l_tiles(PT1 - am);
+                const T* A = Tz + a.coax_off[am];
 #pragma unroll
                 for (int t = 0; t < kMaxTiles2; ++t)
                     if (t < nt) {
                         const T* At = A + ((size_t)(t * ntab + kg) * 64 + lane) * 8;
-                        are[t] = Mma<T>::load4(At);
-                        aim[t] = Mma<T>::load4(At + 4);
+                        a2r[t] = Mma<T>::load4(At);
+                        a2i[t] = Mma<T>::load4(At + 4);
                     }
+            };
+            auto zero2 = [&]() {
+#pragma unroll
+                for (int sg = 0; sg < 2; ++sg)
+#pragma unroll
+                    for (int t = 0; t < kMaxTiles2; ++t)
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ++ct) c4[sg][t][ct] = V4{0, 0, 0, 0};
+            };
+            auto mma2 = [&](int am, int kg, const V4 (&are)[kMaxTiles2], const V4 (&aim)[kMaxTiles2]) {
+                const int ns = am > 0 ? 2 : 1;                  // orders +am and -am
+                const int len = P1 - am, lent = PT1 - am;       // source and target block sizes
+                const int nt = m2l_tiles(lent), nks = (len + 3) >> 2;
+                // source rows of this k group: degree nn_j = am + 16 kg + 4 j + lk, row
+                // nn_j (nn_j + 1) +- am; nn_j (nn_j + 1) = q0 + 8 j nn_0 + 16 j^2 + 4 j
+                const int nn0 = am + 16 * kg + lk;
+                const int q0 = __umul24(nn0, nn0 + 1);
+                int rowp[4], rowm[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int q = q0 + __umul24(8 * j, nn0) + 16 * j * j + 4 * j;
+                    const bool ok = 16 * kg + 4 * j + lk < len;
+                    rowp[j] = __umul24(ok ? q + am : zero_row, LDX);
+                    rowm[j] = __umul24(ok ? q - am : zero_row, LDX);
+                }
 #pragma unroll
                 for (int sg = 0; sg < 2; ++sg) {
                     if (sg < ns) {
-                        const int m = sg ? -am : am;
                         T b0[4][NCT], b1[4][NCT];
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
-                            const int kin = 16 * kg + 4 * j + lk;
-                            const int nn = am + kin;
-                            const int cin = kin < len ? nn * nn + nn + m : zero_row;
+                            const int cin = sg ? rowm[j] : rowp[j];
 #pragma unroll
                             for (int ct = 0; ct < NCT; ++ct) {
-                                const int col = ct * 16 + li;
-                                b0[j][ct] = X[cin * LDX + col];
-                                b1[j][ct] = sgn1 * X[cin * LDX + (col ^ 1)];
+                                b0[j][ct] = X[cin + ct * 16 + li];
+                                b1[j][ct] = sgn1 * X[cin + ct * 16 + (li ^ 1)];
                             }
                         }
 #pragma unroll
@@ -300,71 +338,98 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                                         }
                     }
                 }
-            }
+            };
+            auto store2 = [&](int am) {
+                const int ns = am > 0 ? 2 : 1, lent = PT1 - am, nt = m2l_tiles(lent);
+                // target rows: degree am + row, row = 16 t + row(lk, r)
 #pragma unroll
-            for (int sg = 0; sg < 2; ++sg)
-                if (sg < ns) {
-                    const int m = sg ? -am : am;
+                for (int t = 0; t < kMaxTiles2; ++t)
+                    if (t < nt) {
+                        const int nr0 = am + t * 16 + Mma<T>::row(lk, 0);
+                        const int q0 = __umul24(nr0, nr0 + 1);
 #pragma unroll
-                    for (int t = 0; t < kMaxTiles2; ++t)
-                        if (t < nt)
+                        for (int r = 0; r < 4; ++r) {
+                            // (nr0 + s)(nr0 + s + 1) = q0 + 2 s nr0 + s^2 + s,  s = r RSTEP
+                            const int s = r * RSTEP;
+                            const int q = q0 + __umul24(2 * s, nr0) + s * s + s;
+                            const bool ok = t * 16 + Mma<T>::row(lk, r) < lent;
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                const int row = t * 16 + Mma<T>::row(lk, r);
-                                const int nn = am + row;
-                                const int dst = (row < lent ? nn * nn + nn + m : spare) * LDX;
+                            for (int sg = 0; sg < 2; ++sg)
+                                if (sg < ns) {
+                                    const int dst = __umul24(ok ? (sg ? q - am : q + am) : spare, LDX);
 #pragma unroll
-                                for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[sg][t][ct][r];
-                            }
+                                    for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[sg][t][ct][r];
+                                }
+                        }
+                    }
+            };
+            auto skip2 = [&](int am) { return am > p || am > pt; };
+            __syncthreads();
+            for (int u = wave; u < a.n_units2; u += 4) {
+                const int am = a.units2[u];
+                if (skip2(am)) continue;
+                const int nkg = m2l_tiles(P1 - am);
+                zero2();
+                for (int kg = 0; kg < nkg; ++kg) {
+                    load2(am, kg);
+                    mma2(am, kg, a2r, a2i);
                 }
+                store2(am);
+            }
         }
-        __syncthreads();
 
         // stage 3: W_n = E_n X_n, times e^{-i m phi} of each column tile's direction, summed
         // over the directions into the accumulators of the targets; unit = (n, row tile)
+        {
+            auto valid3 = [&](int uu) {
+                const int u = wave + 4 * uu;
+                return u < a.n_units3 && (a.units3[u] & 255) <= pt;
+            };
+            auto frag3 = [&](int uu, int kg) {
+                const int code = a.units3[wave + 4 * uu];
+                const int n = code & 255, mt = code >> 8, nt = m2l_tiles(2 * n + 1);
+                return Mma<T>::load4(R3 + a.rot_off[n] + (size_t)mt * nt * 256 + ((size_t)kg * 64 + lane) * 4);
+            };
+            __syncthreads();
 #pragma unroll
-        for (int uu = 0; uu < MAXU; ++uu) {
-            const int u = wave + 4 * uu;
-            if (u < a.n_units3 && (a.units3[u] & 255) <= pt) {
-                const int code = a.units3[u];
-                const int n = code & 255, mt = code >> 8;
-                const int w = 2 * n + 1, base = n * n;
-                const int nt = m2l_tiles(w), nks = (w + 3) >> 2;
-                const T* A = R3 + a.rot_off[n] + (size_t)mt * nt * 256;
-                V4 c4[NCT];
+            for (int uu = 0; uu < MAXU; ++uu) {
+                if (valid3(uu)) {
+                    const int code = a.units3[wave + 4 * uu];
+                    const int n = code & 255, mt = code >> 8;
+                    const int w = 2 * n + 1, base = n * n;
+                    const int nt = m2l_tiles(w), nks = (w + 3) >> 2;
+                    V4 c4[NCT];
 #pragma unroll
-                for (int ct = 0; ct < NCT; ++ct) c4[ct] = V4{0, 0, 0, 0};
-                for (int kg = 0; kg < nt; ++kg) {
-                    const V4 av = Mma<T>::load4(A + ((size_t)kg * 64 + lane) * 4);
-                    T bv[4][NCT];
+                    for (int ct = 0; ct < NCT; ++ct) c4[ct] = V4{0, 0, 0, 0};
+                    for (int kg = 0; kg < nt; ++kg) {
+                        const V4 av = frag3(uu, kg);
+                        T bv[4][NCT];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const int row = base + 16 * kg + 4 * j + lk;
+                        for (int j = 0; j < 4; ++j) {
+                            const int row = base + 16 * kg + 4 * j + lk;
 #pragma unroll
-                        for (int ct = 0; ct < NCT; ++ct) bv[j][ct] = X[row * LDX + ct * 16 + li];
+                            for (int ct = 0; ct < NCT; ++ct) bv[j][ct] = X[row * LDX + ct * 16 + li];
+                        }
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (4 * kg + j < nks)
+#pragma unroll
+                                for (int ct = 0; ct < NCT; ++ct) c4[ct] = Mma<T>::mma(av[j], bv[j][ct], c4[ct]);
                     }
+                    // times e^{-i m phi}: Re += v cos + v' sin, Im += v cos - v' sin with v' the
+                    // partner column (the table holds the sine with this lane's sign).  Rows
+                    // past the block are zero (A is zero-padded); their table entries are finite.
+                    const C2* pu = ps_lane + (mt * 16 - n);
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        if (4 * kg + j < nks)
+                    for (int r = 0; r < 4; ++r) {
 #pragma unroll
-                            for (int ct = 0; ct < NCT; ++ct) c4[ct] = Mma<T>::mma(av[j], bv[j][ct], c4[ct]);
-                }
-                // rows past the block are zero (A is zero-padded); their phase is irrelevant.
-                // Re += v cos + v' sin, Im += v cos - v' sin with v' the partner column and
-                // sin = sin(m phi): the sign is folded into one coefficient per row.
+                        for (int g = 0; g < G; ++g) {
+                            const C2 pm = pu[g * TL + r * RSTEP];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int m = mt * 16 + Mma<T>::row(lk, r) - n;
-                    const int am = min(m < 0 ? -m : m, PH - 1);
-                    const bool flip = (m < 0) != ((li & 1) != 0);
-#pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        const C2 pm = Ps[g * PH + am];
-                        const T cs = pm.x, sn = flip ? -pm.y : pm.y;
-#pragma unroll
-                        for (int cd = 0; cd < NCTD; ++cd) {
-                            const T v = c4[g * NCTD + cd][r];
-                            acc[uu][cd][r] = fma(v, cs, fma(Mma<T>::swap_pair(v), sn, acc[uu][cd][r]));
+                            for (int cd = 0; cd < NCTD; ++cd) {
+                                const T v = c4[g * NCTD + cd][r];
+                                acc[uu][cd][r] = fma(v, pm.x, fma(Mma<T>::swap_pair(v), pm.y, acc[uu][cd][r]));
+                            }
                         }
                     }
                 }

@@ -178,9 +178,12 @@ void check_device() {
 
 int octant(uint64_t code) { return (int)(code & 7); }
 
-// Units of work of a translation from degree ps to degree pt, costliest first (dealt
-// round-robin to the 4 waves): forward rotations n <= ps, coaxial orders |m| <= min,
-// back-rotation row tiles (n, mt) with n <= pt.
+// Units of work of a translation from degree ps to degree pt: forward rotations n <= ps,
+// coaxial orders |m| <= min, back-rotation row tiles (n, mt) with n <= pt.  The kernel
+// deals list entry u to wave u % 4; the lists are laid out in rows of 4 so that every
+// wave gets about the same number of MFMAs: units sorted by cost, each row's units given,
+// largest first, to the waves with the least work so far.  Rows are padded with code 255,
+// which every stage skips (a degree or order above any table).
 void make_units(int ps, int pt, std::vector<int>& units1, std::vector<int>& units2, std::vector<int>& units3) {
     std::vector<std::pair<int, int>> u1, u2, u3;   // (cost, code)
     for (int n = 0; n <= ps; ++n) {
@@ -189,22 +192,31 @@ void make_units(int ps, int pt, std::vector<int>& units1, std::vector<int>& unit
     }
     for (int n = 0; n <= pt; ++n) {
         const int w = 2 * n + 1, nt = m2l_tiles(w), ks = (w + 3) / 4;
-        for (int mt = 0; mt < nt; ++mt) u3.push_back({ks, n | (mt << 8)});
+        for (int mt = 0; mt < nt; ++mt) u3.push_back({ks + 2, n | (mt << 8)});   // + phase epilogue
     }
     for (int m = 0; m <= std::min(ps, pt); ++m) {
         const int lens = ps + 1 - m, lent = pt + 1 - m;
         u2.push_back({(m ? 2 : 1) * m2l_tiles(lent) * ((lens + 3) / 4), m});
     }
-    auto by_cost = [](const std::pair<int, int>& a, const std::pair<int, int>& b) { return a.first > b.first; };
-    std::stable_sort(u1.begin(), u1.end(), by_cost);
-    std::stable_sort(u2.begin(), u2.end(), by_cost);
-    std::stable_sort(u3.begin(), u3.end(), by_cost);
-    units1.clear();
-    units2.clear();
-    units3.clear();
-    for (auto& x : u1) units1.push_back(x.second);
-    for (auto& x : u2) units2.push_back(x.second);
-    for (auto& x : u3) units3.push_back(x.second);
+    auto deal = [](std::vector<std::pair<int, int>> u, std::vector<int>& out) {
+        std::stable_sort(u.begin(), u.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+            return a.first > b.first;
+        });
+        while (u.size() % 4) u.push_back({0, 255});
+        out.assign(u.size(), 255);
+        long load[4] = {0, 0, 0, 0};
+        for (size_t r = 0; r < u.size(); r += 4) {
+            int order[4] = {0, 1, 2, 3};
+            std::stable_sort(order, order + 4, [&](int a, int b) { return load[a] < load[b]; });
+            for (int i = 0; i < 4; ++i) {
+                out[r + order[i]] = u[r + i].second;
+                load[order[i]] += u[r + i].first;
+            }
+        }
+    };
+    deal(u1, units1);
+    deal(u2, units2);
+    deal(u3, units3);
 }
 
 // Coaxial tables as A fragments: per distance, per order m = 0..min(ps, pt) the packed
