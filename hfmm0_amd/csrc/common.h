@@ -73,6 +73,8 @@ void hhat_host(double x, int p, double* re, double* im);
 
 // Rotation about y: E^n(theta) with Y_n^m(R_y(theta) u) = sum_m' E[m][m'] Y_n^m'(u).
 void rotation_table(double theta, int p, double* out /* rot_size(p) */);
+// the rotation's blocks in the (u, v) basis of a degree block (tables.cpp)
+void rotation_uv_blocks(const double* E, int n, bool transpose, std::vector<double>& Wu, std::vector<double>& Wv);
 
 // Gaunt-type coaxial coupling integrals, computed once per plan for degree pmax.
 struct Gaunt {

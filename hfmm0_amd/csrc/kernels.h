@@ -90,21 +90,35 @@ void launch_direct(const double* X, const double* Y, const double* Z, const doub
 // stored as MFMA A fragments: for a tile (row tile mt, k group kg of four 4-wide
 // k-steps) lane l holds, as four consecutive values j = 0..3,
 // A[16 mt + l % 16][16 kg + 4 j + l / 16], zero outside the block.
+//
+// Rotations act on each degree block in the basis u_m = x_m + (-1)^m x_{-m} (m >= 1),
+// u_0 = x_0, v_m = x_m - (-1)^m x_{-m} (m >= 1): the reflection x_m -> (-1)^m x_{-m}
+// commutes with the rotation exp(-theta G) (G the tridiagonal generator, whose entries are
+// symmetric under m -> -m-1), so E_n splits into an (n+1) x (n+1) block on the u and an
+// n x n block on the v.  In the LDS buffer u_m sits in the row of x_m and v_m in the row of
+// x_{-m}: the u block of degree n is rows n^2+n .. n^2+2n (m ascending), the v block rows
+// n^2 .. n^2+n-1 (m = n down to 1).
 constexpr int kM2LBlockMaxP = 23;  // largest degree handled by the block kernel
 constexpr int kXcds = 8;            // accelerator dies; workgroups are dealt to them round-robin
 constexpr int kM2LMaxAccRegs = 128; // VGPRs per lane for the stage-3 accumulators (block-size choice)
 __host__ __device__ constexpr int m2l_tiles(int len) { return (len + 15) / 16; }
-// accumulator VGPRs of a (MAXU units per wave, TB targets, element size) configuration
+// stage-3 units per wave the M2L block kernel is instantiated for at table degree p: the
+// units (n, row tile of the n+1 orders), sum_{n<=p} m2l_tiles(n+1), dealt to 4 waves
+__host__ __device__ constexpr int m2l_block_maxu(int p) {
+    return p <= 7 ? 2 : p <= 11 ? 3 : p <= 15 ? 4 : p <= 17 ? 5 : p <= 19 ? 6 : 8;
+}
+// accumulator VGPRs of a (MAXU units per wave, TB targets, element size) configuration:
+// two (u and v) 16x16 tiles per unit and direction column tile
 __host__ __device__ constexpr int m2l_acc_regs(int maxu, int tb, int elem_bytes) {
-    return maxu * (2 * tb / 16) * 4 * (elem_bytes / 4);
+    return maxu * 2 * (2 * tb / 16) * 4 * (elem_bytes / 4);
 }
 // LDS elements of the expansion block: (p+1)^2 rows x ncol + 1 columns, one scratch row
 // that receives the stores of rows outside a block, and 16 zero rows that the reads of
 // the last block's padding run into; padded to 16 bytes
 __host__ __device__ constexpr int m2l_lds_x_rows(int p) { return (p + 1) * (p + 1) + 17; }
-// phase table of the M2L block kernel: entries per (lane parity, direction), indexed by
-// the signed order m + 8 KT (KT = m2l_tiles(2 p + 1)); covers -p <= m <= p + 15
-__host__ __device__ constexpr int m2l_phase_len(int p) { return 16 * m2l_tiles(2 * p + 1) + 16; }
+// phase table of the M2L block kernel: entries per (lane parity, direction) for the
+// orders m = 0 .. 16 KT - 1 (KT = m2l_tiles(p + 1))
+__host__ __device__ constexpr int m2l_phase_len(int p) { return 16 * m2l_tiles(p + 1); }
 __host__ __device__ constexpr int m2l_lds_x_elems(int p, int ncol) {
     return (m2l_lds_x_rows(p) * (ncol + 1) + 3) & ~3;
 }
@@ -119,16 +133,17 @@ struct M2LBlockArgs {
     const int* blk_targets;          // blocks x TB target boxes (-1: empty slot)
     const unsigned short* grp_dir;   // per group: G directions of equal theta and |d| (0xFFFF: none)
     const int* grp_off;              // per group: G x TB source expansion offsets (-1: none)
-    const int* units1;               // forward rotation units: degree n
+    const int* units1;               // forward rotation units: n | block << 8 (0: u, 1: v)
     int n_units1;
     const int* units2;               // coaxial units: |m| (orders +|m| and -|m|)
     int n_units2;
-    const int* units3;               // back rotation units: n | row tile << 8
+    const int* units3;               // back rotation units: n | row tile << 8 (u and v)
     int n_units3;
-    const void* rot1;                // per rotation: fragments of E_n^T, n = 0..kM2LBlockMaxP (T)
-    const void* rot3;                // per rotation: fragments of E_n (T)
-    const int* rot_off;              // offset of degree n's fragments within one rotation
-    int rot_stride;
+    const void* rot1;                // per rotation: fragments of the u and v blocks of E_n^T (T)
+    const void* rot3;                // per rotation: fragments of the u and v blocks of E_n (T)
+    const int* rot_off1;             // offsets of (n, block) [2 n + block] within rot1's rotation
+    const int* rot_off3;             // the same within rot3's
+    int rot_stride1, rot_stride3;
     const void* phase;               // e^{i m phi} per direction (T complex)
     int phase_stride;
     const DispInfo* dinfo;

@@ -81,35 +81,40 @@ struct Mma<double> {
     }
 };
 
-// coefficient index c = n^2 + n + m -> (n, m).  sqrt(c + 1/2) lies at least 1/(4(n+1)) away
-// from an integer for 0 <= c < 2^16, so the hardware square root (1 ulp) truncates to n.
-__device__ inline void decode_nm(int c, int& n, int& m) {
-    n = (int)__builtin_amdgcn_sqrtf((float)c + 0.5f);
-    m = c - __umul24(n, n + 1);
+// index q of the pair (n, m >= 0), q = n (n+1) / 2 + m -> (n, m).  With x = 8 q + 1.5,
+// (2n+1)^2 + 1/2 <= x <= (2n+3)^2 - 15/2, so sqrt(x) lies at least 1/(4 (2n+3)) inside
+// (2n+1, 2n+3); for q < 2^16 the hardware square root (1 ulp) gives
+// n = floor((sqrt(x) - 1) / 2).
+__device__ inline void decode_q(int q, int& n, int& m) {
+    n = (int)((__builtin_amdgcn_sqrtf(8.0f * (float)q + 1.5f) - 1.0f) * 0.5f);
+    m = q - ((n * (n + 1)) >> 1);
 }
 
 // Work item: groups [g0, g1) of one target block.  When a block is split over several
 // items (coarse levels, to fill the GPU) the results are added atomically.  KT is the
-// number of 16-row tiles of the largest degree block, m2l_tiles(2 p_tab + 1) (1 up to
-// p = 7, 2 up to 15, 3 up to 23); it bounds the A fragments a unit keeps in registers.
+// number of 16-row tiles of the largest (u or v) block, m2l_tiles(p_tab + 1) (1 up to
+// p = 15, 2 up to 23); it bounds the A fragments a unit keeps in registers.
 template <typename T, int TB, int G, int KT, int MAXU>
 __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     using V4 = typename Mma<T>::v4;
     using C2 = typename Mma<T>::c2;
-    constexpr int kMaxTiles = KT;                    // row tiles of a degree block
-    constexpr int kMaxTiles2 = KT == 3 ? 2 : 1;      // row tiles of an order block
+    constexpr int kMaxTiles = KT;           // row tiles of a rotation block or an order block
     constexpr int NCOLD = 2 * TB;           // columns of one direction: (pair, re/im)
     constexpr int NCTD = NCOLD / 16;        // 16-wide column tiles of one direction
     constexpr int NCT = G * NCTD;           // column tiles of the group
     constexpr int LDX = G * NCOLD + 1;      // odd row stride in LDS
-    // phase table: per lane parity and direction, entries for the signed order
-    // m = -8 KT .. 8 KT + 15 at index m + MOFF: (cos m phi, +-sin m phi), the sine negated
-    // for odd (imaginary) columns; orders |m| >= PH hold zero
-    constexpr int TL = 16 * KT + 16, MOFF = 8 * KT;
+    // phase table: per lane parity and direction, entries for the orders m = 0 .. TL-1:
+    // (cos m phi, +-sin m phi), the sine negated for odd (imaginary) columns; orders
+    // m >= PH hold zero
+    constexpr int TL = 16 * KT;
     constexpr int RSTEP = Mma<T>::row(0, 1) - Mma<T>::row(0, 0);   // rows between output registers
+    // FP32 with one tile per block (p_tab <= 15, the finest levels): every unit of every
+    // stage uses one fragment set, and the next unit's set is loaded while the current one
+    // is multiplied (the L2 latency of the operator tables is otherwise exposed per unit)
+    constexpr bool PF = sizeof(T) == 4 && KT == 1;
     extern __shared__ char smem_raw[];
     // source degree p (multipoles or parent locals) and target degree pt; equal for M2L
-    const int p = a.p, P1 = p + 1, nc = P1 * P1;
+    const int p = a.p, P1 = p + 1, nc = P1 * P1, nq = P1 * (P1 + 1) / 2;
     const int pt = a.pt, PT1 = pt + 1, nct = PT1 * PT1;
     const int ptab = max(a.p_tab, a.pt_tab);           // LDS rows are sized for this degree
     const int PH = max(P1, PT1);                        // phases per direction
@@ -120,9 +125,9 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     // derived from it) in scalar registers so that the unit branches are scalar
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int li = lane & 15, lk = lane >> 4;
-    // this lane's phase entries of stage 3: add (16 mt - n) for a unit (n, mt) and
+    // this lane's phase entries of stage 3: add 16 mt for a unit's row tile mt and
     // g TL + r RSTEP for direction g and output register r
-    const C2* ps_lane = Ps + (li & 1) * G * TL + Mma<T>::row(lk, 0) + MOFF;
+    const C2* ps_lane = Ps + (li & 1) * G * TL + Mma<T>::row(lk, 0);
     // workgroup i runs on XCD i % 8: give XCD x the contiguous item range [x n/8, (x+1) n/8)
     // (the host pads the item list to a multiple of 8 and orders it in space)
     const int per_xcd = a.n_items / kXcds;
@@ -131,70 +136,114 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     const int* tg = a.blk_targets + (size_t)item.x * TB;
     const C2* Mexp = (const C2*)a.M;
 
-    V4 acc[MAXU][NCTD];
+    // stage-3 accumulators of the wave's units in the (u, v) basis, summed over all groups
+    V4 accu[MAXU][NCTD], accv[MAXU][NCTD];
 #pragma unroll
     for (int u = 0; u < MAXU; ++u)
 #pragma unroll
-        for (int cd = 0; cd < NCTD; ++cd) acc[u][cd] = V4{0, 0, 0, 0};
+        for (int cd = 0; cd < NCTD; ++cd) {
+            accu[u][cd] = V4{0, 0, 0, 0};
+            accv[u][cd] = V4{0, 0, 0, 0};
+        }
     // zero the whole block once: rows that the padded tiles read beyond the gathered
     // degrees are then finite (stale results of earlier groups) or zero, and their
     // products with the zero padding of A vanish
     const int zero_row = spare + 1;   // the 16 rows after the scratch row stay zero
     for (int i = threadIdx.x; i < m2l_lds_x_rows(ptab) * LDX; i += 256) X[i] = (T)0;
+    // with PF: the wave's units of each stage in its lanes (lane i: unit wave + 4 i), as
+    // fragment offset << 9 | code, 255 for a unit to skip; read with v_readlane, so stepping
+    // through the units needs no scalar memory load (whose wait would also drain the LDS
+    // queue)
+    const int nu1 = a.n_units1 >> 2, nu2 = a.n_units2 >> 2;   // the lists are padded to rows of 4
+    int vt1 = 255, vt2 = 255, vt3 = 255;
+    if constexpr (PF) {
+        if (lane < nu1) {
+            const int c = a.units1[wave + 4 * lane];
+            if ((c & 255) <= p) vt1 = (a.rot_off1[2 * (c & 255) + (c >> 8)] << 9) | c;
+        }
+        if (lane < nu2) {
+            const int c = a.units2[wave + 4 * lane];
+            if (c <= p && c <= pt) vt2 = (a.coax_off[c] << 9) | c;
+        }
+        if (lane < MAXU && wave + 4 * lane < a.n_units3) {
+            const int c = a.units3[wave + 4 * lane];
+            if (c <= pt) vt3 = (a.rot_off3[2 * c] << 9) | c;   // KT = 1: row tile 0, code = n
+        }
+    }
+    auto skip = [](int t) { return (t & 511) == 255; };
     __syncthreads();
 
     for (int e = item.y; e < item.z; ++e) {
         const unsigned short* gd = a.grp_dir + (size_t)e * G;
         const DispInfo di = a.dinfo[gd[0]];   // shared by the directions of the group
-        const T* R1 = (const T*)a.rot1 + (size_t)di.theta * a.rot_stride;
-        const T* R3 = (const T*)a.rot3 + (size_t)di.theta * a.rot_stride;
+        const T* R1 = (const T*)a.rot1 + (size_t)di.theta * a.rot_stride1;
+        const T* R3 = (const T*)a.rot3 + (size_t)di.theta * a.rot_stride3;
         const T* Tz = (const T*)a.coax + (size_t)di.dist * a.coax_stride;
         const int* soff = a.grp_off + (size_t)e * G * TB;   // -1: no source (or no direction)
 
-        // stage 0: phase table, gather + e^{i m phi}
+        // stage 0: phase table; gather, times e^{i m phi}, into the (u, v) basis
         for (int i = threadIdx.x; i < 2 * G * TL; i += 256) {
-            const int par = i / (G * TL), g = (i / TL) % G, m = i % TL - MOFF;
-            const int am = m < 0 ? -m : m;
+            const int par = i / (G * TL), g = (i / TL) % G, m = i % TL;
             const unsigned short d = gd[g];
             C2 v;
             v.x = 0;
             v.y = 0;
-            if (am < PH) {
+            if (m < PH) {
                 v.x = 1;
-                if (d != 0xFFFF) v = ((const C2*)a.phase)[(size_t)d * a.phase_stride + am];
-                if ((m < 0) != (par != 0)) v.y = -v.y;
+                if (d != 0xFFFF) v = ((const C2*)a.phase)[(size_t)d * a.phase_stride + m];
+                if (par) v.y = -v.y;
             }
             Ps[i] = v;
         }
         // every wave gathers for one direction (uniform source offsets, scalar loads),
-        // lanes over coefficients (consecutive lanes read consecutive words of one
-        // expansion); the coefficient's (n, m) is decoded once per pass, not per source
+        // lanes over the pairs (n, m >= 0): x_m e^{i m phi} and x_{-m} e^{-i m phi} of each
+        // source give u_m (row of x_m) and v_m (row of x_{-m})
         {
             constexpr int WPD = 4 / G;   // waves per direction
             const int g = wave / WPD;
             const unsigned short d = gd[g];
             const int* so = soff + g * TB;
-            for (int c = (wave % WPD) * 64 + lane; c < nc; c += 64 * WPD) {
+            for (int q = (wave % WPD) * 64 + lane; q < nq; q += 64 * WPD) {
                 int n, m;
-                decode_nm(c, n, m);
+                decode_q(q, n, m);
+                const int cp = n * n + n + m, cm = n * n + n - m;
                 C2 pm;
                 pm.x = 1;
                 pm.y = 0;
-                if (d != 0xFFFF) pm = ((const C2*)a.phase)[(size_t)d * a.phase_stride + (m < 0 ? -m : m)];
-                if (m < 0) pm.y = -pm.y;
-                C2 raw[TB];
+                if (d != 0xFFFF) pm = ((const C2*)a.phase)[(size_t)d * a.phase_stride + m];
+                const T sg = (m & 1) ? (T)-1 : (T)1;
+                T* xu = X + cp * LDX + g * NCOLD;
+                T* xv = X + cm * LDX + g * NCOLD;
+                // sources in batches of JB (all loads of a batch in flight together)
+                constexpr int JB = sizeof(T) == 8 ? TB / 2 : TB;
 #pragma unroll
-                for (int j = 0; j < TB; ++j) {
-                    const int o = so[j];
-                    raw[j].x = 0;
-                    raw[j].y = 0;
-                    if (o >= 0) raw[j] = Mexp[(int64_t)o + c];
-                }
-                T* xr = X + c * LDX + g * NCOLD;
+                for (int j0 = 0; j0 < TB; j0 += JB) {
+                    C2 rp[JB], rm[JB];
 #pragma unroll
-                for (int j = 0; j < TB; ++j) {
-                    xr[2 * j] = raw[j].x * pm.x - raw[j].y * pm.y;
-                    xr[2 * j + 1] = raw[j].x * pm.y + raw[j].y * pm.x;
+                    for (int j = 0; j < JB; ++j) {
+                        const int o = so[j0 + j];
+                        rp[j].x = 0;
+                        rp[j].y = 0;
+                        rm[j].x = 0;
+                        rm[j].y = 0;
+                        if (o >= 0) {
+                            rp[j] = Mexp[(int64_t)o + cp];
+                            if (m > 0) rm[j] = Mexp[(int64_t)o + cm];
+                        }
+                    }
+#pragma unroll
+                    for (int j = 0; j < JB; ++j) {
+                        // a = x_m e^{i m phi}, b = (-1)^m x_{-m} e^{-i m phi}
+                        const T ar = rp[j].x * pm.x - rp[j].y * pm.y, ai = rp[j].x * pm.y + rp[j].y * pm.x;
+                        const T br = sg * (rm[j].x * pm.x + rm[j].y * pm.y);
+                        const T bi = sg * (rm[j].y * pm.x - rm[j].x * pm.y);
+                        xu[2 * (j0 + j)] = ar + br;
+                        xu[2 * (j0 + j) + 1] = ai + bi;
+                        if (m > 0) {
+                            xv[2 * (j0 + j)] = ar - br;
+                            xv[2 * (j0 + j) + 1] = ai - bi;
+                        }
+                    }
                 }
             }
             // target degrees above the source degree start from zero (orders |m| > p
@@ -205,31 +254,74 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                 for (int j = 0; j < NCOLD; ++j) xr[j] = (T)0;
             }
         }
-        // stage 1: X_n <- E_n^T X_n in place.  A unit is one degree n with all its row and
-        // column tiles: the A fragments of a k group are applied to every column tile; the
-        // wave writes back only after all products of the unit are formed, and no other
-        // wave touches the rows of degree n.  Steps: (unit, k group).
-        {
-            V4 a1[kMaxTiles], c4[kMaxTiles][NCT];
-            auto load1 = [&](int n, int kg) {
-                const int nt = m2l_tiles(2 * n + 1);
-                const T* A = R1 + a.rot_off[n];
+
+        // stage 1: forward rotation in place, per degree n and block (u: rows n^2+n ..
+        // n^2+2n, v: rows n^2 .. n^2+n-1), X_b <- W_b X_b.  A unit is one block with all
+        // its row and column tiles: the A fragments of a k group are applied to every
+        // column tile; the wave writes back only after all products of the unit are formed,
+        // and no other wave touches the rows of the block.
+        if constexpr (PF) {
+            auto nx = [&](int i) {
+                while (i < nu1 && skip(__builtin_amdgcn_readlane(vt1, i))) ++i;
+                return i;
+            };
+            int i1 = nx(0);
+            V4 an = V4{0, 0, 0, 0};
+            if (i1 < nu1) an = Mma<T>::load4(R1 + (__builtin_amdgcn_readlane(vt1, i1) >> 9) + lane * 4);
+            __syncthreads();
+            while (i1 < nu1) {
+                const int code = __builtin_amdgcn_readlane(vt1, i1) & 511;
+                const int n = code & 255, h = code >> 8;
+                const int len = h ? n : n + 1, base = h ? n * n : n * n + n, nks = (len + 3) >> 2;
+                const V4 av = an;
+                i1 = nx(i1 + 1);
+                if (i1 < nu1) an = Mma<T>::load4(R1 + (__builtin_amdgcn_readlane(vt1, i1) >> 9) + lane * 4);
+                V4 c4[NCT];
+                T bv[4][NCT];
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) c4[ct] = V4{0, 0, 0, 0};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int row = base + 4 * j + lk;   // past len: A is 0, X is finite
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct) bv[j][ct] = X[row * LDX + ct * 16 + li];
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (j < nks)
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ++ct) c4[ct] = Mma<T>::mma(av[j], bv[j][ct], c4[ct]);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = Mma<T>::row(lk, r);
+                    const int dst = __umul24(row < len ? base + row : spare, LDX);
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[ct][r];
+                }
+            }
+        } else {
+        __syncthreads();
+        for (int u = wave; u < a.n_units1; u += 4) {
+            const int code = a.units1[u];
+            const int n = code & 255, h = code >> 8;
+            if (n > p) continue;
+            const int len = h ? n : n + 1, base = h ? n * n : n * n + n;
+            const int nt = m2l_tiles(len), nks = (len + 3) >> 2;
+            const T* A = R1 + a.rot_off1[2 * n + h];
+            V4 c4[kMaxTiles][NCT];
+#pragma unroll
+            for (int mt = 0; mt < kMaxTiles; ++mt)
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) c4[mt][ct] = V4{0, 0, 0, 0};
+            for (int kg = 0; kg < nt; ++kg) {
+                V4 av[kMaxTiles];
 #pragma unroll
                 for (int mt = 0; mt < kMaxTiles; ++mt)
-                    if (mt < nt) a1[mt] = Mma<T>::load4(A + ((size_t)(mt * nt + kg) * 64 + lane) * 4);
-            };
-            auto zero1 = [&]() {
-#pragma unroll
-                for (int mt = 0; mt < kMaxTiles; ++mt)
-#pragma unroll
-                    for (int ct = 0; ct < NCT; ++ct) c4[mt][ct] = V4{0, 0, 0, 0};
-            };
-            auto mma1 = [&](int n, int kg, const V4 (&av)[kMaxTiles]) {
-                const int w = 2 * n + 1, base = n * n, nt = m2l_tiles(w), nks = (w + 3) >> 2;
+                    if (mt < nt) av[mt] = Mma<T>::load4(A + ((size_t)(mt * nt + kg) * 64 + lane) * 4);
                 T bv[4][NCT];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const int row = base + 16 * kg + 4 * j + lk;   // past w: A is 0, X is finite
+                    const int row = base + 16 * kg + 4 * j + lk;   // past len: A is 0, X is finite
 #pragma unroll
                     for (int ct = 0; ct < NCT; ++ct) bv[j][ct] = X[row * LDX + ct * 16 + li];
                 }
@@ -242,184 +334,307 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
 #pragma unroll
                                 for (int ct = 0; ct < NCT; ++ct)
                                     c4[mt][ct] = Mma<T>::mma(av[mt][j], bv[j][ct], c4[mt][ct]);
-            };
-            auto store1 = [&](int n) {
-                const int w = 2 * n + 1, base = n * n, nt = m2l_tiles(w);
-#pragma unroll
-                for (int mt = 0; mt < kMaxTiles; ++mt)
-                    if (mt < nt)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int row = mt * 16 + Mma<T>::row(lk, r);
-                            const int dst = __umul24(row < w ? base + row : spare, LDX);
-#pragma unroll
-                            for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[mt][ct][r];
-                        }
-            };
-            __syncthreads();
-            for (int u = wave; u < a.n_units1; u += 4) {
-                const int n = a.units1[u];
-                if (n > p) continue;
-                const int nt = m2l_tiles(2 * n + 1);
-                zero1();
-                for (int kg = 0; kg < nt; ++kg) {
-                    load1(n, kg);
-                    mma1(n, kg, a1);
-                }
-                store1(n);
             }
+#pragma unroll
+            for (int mt = 0; mt < kMaxTiles; ++mt)
+                if (mt < nt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = mt * 16 + Mma<T>::row(lk, r);
+                        const int dst = __umul24(row < len ? base + row : spare, LDX);
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[mt][ct][r];
+                    }
         }
 
-        // stage 2: X^m <- T^{|m|} X^m in place.  A unit is one |m| with the orders +|m| and
-        // -|m| (same operator): its A fragments of a k group are applied to both orders and
+        }
+        // stage 2: X^m <- T^{|m|} X^m in place on the u_m and the v_m (the operator is the
+        // same for the orders +m and -m, hence for their sum and difference).  A unit is
+        // one m with both blocks: its A fragments of a k group are applied to both and to
         // every column tile.  The table blocks are built for p_tab; source degrees beyond
-        // the evaluated p read a zero row.  Steps: (unit, k group).
+        // the evaluated p read a zero row.
         {
             const T sgn1 = (li & 1) ? (T)1 : (T)-1;   // Im(T) x X: (-Im X, Re X) per column pair
-            V4 a2r[kMaxTiles2], a2i[kMaxTiles2], c4[2][kMaxTiles2][NCT];
-            auto load2 = [&](int am, int kg) {
-                const int ntab = m2l_tiles(a.p_tab + 1 - am), nt = m2l_tiles(PT1 - am);
-                const T* A = Tz + a.coax_off[am];
+            if constexpr (PF) {
+                auto nx = [&](int i) {
+                    while (i < nu2 && skip(__builtin_amdgcn_readlane(vt2, i))) ++i;
+                    return i;
+                };
+                V4 anr = V4{0, 0, 0, 0}, ani = V4{0, 0, 0, 0};
+                auto pre = [&](int i) {   // fragment (row tile 0, k group 0) of the order's block
+                    const T* At = Tz + (__builtin_amdgcn_readlane(vt2, i) >> 9) + lane * 8;
+                    anr = Mma<T>::load4(At);
+                    ani = Mma<T>::load4(At + 4);
+                };
+                int i2 = nx(0);
+                if (i2 < nu2) pre(i2);
+                __syncthreads();
+                while (i2 < nu2) {
+                    const int am = __builtin_amdgcn_readlane(vt2, i2) & 511;
+                    const int ns = am > 0 ? 2 : 1;
+                    const int len = P1 - am, lent = PT1 - am, nks = (len + 3) >> 2;
+                    const V4 are = anr, aim = ani;
+                    i2 = nx(i2 + 1);
+                    if (i2 < nu2) pre(i2);
+                    const int q0 = __umul24(am + lk, am + lk + 1);
+                    int rowp[4], rowm[4];
 #pragma unroll
-                for (int t = 0; t < kMaxTiles2; ++t)
-                    if (t < nt) {
-                        const T* At = A + ((size_t)(t * ntab + kg) * 64 + lane) * 8;
-                        a2r[t] = Mma<T>::load4(At);
-                        a2i[t] = Mma<T>::load4(At + 4);
+                    for (int j = 0; j < 4; ++j) {
+                        const int q = q0 + __umul24(8 * j, am + lk) + 16 * j * j + 4 * j;
+                        const bool ok = 4 * j + lk < len;
+                        rowp[j] = __umul24(ok ? q + am : zero_row, LDX);
+                        rowm[j] = __umul24(ok ? q - am : zero_row, LDX);
                     }
-            };
-            auto zero2 = [&]() {
+                    V4 c4[2][NCT];
 #pragma unroll
-                for (int sg = 0; sg < 2; ++sg)
+                    for (int sg = 0; sg < 2; ++sg) {
 #pragma unroll
-                    for (int t = 0; t < kMaxTiles2; ++t)
+                        for (int ct = 0; ct < NCT; ++ct) c4[sg][ct] = V4{0, 0, 0, 0};
+                        if (sg < ns) {
+                            T b0[4][NCT], b1[4][NCT];
 #pragma unroll
-                        for (int ct = 0; ct < NCT; ++ct) c4[sg][t][ct] = V4{0, 0, 0, 0};
-            };
-            auto mma2 = [&](int am, int kg, const V4 (&are)[kMaxTiles2], const V4 (&aim)[kMaxTiles2]) {
-                const int ns = am > 0 ? 2 : 1;                  // orders +am and -am
-                const int len = P1 - am, lent = PT1 - am;       // source and target block sizes
-                const int nt = m2l_tiles(lent), nks = (len + 3) >> 2;
-                // source rows of this k group: degree nn_j = am + 16 kg + 4 j + lk, row
-                // nn_j (nn_j + 1) +- am; nn_j (nn_j + 1) = q0 + 8 j nn_0 + 16 j^2 + 4 j
-                const int nn0 = am + 16 * kg + lk;
-                const int q0 = __umul24(nn0, nn0 + 1);
-                int rowp[4], rowm[4];
+                            for (int j = 0; j < 4; ++j) {
+                                const int cin = sg ? rowm[j] : rowp[j];
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int q = q0 + __umul24(8 * j, nn0) + 16 * j * j + 4 * j;
-                    const bool ok = 16 * kg + 4 * j + lk < len;
-                    rowp[j] = __umul24(ok ? q + am : zero_row, LDX);
-                    rowm[j] = __umul24(ok ? q - am : zero_row, LDX);
+                                for (int ct = 0; ct < NCT; ++ct) {
+                                    b0[j][ct] = X[cin + ct * 16 + li];
+                                    b1[j][ct] = sgn1 * X[cin + ct * 16 + (li ^ 1)];
+                                }
+                            }
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                if (j < nks)
+#pragma unroll
+                                    for (int ct = 0; ct < NCT; ++ct) {
+                                        c4[sg][ct] = Mma<T>::mma(are[j], b0[j][ct], c4[sg][ct]);
+                                        c4[sg][ct] = Mma<T>::mma(aim[j], b1[j][ct], c4[sg][ct]);
+                                    }
+                        }
+                    }
+                    const int nr0 = am + Mma<T>::row(lk, 0);
+                    const int qr = __umul24(nr0, nr0 + 1);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int sr = r * RSTEP;
+                        const int q = qr + __umul24(2 * sr, nr0) + sr * sr + sr;
+                        const bool ok = Mma<T>::row(lk, r) < lent;
+#pragma unroll
+                        for (int sg = 0; sg < 2; ++sg)
+                            if (sg < ns) {
+                                const int dst = __umul24(ok ? (sg ? q - am : q + am) : spare, LDX);
+#pragma unroll
+                                for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[sg][ct][r];
+                            }
+                    }
                 }
-#pragma unroll
-                for (int sg = 0; sg < 2; ++sg) {
-                    if (sg < ns) {
-                        T b0[4][NCT], b1[4][NCT];
-#pragma unroll
+            } else {
+                __syncthreads();
+                for (int u = wave; u < a.n_units2; u += 4) {
+                    const int am = a.units2[u];
+                    if (am > p || am > pt) continue;
+                    const int ns = am > 0 ? 2 : 1;                  // u_m and v_m
+                    const int len = P1 - am, lent = PT1 - am;       // source and target block sizes
+                    const int ntab = m2l_tiles(a.p_tab + 1 - am);   // k groups of the table block
+                    const int nt = m2l_tiles(lent), nkg = m2l_tiles(len), nks = (len + 3) >> 2;
+                    const T* A = Tz + a.coax_off[am];
+                    V4 c4[2][kMaxTiles][NCT];
+    #pragma unroll
+                    for (int sg = 0; sg < 2; ++sg)
+    #pragma unroll
+                        for (int t = 0; t < kMaxTiles; ++t)
+    #pragma unroll
+                            for (int ct = 0; ct < NCT; ++ct) c4[sg][t][ct] = V4{0, 0, 0, 0};
+                    for (int kg = 0; kg < nkg; ++kg) {
+                        V4 are[kMaxTiles], aim[kMaxTiles];
+    #pragma unroll
+                        for (int t = 0; t < kMaxTiles; ++t)
+                            if (t < nt) {
+                                const T* At = A + ((size_t)(t * ntab + kg) * 64 + lane) * 8;
+                                are[t] = Mma<T>::load4(At);
+                                aim[t] = Mma<T>::load4(At + 4);
+                            }
+                        // source rows of this k group: degree nn_j = am + 16 kg + 4 j + lk, row
+                        // nn_j (nn_j + 1) +- am; nn_j (nn_j + 1) = q0 + 8 j nn_0 + 16 j^2 + 4 j
+                        const int nn0 = am + 16 * kg + lk;
+                        const int q0 = __umul24(nn0, nn0 + 1);
+                        int rowp[4], rowm[4];
+    #pragma unroll
                         for (int j = 0; j < 4; ++j) {
-                            const int cin = sg ? rowm[j] : rowp[j];
-#pragma unroll
-                            for (int ct = 0; ct < NCT; ++ct) {
-                                b0[j][ct] = X[cin + ct * 16 + li];
-                                b1[j][ct] = sgn1 * X[cin + ct * 16 + (li ^ 1)];
+                            const int q = q0 + __umul24(8 * j, nn0) + 16 * j * j + 4 * j;
+                            const bool ok = 16 * kg + 4 * j + lk < len;
+                            rowp[j] = __umul24(ok ? q + am : zero_row, LDX);
+                            rowm[j] = __umul24(ok ? q - am : zero_row, LDX);
+                        }
+    #pragma unroll
+                        for (int sg = 0; sg < 2; ++sg) {
+                            if (sg < ns) {
+                                T b0[4][NCT], b1[4][NCT];
+    #pragma unroll
+                                for (int j = 0; j < 4; ++j) {
+                                    const int cin = sg ? rowm[j] : rowp[j];
+    #pragma unroll
+                                    for (int ct = 0; ct < NCT; ++ct) {
+                                        b0[j][ct] = X[cin + ct * 16 + li];
+                                        b1[j][ct] = sgn1 * X[cin + ct * 16 + (li ^ 1)];
+                                    }
+                                }
+    #pragma unroll
+                                for (int j = 0; j < 4; ++j)
+                                    if (4 * kg + j < nks)
+    #pragma unroll
+                                        for (int t = 0; t < kMaxTiles; ++t)
+                                            if (t < nt)
+    #pragma unroll
+                                                for (int ct = 0; ct < NCT; ++ct) {
+                                                    c4[sg][t][ct] = Mma<T>::mma(are[t][j], b0[j][ct], c4[sg][t][ct]);
+                                                    c4[sg][t][ct] = Mma<T>::mma(aim[t][j], b1[j][ct], c4[sg][t][ct]);
+                                                }
                             }
                         }
-#pragma unroll
-                        for (int j = 0; j < 4; ++j)
-                            if (4 * kg + j < nks)
-#pragma unroll
-                                for (int t = 0; t < kMaxTiles2; ++t)
-                                    if (t < nt)
-#pragma unroll
-                                        for (int ct = 0; ct < NCT; ++ct) {
-                                            c4[sg][t][ct] = Mma<T>::mma(are[t][j], b0[j][ct], c4[sg][t][ct]);
-                                            c4[sg][t][ct] = Mma<T>::mma(aim[t][j], b1[j][ct], c4[sg][t][ct]);
-                                        }
                     }
-                }
-            };
-            auto store2 = [&](int am) {
-                const int ns = am > 0 ? 2 : 1, lent = PT1 - am, nt = m2l_tiles(lent);
-                // target rows: degree am + row, row = 16 t + row(lk, r)
-#pragma unroll
-                for (int t = 0; t < kMaxTiles2; ++t)
-                    if (t < nt) {
-                        const int nr0 = am + t * 16 + Mma<T>::row(lk, 0);
-                        const int q0 = __umul24(nr0, nr0 + 1);
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            // (nr0 + s)(nr0 + s + 1) = q0 + 2 s nr0 + s^2 + s,  s = r RSTEP
-                            const int s = r * RSTEP;
-                            const int q = q0 + __umul24(2 * s, nr0) + s * s + s;
-                            const bool ok = t * 16 + Mma<T>::row(lk, r) < lent;
-#pragma unroll
-                            for (int sg = 0; sg < 2; ++sg)
-                                if (sg < ns) {
-                                    const int dst = __umul24(ok ? (sg ? q - am : q + am) : spare, LDX);
-#pragma unroll
-                                    for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[sg][t][ct][r];
-                                }
+                    // target rows: degree am + row, row = 16 t + row(lk, r)
+    #pragma unroll
+                    for (int t = 0; t < kMaxTiles; ++t)
+                        if (t < nt) {
+                            const int nr0 = am + t * 16 + Mma<T>::row(lk, 0);
+                            const int q0 = __umul24(nr0, nr0 + 1);
+    #pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                // (nr0 + s)(nr0 + s + 1) = q0 + 2 s nr0 + s^2 + s,  s = r RSTEP
+                                const int s = r * RSTEP;
+                                const int q = q0 + __umul24(2 * s, nr0) + s * s + s;
+                                const bool ok = t * 16 + Mma<T>::row(lk, r) < lent;
+    #pragma unroll
+                                for (int sg = 0; sg < 2; ++sg)
+                                    if (sg < ns) {
+                                        const int dst = __umul24(ok ? (sg ? q - am : q + am) : spare, LDX);
+    #pragma unroll
+                                        for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[sg][t][ct][r];
+                                    }
+                            }
                         }
-                    }
-            };
-            auto skip2 = [&](int am) { return am > p || am > pt; };
-            __syncthreads();
-            for (int u = wave; u < a.n_units2; u += 4) {
-                const int am = a.units2[u];
-                if (skip2(am)) continue;
-                const int nkg = m2l_tiles(P1 - am);
-                zero2();
-                for (int kg = 0; kg < nkg; ++kg) {
-                    load2(am, kg);
-                    mma2(am, kg, a2r, a2i);
                 }
-                store2(am);
             }
         }
 
-        // stage 3: W_n = E_n X_n, times e^{-i m phi} of each column tile's direction, summed
-        // over the directions into the accumulators of the targets; unit = (n, row tile)
-        {
-            auto valid3 = [&](int uu) {
-                const int u = wave + 4 * uu;
-                return u < a.n_units3 && (a.units3[u] & 255) <= pt;
+        // stage 3: back rotation of both blocks of degree n, output rows indexed by the
+        // order m = 0..n in both (so u'_m and v'_m of one m sit in the same lane and
+        // register), times e^{-i m phi} of each column tile's direction:
+        //   U += cos(m phi) u' - i sin(m phi) v',   V += cos(m phi) v' - i sin(m phi) u'
+        // (the phase applied to x_m and x_{-m}, written in the (u, v) basis), summed over
+        // the group's directions into the accumulators; unit = (n, row tile)
+        if constexpr (PF) {
+            int n3 = 0;   // the wave's next unit
+            while (n3 < MAXU && skip(__builtin_amdgcn_readlane(vt3, n3))) ++n3;
+            V4 anu = V4{0, 0, 0, 0}, anv = V4{0, 0, 0, 0};
+            auto pre = [&](int uu) {   // the u block's fragment, then the v block's (one tile each)
+                const int t = __builtin_amdgcn_readlane(vt3, uu);
+                const T* Au = R3 + (t >> 9) + lane * 4;
+                anu = Mma<T>::load4(Au);
+                if ((t & 255) > 0) anv = Mma<T>::load4(Au + 256);
             };
-            auto frag3 = [&](int uu, int kg) {
-                const int code = a.units3[wave + 4 * uu];
-                const int n = code & 255, mt = code >> 8, nt = m2l_tiles(2 * n + 1);
-                return Mma<T>::load4(R3 + a.rot_off[n] + (size_t)mt * nt * 256 + ((size_t)kg * 64 + lane) * 4);
-            };
+            if (n3 < MAXU) pre(n3);
             __syncthreads();
 #pragma unroll
             for (int uu = 0; uu < MAXU; ++uu) {
-                if (valid3(uu)) {
-                    const int code = a.units3[wave + 4 * uu];
-                    const int n = code & 255, mt = code >> 8;
-                    const int w = 2 * n + 1, base = n * n;
-                    const int nt = m2l_tiles(w), nks = (w + 3) >> 2;
-                    V4 c4[NCT];
+                if (uu == n3) {
+                    const int n = __builtin_amdgcn_readlane(vt3, uu) & 255;
+                    const int nku = (n + 4) >> 2, nkv = (n + 3) >> 2;
+                    const V4 avu = anu, avv = anv;
+                    int nx = uu + 1;
+                    while (nx < MAXU && skip(__builtin_amdgcn_readlane(vt3, nx))) ++nx;
+                    n3 = nx;
+                    if (nx < MAXU) pre(nx);
+                    V4 cu[NCT], cv[NCT];
+                    T bu[4][NCT], bw[4][NCT];
 #pragma unroll
-                    for (int ct = 0; ct < NCT; ++ct) c4[ct] = V4{0, 0, 0, 0};
-                    for (int kg = 0; kg < nt; ++kg) {
-                        const V4 av = frag3(uu, kg);
+                    for (int j = 0; j < 4; ++j)
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ++ct) {
+                            bu[j][ct] = X[(n * n + n + 4 * j + lk) * LDX + ct * 16 + li];
+                            bw[j][ct] = X[(n * n + 4 * j + lk) * LDX + ct * 16 + li];
+                        }
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct) {
+                        cu[ct] = V4{0, 0, 0, 0};
+                        cv[ct] = V4{0, 0, 0, 0};
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        if (j < nku)
+#pragma unroll
+                            for (int ct = 0; ct < NCT; ++ct) cu[ct] = Mma<T>::mma(avu[j], bu[j][ct], cu[ct]);
+                        if (j < nkv)
+#pragma unroll
+                            for (int ct = 0; ct < NCT; ++ct) cv[ct] = Mma<T>::mma(avv[j], bw[j][ct], cv[ct]);
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                        for (int g = 0; g < G; ++g) {
+                            const C2 pm = ps_lane[g * TL + r * RSTEP];
+#pragma unroll
+                            for (int cd = 0; cd < NCTD; ++cd) {
+                                const T wu = cu[g * NCTD + cd][r], wv = cv[g * NCTD + cd][r];
+                                accu[uu][cd][r] = fma(wu, pm.x, fma(Mma<T>::swap_pair(wv), pm.y, accu[uu][cd][r]));
+                                accv[uu][cd][r] = fma(wv, pm.x, fma(Mma<T>::swap_pair(wu), pm.y, accv[uu][cd][r]));
+                            }
+                        }
+                    }
+                }
+            }
+        } else {
+            __syncthreads();
+#pragma unroll
+            for (int uu = 0; uu < MAXU; ++uu) {
+                const int u = wave + 4 * uu;
+                if (u < a.n_units3 && (a.units3[u] & 255) <= pt) {
+                    const int code = a.units3[u];
+                    const int n = code & 255, mt = code >> 8;
+                    const int ntu = m2l_tiles(n + 1), nku = (n + 4) >> 2;   // u: n+1 inputs
+                    const int ntv = m2l_tiles(n), nkv = (n + 3) >> 2;       // v: n inputs
+                    const T* Au = R3 + a.rot_off3[2 * n] + (size_t)mt * ntu * 256;
+                    const T* Av = R3 + a.rot_off3[2 * n + 1] + (size_t)mt * ntv * 256;
+                    V4 cu[NCT], cv[NCT];
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct) {
+                        cu[ct] = V4{0, 0, 0, 0};
+                        cv[ct] = V4{0, 0, 0, 0};
+                    }
+                    for (int kg = 0; kg < ntu; ++kg) {
+                        const V4 av = Mma<T>::load4(Au + ((size_t)kg * 64 + lane) * 4);
                         T bv[4][NCT];
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
-                            const int row = base + 16 * kg + 4 * j + lk;
+                            const int row = n * n + n + 16 * kg + 4 * j + lk;
 #pragma unroll
                             for (int ct = 0; ct < NCT; ++ct) bv[j][ct] = X[row * LDX + ct * 16 + li];
                         }
 #pragma unroll
                         for (int j = 0; j < 4; ++j)
-                            if (4 * kg + j < nks)
+                            if (4 * kg + j < nku)
 #pragma unroll
-                                for (int ct = 0; ct < NCT; ++ct) c4[ct] = Mma<T>::mma(av[j], bv[j][ct], c4[ct]);
+                                for (int ct = 0; ct < NCT; ++ct) cu[ct] = Mma<T>::mma(av[j], bv[j][ct], cu[ct]);
                     }
-                    // times e^{-i m phi}: Re += v cos + v' sin, Im += v cos - v' sin with v' the
-                    // partner column (the table holds the sine with this lane's sign).  Rows
-                    // past the block are zero (A is zero-padded); their table entries are finite.
-                    const C2* pu = ps_lane + (mt * 16 - n);
+                    for (int kg = 0; kg < ntv; ++kg) {
+                        const V4 av = Mma<T>::load4(Av + ((size_t)kg * 64 + lane) * 4);
+                        T bv[4][NCT];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int row = n * n + 16 * kg + 4 * j + lk;
+#pragma unroll
+                            for (int ct = 0; ct < NCT; ++ct) bv[j][ct] = X[row * LDX + ct * 16 + li];
+                        }
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (4 * kg + j < nkv)
+#pragma unroll
+                                for (int ct = 0; ct < NCT; ++ct) cv[ct] = Mma<T>::mma(av[j], bv[j][ct], cv[ct]);
+                    }
+                    // Re U += c Re u' + s Im v', Im U += c Im u' - s Re v' (V likewise with u, v
+                    // exchanged): the partner lane holds the other half, the table the sine with
+                    // this lane's sign.  Rows past the blocks are zero; their entries are finite.
+                    const C2* pu = ps_lane + mt * 16;
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
 #pragma unroll
@@ -427,8 +642,9 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                             const C2 pm = pu[g * TL + r * RSTEP];
 #pragma unroll
                             for (int cd = 0; cd < NCTD; ++cd) {
-                                const T v = c4[g * NCTD + cd][r];
-                                acc[uu][cd][r] = fma(v, pm.x, fma(Mma<T>::swap_pair(v), pm.y, acc[uu][cd][r]));
+                                const T wu = cu[g * NCTD + cd][r], wv = cv[g * NCTD + cd][r];
+                                accu[uu][cd][r] = fma(wu, pm.x, fma(Mma<T>::swap_pair(wv), pm.y, accu[uu][cd][r]));
+                                accv[uu][cd][r] = fma(wv, pm.x, fma(Mma<T>::swap_pair(wu), pm.y, accv[uu][cd][r]));
                             }
                         }
                     }
@@ -438,7 +654,8 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
         __syncthreads();
     }
 
-    // add to the FP64 local expansions
+    // back to the x basis and added to the FP64 local expansions:
+    // x_0 = U_0, x_m = (U + V) / 2, x_{-m} = (-1)^m (U - V) / 2
 #pragma unroll
     for (int uu = 0; uu < MAXU; ++uu) {
         const int u = wave + 4 * uu;
@@ -453,15 +670,23 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
                     double* dst = (double*)(a.L + a.exp_off[t]) + (col & 1);
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const int row = mt * 16 + Mma<T>::row(lk, r);
-                        if (row < 2 * n + 1) {
-                            double* x = dst + 2 * (n * n + row);
-                            if (item.w)
-                                atomicAdd(x, (double)acc[uu][cd][r]);
-                            else if (a.overwrite)
-                                *x = (double)acc[uu][cd][r];
-                            else
-                                *x += (double)acc[uu][cd][r];
+                        const int m = mt * 16 + Mma<T>::row(lk, r);
+                        if (m <= n) {
+                            const double U = (double)accu[uu][cd][r], V = (double)accv[uu][cd][r];
+                            const double xp = m == 0 ? U : 0.5 * (U + V);
+                            const double xm = (m & 1) ? -0.5 * (U - V) : 0.5 * (U - V);
+                            double* yp = dst + 2 * (n * n + n + m);
+                            double* ym = dst + 2 * (n * n + n - m);
+                            if (item.w) {
+                                atomicAdd(yp, xp);
+                                if (m > 0) atomicAdd(ym, xm);
+                            } else if (a.overwrite) {
+                                *yp = xp;
+                                if (m > 0) *ym = xm;
+                            } else {
+                                *yp += xp;
+                                if (m > 0) *ym += xm;
+                            }
                         }
                     }
                 }
@@ -495,20 +720,21 @@ void launch_one(const M2LBlockArgs* a, size_t shm, hipStream_t s) {
     hipLaunchKernelGGL((m2l_block_kernel<T, TB, G, KT, MAXU>), dim3(a->n_items), dim3(256), shm, s, *a);
 }
 
-// (KT, MAXU) follow from the table degree: p <= 7: (1, 4); p <= 11: (2, 4);
-// p <= 15: (2, 8); p <= 17: (3, 8); p <= 23: (3, 12)
+// (KT, MAXU) follow from the table degree: KT = m2l_tiles(p + 1), MAXU = m2l_block_maxu(p)
 template <typename T, int TB, int G>
 void launch_p(const M2LBlockArgs* a, size_t shm, int p, hipStream_t s) {
     if (p <= 7)
-        launch_one<T, TB, G, 1, 4>(a, shm, s);
+        launch_one<T, TB, G, 1, m2l_block_maxu(7)>(a, shm, s);
     else if (p <= 11)
-        launch_one<T, TB, G, 2, 4>(a, shm, s);
+        launch_one<T, TB, G, 1, m2l_block_maxu(11)>(a, shm, s);
     else if (p <= 15)
-        launch_one<T, TB, G, 2, 8>(a, shm, s);
+        launch_one<T, TB, G, 1, m2l_block_maxu(15)>(a, shm, s);
     else if (p <= 17)
-        launch_one<T, TB, G, 3, 8>(a, shm, s);
+        launch_one<T, TB, G, 2, m2l_block_maxu(17)>(a, shm, s);
+    else if (p <= 19)
+        launch_one<T, TB, G, 2, m2l_block_maxu(19)>(a, shm, s);
     else if (p <= kM2LBlockMaxP)
-        launch_one<T, TB, G, 3, 12>(a, shm, s);
+        launch_one<T, TB, G, 2, m2l_block_maxu(23)>(a, shm, s);
     else
         fail(-9, "internal: degree above the block kernel's range");
 }

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <tuple>
@@ -119,8 +120,8 @@ struct hfmm_plan {
     // rotation blocks as MFMA A fragments (E^T for the forward, E for the back rotation)
     double *d_rf1 = nullptr, *d_rf3 = nullptr;
     float *d_rf1_32 = nullptr, *d_rf3_32 = nullptr;
-    int* d_rf_off = nullptr;
-    int rf_stride = 0;
+    int *d_rf_off1 = nullptr, *d_rf_off3 = nullptr;
+    int rf_stride1 = 0, rf_stride3 = 0;
     float2* d_phase32 = nullptr;
     float2* dM32 = nullptr;
     double2 *dM = nullptr, *dL = nullptr, *d_q = nullptr, *d_far = nullptr;
@@ -178,8 +179,9 @@ void check_device() {
 
 int octant(uint64_t code) { return (int)(code & 7); }
 
-// Units of work of a translation from degree ps to degree pt: forward rotations n <= ps,
-// coaxial orders |m| <= min, back-rotation row tiles (n, mt) with n <= pt.  The kernel
+// Units of work of a translation from degree ps to degree pt: forward rotations of the u
+// and v blocks of degrees n <= ps, coaxial orders |m| <= min, back rotations (n, row tile
+// of the orders 0..n) with n <= pt.  The kernel
 // deals list entry u to wave u % 4; the lists are laid out in rows of 4 so that every
 // wave gets about the same number of MFMAs: units sorted by cost, each row's units given,
 // largest first, to the waves with the least work so far.  Rows are padded with code 255,
@@ -187,13 +189,12 @@ int octant(uint64_t code) { return (int)(code & 7); }
 void make_units(int ps, int pt, std::vector<int>& units1, std::vector<int>& units2, std::vector<int>& units3) {
     std::vector<std::pair<int, int>> u1, u2, u3;   // (cost, code)
     for (int n = 0; n <= ps; ++n) {
-        const int w = 2 * n + 1;
-        u1.push_back({m2l_tiles(w) * ((w + 3) / 4), n});
+        u1.push_back({m2l_tiles(n + 1) * ((n + 4) / 4), n});                   // u block
+        if (n > 0) u1.push_back({m2l_tiles(n) * ((n + 3) / 4), n | (1 << 8)});  // v block
     }
-    for (int n = 0; n <= pt; ++n) {
-        const int w = 2 * n + 1, nt = m2l_tiles(w), ks = (w + 3) / 4;
-        for (int mt = 0; mt < nt; ++mt) u3.push_back({ks + 2, n | (mt << 8)});   // + phase epilogue
-    }
+    for (int n = 0; n <= pt; ++n)
+        for (int mt = 0; mt < m2l_tiles(n + 1); ++mt)
+            u3.push_back({(n + 4) / 4 + (n + 3) / 4 + 2, n | (mt << 8)});   // + phase epilogue
     for (int m = 0; m <= std::min(ps, pt); ++m) {
         const int lens = ps + 1 - m, lent = pt + 1 - m;
         u2.push_back({(m ? 2 : 1) * m2l_tiles(lent) * ((lens + 3) / 4), m});
@@ -294,9 +295,7 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     if (p > kM2LBlockMaxP) return;   // higher degrees use the generic kernel (coarse levels only)
     const int fp32 = P.precision == HFMM_PREC_MIXED;
     const int eb = fp32 ? 4 : 8;
-    int n_units3 = 0;
-    for (int n = 0; n <= p; ++n) n_units3 += m2l_tiles(2 * n + 1);
-    const int maxu = (n_units3 + 3) / 4;
+    const int maxu = m2l_block_maxu(p);   // the instantiation's stage-3 units per wave
     // shape of one operator application: TB target boxes x G directions (of equal theta
     // and |d|).  A requested pair count TB*G (m2l_kernel = 8, 16, 32, 64) is used if it
     // fits.  Otherwise, from measurements at N = 1M, k = 40 (finest level, 5.4M pairs):
@@ -305,7 +304,7 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     // 48.3 for 8 x 4); the next smaller shape when LDS is short.
     auto fits = [&](int c, int g) {
         if (!fp32 && !(c == 8 || (c == 16 && g == 1))) return false;   // FP64 instantiations
-        if (m2l_acc_regs((maxu + 3) / 4 * 4, c, eb) > kM2LMaxAccRegs) return false;
+        if (m2l_acc_regs(maxu, c, eb) > kM2LMaxAccRegs) return false;
         return m2l_block_lds_bytes(p, c, g, eb) <= P.max_lds;
     };
     int tb = 0, grp = 0;
@@ -528,30 +527,61 @@ void build_tables(hfmm_plan& P) {
     for (size_t i = 0; i < thetas.size(); ++i) rotation_table(thetas[i], pmax, &rot[i * rsz]);
     P.d_rot = P.upload(rot);
     if (P.m2l_kernel != 1) {
-        // rotation blocks n <= min(pmax, kM2LBlockMaxP) as A fragments, padded to 16x16
-        // tiles: rf1 holds A = E_n^T (forward rotation), rf3 holds A = E_n (back rotation)
+        // rotation blocks n <= min(pmax, kM2LBlockMaxP) in the (u, v) basis as A fragments:
+        // rf1 holds the blocks of E_n^T (forward rotation; u and v in their LDS order),
+        // rf3 those of E_n (back rotation; output rows of both blocks indexed by the order
+        // m = 0..n, the v block's row 0 zero, inputs in LDS order)
         const int pb = std::min(pmax, kM2LBlockMaxP);
-        std::vector<int> off(pb + 2, 0);
-        for (int n = 0; n <= pb; ++n) off[n + 1] = off[n] + m2l_tiles(2 * n + 1) * m2l_tiles(2 * n + 1) * 256;
-        const int stride = off[pb + 1];
-        std::vector<double> f1((size_t)thetas.size() * stride, 0.0), f3(f1.size(), 0.0);
+        std::vector<int> off1(2 * (pb + 1), 0), off3(2 * (pb + 1), 0);
+        int s1 = 0, s3 = 0;
+        for (int n = 0; n <= pb; ++n) {
+            off1[2 * n] = s1;
+            s1 += m2l_tiles(n + 1) * m2l_tiles(n + 1) * 256;
+            off1[2 * n + 1] = s1;
+            s1 += m2l_tiles(n) * m2l_tiles(n) * 256;
+            off3[2 * n] = s3;
+            s3 += m2l_tiles(n + 1) * m2l_tiles(n + 1) * 256;
+            off3[2 * n + 1] = s3;
+            s3 += m2l_tiles(n + 1) * m2l_tiles(n) * 256;
+        }
+        std::vector<double> f1((size_t)thetas.size() * s1, 0.0), f3((size_t)thetas.size() * s3, 0.0);
+        // A fragments of an R x K matrix a(i, k) at dst
+        auto put = [](double* dst, int R, int K, const std::function<double(int, int)>& a) {
+            const int nk = m2l_tiles(K);
+            for (int mt = 0; mt < m2l_tiles(R); ++mt)
+                for (int kg = 0; kg < nk; ++kg)
+                    for (int lane = 0; lane < 64; ++lane)
+                        for (int j = 0; j < 4; ++j) {
+                            const int i = 16 * mt + (lane & 15), k = 16 * kg + 4 * j + (lane >> 4);
+                            if (i < R && k < K) dst[((size_t)(mt * nk + kg) * 64 + lane) * 4 + j] = a(i, k);
+                        }
+        };
+        std::vector<double> Wu, Wv;
         for (size_t th = 0; th < thetas.size(); ++th)
             for (int n = 0; n <= pb; ++n) {
-                const int w = 2 * n + 1, nt = m2l_tiles(w);
                 const double* E = &rot[th * rsz + rot_offset(n)];
-                for (int mt = 0; mt < nt; ++mt)
-                    for (int kg = 0; kg < nt; ++kg)
-                        for (int lane = 0; lane < 64; ++lane)
-                            for (int j = 0; j < 4; ++j) {
-                                const int i = 16 * mt + (lane & 15), k = 16 * kg + 4 * j + (lane >> 4);
-                                if (i >= w || k >= w) continue;
-                                const size_t at = th * stride + off[n] + ((size_t)(mt * nt + kg) * 64 + lane) * 4 + j;
-                                f1[at] = E[k * w + i];
-                                f3[at] = E[i * w + k];
-                            }
+                for (int st = 0; st < 2; ++st) {
+                    rotation_uv_blocks(E, n, st == 0, Wu, Wv);   // stage 1: E^T, stage 3: E
+                    auto wu = [&](int i, int k) { return Wu[(size_t)i * (n + 1) + k]; };
+                    // v index m = 1..n; LDS order k' = n - m
+                    auto wv = [&](int mi, int mk) { return Wv[(size_t)(mi - 1) * n + (mk - 1)]; };
+                    if (st == 0) {
+                        double* b = &f1[th * s1];
+                        put(b + off1[2 * n], n + 1, n + 1, wu);
+                        if (n > 0) put(b + off1[2 * n + 1], n, n, [&](int i, int k) { return wv(n - i, n - k); });
+                    } else {
+                        double* b = &f3[th * s3];
+                        put(b + off3[2 * n], n + 1, n + 1, wu);
+                        if (n > 0)
+                            put(b + off3[2 * n + 1], n + 1, n,
+                                [&](int i, int k) { return i == 0 ? 0.0 : wv(i, n - k); });
+                    }
+                }
             }
-        P.rf_stride = stride;
-        P.d_rf_off = P.upload(off);
+        P.rf_stride1 = s1;
+        P.rf_stride3 = s3;
+        P.d_rf_off1 = P.upload(off1);
+        P.d_rf_off3 = P.upload(off3);
         if (P.precision == HFMM_PREC_MIXED) {
             P.d_rf1_32 = P.upload(std::vector<float>(f1.begin(), f1.end()));
             P.d_rf3_32 = P.upload(std::vector<float>(f3.begin(), f3.end()));
@@ -863,8 +893,10 @@ M2LBlockArgs block_args(const hfmm_plan& P, const M2LBlockLevel& B, bool fp32, c
     a.n_units3 = B.n_units3;
     a.rot1 = fp32 ? (const void*)P.d_rf1_32 : (const void*)P.d_rf1;
     a.rot3 = fp32 ? (const void*)P.d_rf3_32 : (const void*)P.d_rf3;
-    a.rot_off = P.d_rf_off;
-    a.rot_stride = P.rf_stride;
+    a.rot_off1 = P.d_rf_off1;
+    a.rot_off3 = P.d_rf_off3;
+    a.rot_stride1 = P.rf_stride1;
+    a.rot_stride3 = P.rf_stride3;
     a.phase = fp32 ? (const void*)P.d_phase32 : (const void*)P.d_phase;
     a.phase_stride = P.pmax + 1;
     a.dinfo = P.d_dinfo;

@@ -273,4 +273,27 @@ void rotation_table(double theta, int p, double* out) {
     }
 }
 
+// Blocks of a rotation (E, or E^T when transpose) in the basis u_0 = x_0,
+// u_m = x_m + (-1)^m x_{-m}, v_m = x_m - (-1)^m x_{-m} (m = 1..n), where it is block
+// diagonal: Wu (n+1) x (n+1), row-major by (output m, input m); Wv n x n by (m-1, m-1).
+// Column j of a block is the image of one basis vector: x = e_0 for u_0, and
+// x_j = 1/2, x_{-j} = +-(-1)^j / 2 for u_j / v_j.
+void rotation_uv_blocks(const double* E, int n, bool transpose, std::vector<double>& Wu, std::vector<double>& Wv) {
+    const int w = 2 * n + 1;
+    auto B = [&](int a, int b) { return transpose ? E[(b + n) * w + (a + n)] : E[(a + n) * w + (b + n)]; };
+    auto sgn = [](int m) { return (m & 1) ? -1.0 : 1.0; };
+    Wu.assign((size_t)(n + 1) * (n + 1), 0.0);
+    Wv.assign((size_t)n * n, 0.0);
+    std::vector<double> y(w);
+    for (int j = 0; j <= n; ++j) {
+        for (int a = -n; a <= n; ++a) y[a + n] = j == 0 ? B(a, 0) : 0.5 * (B(a, j) + sgn(j) * B(a, -j));
+        Wu[j] = y[n];
+        for (int i = 1; i <= n; ++i) Wu[(size_t)i * (n + 1) + j] = y[i + n] + sgn(i) * y[n - i];
+    }
+    for (int j = 1; j <= n; ++j) {
+        for (int a = -n; a <= n; ++a) y[a + n] = 0.5 * (B(a, j) - sgn(j) * B(a, -j));
+        for (int i = 1; i <= n; ++i) Wv[(size_t)(i - 1) * n + (j - 1)] = y[i + n] - sgn(i) * y[n - i];
+    }
+}
+
 }  // namespace hfmm
