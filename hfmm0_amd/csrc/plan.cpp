@@ -297,23 +297,26 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     const int eb = fp32 ? 4 : 8;
     const int maxu = m2l_block_maxu(p);   // the instantiation's stage-3 units per wave
     // shape of one operator application: TB target boxes x G directions (of equal theta
-    // and |d|).  A requested pair count TB*G (m2l_kernel = 8, 16, 32, 64) is used if it
-    // fits.  Otherwise, from measurements at N = 1M, k = 40 (finest level, 5.4M pairs):
-    // FP32 takes 8 x 2 (6.0 ms at p = 11 vs 6.6 for 8 x 4 and 9.9 for 16 x 4; 7.5 ms at
-    // p = 12 vs 7.8 for 8 x 4), FP64 takes 8 x 1 (38.9 ms at p = 17 vs 48.4 for 8 x 2 and
-    // 48.3 for 8 x 4); the next smaller shape when LDS is short.
+    // and |d|).  A requested pair count TB*G (m2l_kernel = 8, 16, 32, 64) takes the first
+    // fitting shape with that count in the order below (larger TB first: partly filled
+    // direction groups waste columns, target blocks rarely do).  Otherwise, from
+    // measurements at N = 1M, k = 40 (finest level, 5.4M pairs): FP32 takes 16 x 2 up to
+    // p = 11 (5.27 ms at p = 11 against 5.61 for 8 x 2; 3 waves per SIMD) and 8 x 2 above
+    // (6.42 ms at p = 12 against 7.02 for 16 x 2, which falls to 2 waves per SIMD); FP64
+    // takes 8 x 1 (38.9 ms at p = 17 against 48.4 for 8 x 2); then the next smaller shape
+    // that fits.
     auto fits = [&](int c, int g) {
         if (!fp32 && !(c == 8 || (c == 16 && g == 1))) return false;   // FP64 instantiations
         if (m2l_acc_regs(maxu, c, eb) > kM2LMaxAccRegs) return false;
         return m2l_block_lds_bytes(p, c, g, eb) <= P.max_lds;
     };
     int tb = 0, grp = 0;
-    const int shapes[6][2] = {{16, 4}, {8, 4}, {16, 2}, {8, 2}, {16, 1}, {8, 1}};
+    const int shapes[6][2] = {{16, 4}, {16, 2}, {8, 4}, {8, 2}, {16, 1}, {8, 1}};
     if (P.m2l_kernel > 1)
         for (auto& sh : shapes)
             if (!tb && sh[0] * sh[1] == P.m2l_kernel && fits(sh[0], sh[1])) tb = sh[0], grp = sh[1];
-    const int pref[2][2][2] = {{{8, 2}, {8, 1}}, {{8, 1}, {8, 2}}};
-    const int* pr = &pref[fp32 ? 0 : 1][0][0];
+    const int pref[3][2][2] = {{{16, 2}, {8, 2}}, {{8, 2}, {8, 1}}, {{8, 1}, {8, 2}}};
+    const int* pr = &pref[!fp32 ? 2 : p <= 11 ? 0 : 1][0][0];
     for (int i = 0; i < 2 && !tb; ++i)
         if (fits(pr[2 * i], pr[2 * i + 1])) tb = pr[2 * i], grp = pr[2 * i + 1];
     for (auto& sh : shapes)
