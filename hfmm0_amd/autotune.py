@@ -130,6 +130,67 @@ def tune(positions, k, charges, tol=1e-6, precision="mixed", leaves=(32, 64, 128
     return best
 
 
+def refine_degrees(positions, charges, k, p_level, tol=1e-6, leaf=128, precision="mixed", n_samp=2048,
+                   seed=7, safety=0.8, reps=10, max_rounds=6, verbose=False):
+    """Certified local search over the per-level degrees, from a seed such as the network's
+    prediction.  A move lowers the degree of one level with V lists by one, alone or while
+    raising the next coarser level by one (the coarse levels are cheap, the finest level
+    holds most pairs).  Every round measures all moves of the current degrees on the device
+    and keeps the fastest one whose error against the FP64 direct sum, on n_samp targets
+    drawn with `seed`, stays within safety * tol; the search stops when no move is at least
+    1% faster.  Returns (p_level, ms, err, history) with history a list of
+    (p_level, ms, err) of every measured configuration."""
+    import torch
+    x = np.ascontiguousarray(positions, dtype=np.float64)
+    n = len(x)
+    idx = np.sort(np.random.default_rng(seed).choice(n, min(n_samp, n), replace=False))
+    ref = api.direct(x, charges, k, idx)
+    q_dev = torch.from_numpy(np.ascontiguousarray(charges)).cuda()
+    out_dev = torch.empty(n, dtype=torch.complex128, device="cuda")
+    history = []
+
+    def measure(pl):
+        st = api.setup(x, k, p=max(pl), N_max=leaf, p_level=pl, precision=precision)
+        try:
+            api.evaluate_device(st, q_dev, out_dev)
+            torch.cuda.synchronize()
+            phi = out_dev[torch.from_numpy(idx).cuda()].cpu().numpy()
+            err = float(np.linalg.norm(phi - ref) / max(np.linalg.norm(ref), 1e-300))
+            ms = api.benchmark(st, q_dev, out_dev, warmup=2, reps=reps)
+        finally:
+            st.close()
+        history.append((list(pl), ms, err))
+        if verbose:
+            print(f"refine: {pl} err {err:.3e} {ms:.3f} ms", flush=True)
+        return ms, err
+
+    best = [int(v) for v in p_level]
+    best_ms, best_err = measure(best)
+    for _ in range(max_rounds):
+        moves = []
+        for i in range(len(best) - 1, 0, -1):   # level i + 1 >= 2, finest first
+            if best[i] <= 2:
+                continue
+            low = list(best)
+            low[i] -= 1
+            moves.append(low)
+            if i >= 2:   # raise the next coarser level with V lists
+                up = list(low)
+                up[i - 1] += 1
+                moves.append(up)
+        pick = None
+        for pl in moves:
+            if any(h[0] == pl for h in history):
+                continue
+            ms, err = measure(pl)
+            if err <= safety * tol and ms < 0.99 * best_ms and (pick is None or ms < pick[1]):
+                pick = (pl, ms, err)
+        if pick is None:
+            break
+        best, best_ms, best_err = pick
+    return best, best_ms, best_err, history
+
+
 class PlannedEvaluator:
     """``fmm = plan(positions, k, tol=1e-6); phi = fmm(charges)``.
 

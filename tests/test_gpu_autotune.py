@@ -29,3 +29,27 @@ def test_plan_tunes_certifies_and_reuses_wisdom(tmp_path, monkeypatch):
     assert not fmm2.tuned and fmm2.key == fmm.key or not fmm2.tuned
     assert fmm2.report[-1][1] <= 1e-6
     fmm2.close()
+
+
+def test_refine_degrees_certifies_and_is_not_slower():
+    """From a generous seed (the network's degrees plus one), the local search returns
+    degrees whose error on targets the search never saw meets the tolerance, that were
+    measured no slower than the seed, and whose history starts with the seed."""
+    import hfmm0_amd.autotune as A
+    from hfmm0_amd.learned import DegreeMLP, predict_p_level, sampled_error
+    import hfmm0_amd as hf
+    from hfmm0_amd.workloads import uniform_cube
+    print("case: refine_degrees n=200000 k=20 leaf=64", flush=True)
+    x, q = uniform_cube(200_000)
+    net = DegreeMLP.load()
+    seed = [p + 1 for p in predict_p_level(net, x, q, 20.0, n_max=64, margin=0.0)]
+    net.close()
+    pl, ms, err, hist = A.refine_degrees(x, q, 20.0, seed, tol=1e-6, leaf=64, reps=5, verbose=True)
+    assert hist[0][0] == seed
+    assert ms <= hist[0][1]
+    assert err <= 0.8e-6
+    st = hf.setup(x, 20.0, p=max(pl), N_max=64, p_level=pl, precision="mixed")
+    held_out = sampled_error(x, q, 20.0, hf.evaluate(st, q), n_samp=2048, seed=11)
+    st.close()
+    print(f"  seed {seed} -> {pl}: {hist[0][1]:.3f} -> {ms:.3f} ms, held-out err {held_out:.2e}", flush=True)
+    assert held_out <= 1e-6
