@@ -158,6 +158,8 @@ size_t m2l_block_lds_bytes(int p, int tb, int g, int precision_bytes);
 void prepare_m2l_block(int p, int tb, int g, int fp32);
 void launch_m2l_block(const M2LBlockArgs& a, int tb, int g, int fp32, hipStream_t s);
 void launch_to_float(const double2* in, float2* out, int64_t n, hipStream_t s);
+// zero the first ncoef coefficients of the expansions of the given boxes
+void launch_zero_boxes(const int* boxes, int n, const int64_t* exp_off, int ncoef, double2* E, hipStream_t s);
 // the result rounded to complex64 (output opt-in)
 void launch_to_c64(const double2* in, float2* out, int64_t n, hipStream_t s);
 

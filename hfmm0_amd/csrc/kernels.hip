@@ -55,14 +55,16 @@ __constant__ LegCoef c_leg;
 // jhat_n(x) for n = 0..p by Miller's downward recurrence
 //   jhat_{n-1} = jhat_n - x^2 jhat_{n+1} / ((2n+1)(2n+3)),
 // normalised with jhat_0 = sin(x)/x (or jhat_1 = 3(sin x - x cos x)/x^3 when that is
-// the larger one).  out[n * stride].
-__device__ void jhat_dev(double x, int p, double* out, int stride) {
+// the larger one).  out[n * stride].  The recurrence starts at N = p + 1.5 xmax + 30 for
+// an upper bound xmax >= x common to the caller's workgroup (k R of the box), so every lane
+// walks the same n and the factor table is read at a uniform index.
+__device__ void jhat_dev(double x, int p, double* out, int stride, double xmax) {
     x = fabs(x);
     if (x == 0.0) {
         for (int n = 0; n <= p; ++n) out[n * stride] = 1.0;
         return;
     }
-    const int N = p + (int)(1.5 * x) + 30;
+    const int N = p + (int)(1.5 * fmax(xmax, x)) + 30;
     double a_next = 0.0, a = 1.0;
     const double x2 = x * x;
     for (int n = N; n >= 1; --n) {
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(256) void p2m_kernel(const int* leaves, int chunk, 
             s_geo[2 * j] = rho > 0.0 ? make_double2(dz / rho, rxy / rho) : make_double2(1.0, 0.0);
             s_geo[2 * j + 1] = rxy > 0.0 ? make_double2(dx / rxy, dy / rxy) : make_double2(1.0, 0.0);
             double* rad = &s_qrad[j * P1].x;
-            jhat_dev(k * rho, p, rad, 2);
+            jhat_dev(k * rho, p, rad, 2, k * R);
             const double2 qq = q[i];
             const double rr = rho / R;
             double pw = 1.0;
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(64) void l2p_kernel(const int* leaves, DevBoxes bx,
         const double ct = r > 0.0 ? dz / r : 1.0;
         const double st = r > 0.0 ? rxy / r : 0.0;
         const double2 eia = rxy > 0.0 ? make_double2(dx / rxy, dy / rxy) : make_double2(1.0, 0.0);
-        jhat_dev(k * r, p, rad, 1);
+        jhat_dev(k * r, p, rad, 1, k * R);
         const double rr = r / R;
         double pw = 1.0;
         for (int n = 0; n <= p; ++n) {
@@ -375,7 +377,7 @@ __global__ __launch_bounds__(64) void l2p_field_kernel(const int* leaves, DevBox
         const double ct = r > 0.0 ? dz / r : 1.0;
         const double st = r > 0.0 ? rxy / r : 0.0;
         const double2 eia = rxy > 0.0 ? make_double2(dx / rxy, dy / rxy) : make_double2(1.0, 0.0);
-        jhat_dev(k * r, p + 1, rad, 1);
+        jhat_dev(k * r, p + 1, rad, 1, k * R);
         const double rr = r / R;
         double pw = 1.0;
         for (int n = 0; n <= p + 1; ++n) {

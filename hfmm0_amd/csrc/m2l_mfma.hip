@@ -789,6 +789,20 @@ void launch_to_float(const double2* in, float2* out, int64_t n, hipStream_t s) {
     if (e != hipSuccess) fail(-3, std::string("kernel launch failed (to_float): ") + hipGetErrorString(e));
 }
 
+__global__ void zero_boxes_kernel(const int* boxes, int n, const int64_t* exp_off, int ncoef, double2* E) {
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    double2* e = E + exp_off[boxes[b]];
+    for (int c = threadIdx.x; c < ncoef; c += blockDim.x) e[c] = make_double2(0.0, 0.0);
+}
+
+void launch_zero_boxes(const int* boxes, int n, const int64_t* exp_off, int ncoef, double2* E, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(zero_boxes_kernel, dim3(n), dim3(256), 0, s, boxes, n, exp_off, ncoef, E);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) fail(-3, std::string("kernel launch failed (zero_boxes): ") + hipGetErrorString(e));
+}
+
 void launch_to_c64(const double2* in, float2* out, int64_t n, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(to_c64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, n);

@@ -62,6 +62,9 @@ struct M2LBlockLevel {
     void* coax = nullptr;          // A fragments of the coaxial tables, kernel precision
     int* coax_off = nullptr;       // per order m
     int coax_stride = 0;
+    bool split = false;            // M2M split over items: the targets are cleared first
+    int n_zero = 0;                // targets to clear
+    int* zero = nullptr;
 };
 
 struct XlateLaunch {
@@ -456,7 +459,11 @@ void build_tree_blocks(hfmm_plan& P, int L, int kind, const std::vector<double>&
         }
     }
     if (blk_ptr.size() < 2) return;
-    const std::vector<int4> items = make_items(t, blk_targets, blk_ptr, tb, false);
+    // coarse M2M levels have few blocks: their groups (child octants) are split over items
+    // that add atomically, after the parents are cleared; L2L blocks hold one group
+    const int nblk = (int)blk_ptr.size() - 1;
+    const bool split = kind == 1 && nblk < 512;
+    const std::vector<int4> items = make_items(t, blk_targets, blk_ptr, tb, split);
     std::vector<int> units1, units2, units3;
     make_units(ps, pt, units1, units2, units3);
     M2LBlockLevel& B = kind == 1 ? P.m2mb[L] : P.l2lb[L];
@@ -478,6 +485,14 @@ void build_tree_blocks(hfmm_plan& P, int L, int kind, const std::vector<double>&
     B.n_units2 = (int)units2.size();
     B.units3 = P.upload(units3);
     B.n_units3 = (int)units3.size();
+    B.split = split;
+    if (split) {
+        std::vector<int> z;
+        for (int b : blk_targets)
+            if (b >= 0) z.push_back(b);
+        B.n_zero = (int)z.size();
+        B.zero = P.upload(z);
+    }
     std::vector<double> cf;
     std::vector<int> coff;
     B.coax_stride = coax_fragments(tab, 1, ps, pt, cf, coff);
@@ -930,8 +945,11 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
     launch_p2m(P.d_far_leaves, P.n_far_leaves, P.p2m_chunk, bx, lv, P.dX, P.dY, P.dZ, P.d_q, P.k, P.p_leafmax, P.dM, s);
     mark(2);
     for (int L = P.depth - 1; L >= 2; --L) {
-        if (P.m2mb[L].use)
-            launch_m2l_block(block_args(P, P.m2mb[L], false, P.dM, P.dM, 1), P.m2mb[L].tb, P.m2mb[L].grp, 0, s);
+        if (P.m2mb[L].use) {
+            const M2LBlockLevel& B = P.m2mb[L];
+            if (B.split) launch_zero_boxes(B.zero, B.n_zero, P.d_expoff, ncoef(P.p_eff[L]), P.dM, s);
+            launch_m2l_block(block_args(P, B, false, P.dM, P.dM, B.split ? 0 : 1), B.tb, B.grp, 0, s);
+        }
         else if (P.m2m[L].n_targets)
             launch_xlate(xargs(P, P.m2m[L], P.dM, P.dM, 0), P.m2m[L].n_targets, s);
     }
