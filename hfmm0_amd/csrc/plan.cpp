@@ -275,7 +275,10 @@ std::vector<int4> make_items(const Tree& t, const std::vector<int>& blk_targets,
         const uint64_t rx = region(x) >> 3, ry = region(y) >> 3;
         return rx != ry ? rx < ry : (region(x) & 7) < (region(y) & 7);
     });
-    const int nsplit = allow_split ? std::max(1, std::min(16, (2048 + nblk - 1) / std::max(nblk, 1))) : 1;
+    // measured at N = 1M, k = 40: 8 blocks (level 2) run best split 64 ways (0.12 ms against
+    // 0.20 at 16), 64 blocks (level 3) at 16 ways (0.19 ms against 0.25 at 32)
+    const int cap = nblk <= 16 ? 64 : 16;
+    const int nsplit = allow_split ? std::max(1, std::min(cap, (2048 + nblk - 1) / std::max(nblk, 1))) : 1;
     std::vector<int4> items;
     for (int b : border) {
         const int e0 = blk_ptr[b], e1 = blk_ptr[b + 1];
