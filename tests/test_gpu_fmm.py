@@ -181,6 +181,20 @@ def test_matrix_core_m2l_matches_generic_fp64(hf, k, p):
     assert d < 1e-13
 
 
+@pytest.mark.parametrize("p_level", [[12, 14, 10, 13], [15, 11, 13, 9]])
+def test_block_translations_match_generic_per_level(hf, p_level):
+    """Degrees that rise and fall between levels give non-square M2M and L2L blocks (child
+    degree != parent degree); in FP64 the matrix-core translations equal the generic
+    thread-per-coefficient kernels to rounding."""
+    say(f"case: block vs generic fp64, per-level p={p_level} N=30000 k=15")
+    x, q = D.uniform_problem(30000)
+    a = hf.evaluate(hf.setup(x, 15.0, p=max(p_level), N_max=32, p_level=p_level), q)
+    b = hf.evaluate(hf.setup(x, 15.0, p=max(p_level), N_max=32, p_level=p_level, m2l_kernel="generic"), q)
+    d = D.relative_error(a, b)
+    say(f"  rel diff {d:.3e}")
+    assert d < 1e-13
+
+
 def test_matrix_core_m2l_fp32(hf):
     say("case: mfma m2l fp32 N=30000 k=20 p=14")
     x, q = D.uniform_problem(30000)
