@@ -115,15 +115,32 @@ class DegreeMLP:
             return cls(params=f["params"])
 
 
+P_CAP = 40   # largest degree of the evaluator
+
+
+def degree_floor(k, extent, level):
+    """High-frequency floor on the degree of a level for a 1e-6 target: ceil(0.75 kD + 2),
+    D = 2 sqrt(3) h the box diameter (h = extent 2^-(level+1)), capped at P_CAP.  Measured
+    on 400k points on a sphere and in Gaussian clusters at k = 100 (kD = 43 at level 2):
+    level-2 degrees 31 / 28 / 26 gave 6.9e-7 / 2.0e-6 / 1.2e-5 with the finer levels as
+    predicted.  The network was trained on k <= 60 with degrees <= 23, below which the
+    floor does not act at the usual leaf sizes."""
+    kd = abs(k) * 2.0 * math.sqrt(3.0) * extent * 0.5 ** (level + 1)
+    return min(P_CAP, int(math.ceil(0.75 * kd + 2.0)))
+
+
 def predict_p_level(net, positions, charges, k, n_max=16, max_depth=None, margin=1.0, p_min=4, p_max=23):
     """Per-level degrees for ``setup(p_level=...)`` (index 0 = level 1) predicted from the
-    host tree's level features.  Level 1 has no far field; it takes the level-2 degree."""
+    host tree's level features, raised to the high-frequency floor (degree_floor) where
+    k times the box diameter is large.  Level 1 has no far field; it takes the level-2
+    degree."""
     from .tree import build_tree
     tree = build_tree(positions, n_max, max_depth)
     if tree["depth"] < 2:
         return [p_min] * max(tree["depth"], 1)
     d = net.predict_degrees(level_features(tree, charges, k), margin=margin, p_min=p_min, p_max=p_max)
-    return [int(d[0])] + [int(v) for v in d]
+    d = [max(int(v), degree_floor(k, tree["extent"], lv)) for lv, v in enumerate(d, start=2)]
+    return [d[0]] + d
 
 
 def m2l_flops(p_level, v_per_level):

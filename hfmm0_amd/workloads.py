@@ -58,3 +58,35 @@ def mixed_problem(n, k_range=(1.0, 60.0), seed=0):
         x = np.vstack([rng.uniform(0.0, 1.0, (a, 3)), sphere(b), clusters(n - a - b)])
     q = rng.normal(size=n) + 1j * rng.normal(size=n)
     return x, q, k, ("uniform", "sphere", "clusters", "mixed")[kind]
+
+
+def sphere_clusters(n_sphere, n_clusters, seed):
+    """Points on the sphere |x - c| = 0.45, c = (0.5, 0.5, 0.5), then points in 8 Gaussian
+    clusters (sigma 0.02, centres uniform in [0.1, 0.9]^3), from default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=(n_sphere, 3))
+    s = 0.5 + 0.45 * v / np.linalg.norm(v, axis=1, keepdims=True)
+    c = rng.uniform(0.1, 0.9, (8, 3))
+    g = c[rng.integers(0, 8, n_clusters)] + rng.normal(scale=0.02, size=(n_clusters, 3))
+    return rng, np.vstack([s, g])
+
+
+def c3_problem(n=2_000_000):
+    """Adaptive configuration: N/2 points on the sphere and N/2 in 8 Gaussian clusters,
+    k = 100, complex N(0,1) charges; default_rng(42 + N) draws the positions, then the
+    charges."""
+    rng, x = sphere_clusters(n // 2, n - n // 2, 42 + n)
+    q = rng.normal(size=n) + 1j * rng.normal(size=n)
+    return x, q, 100.0
+
+
+def c5_problem(n=4_000_000):
+    """Mixed configuration: a third uniform in the unit cube, a third on the sphere, the
+    rest in 8 Gaussian clusters, k = 60, complex N(0,1) charges; default_rng(42 + N)."""
+    rng = np.random.default_rng(42 + n)
+    a = n // 3
+    u = rng.uniform(0.0, 1.0, (a, 3))
+    _, sg = sphere_clusters(a, n - 2 * a, int(rng.integers(0, 2**31)))
+    x = np.vstack([u, sg])
+    q = rng.normal(size=n) + 1j * rng.normal(size=n)
+    return x, q, 60.0

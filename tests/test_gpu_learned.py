@@ -73,3 +73,23 @@ def test_certified_evaluate_falls_back_to_plain():
     np.testing.assert_allclose(phi, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
     phi2, err2, used_plain2 = evaluate_certified(plain, too_low, x, q, 20.0, tol=TOL, n_samp=512)
     assert not used_plain2 and err2 <= TOL
+
+
+def test_high_frequency_floor_certifies_c3_shape():
+    """k = 100 on a sphere plus Gaussian clusters (kD = 43 at level 2, outside the network's
+    training range): the predicted degrees, raised to the high-frequency floor, meet 1e-6;
+    the network alone (capped at degree 23) does not."""
+    import hfmm0_amd as hf
+    from hfmm0_amd.learned import DegreeMLP, predict_p_level, sampled_error
+    from hfmm0_amd.workloads import c3_problem
+    print("case: c3 shape n=400000 k=100 leaf=128 max_depth=10", flush=True)
+    x, q, k = c3_problem(400_000)
+    net = DegreeMLP.load()
+    pl = predict_p_level(net, x, q, k, n_max=128, max_depth=10, margin=0.0)
+    net.close()
+    st = hf.setup(x, k, p=max(pl), N_max=128, max_depth=10, p_level=pl, precision="mixed", tree_device=True)
+    err = sampled_error(x, q, k, hf.evaluate(st, q), n_samp=2048, seed=5)
+    st.close()
+    print(f"  degrees {pl} err {err:.2e}", flush=True)
+    assert pl[1] >= 31
+    assert err <= 1e-6
