@@ -29,7 +29,8 @@ MAX_LEVELS = 24
 class Opts(ctypes.Structure):
     _fields_ = [("p", ctypes.c_int), ("p_level", ctypes.c_int * MAX_LEVELS), ("n_max", ctypes.c_int),
                 ("max_depth", ctypes.c_int), ("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("m2l_kernel", ctypes.c_int), ("tree_device", ctypes.c_int)]
+                ("m2l_kernel", ctypes.c_int), ("tree_device", ctypes.c_int),
+                ("m2l_fp64_levels", ctypes.c_int)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -38,7 +39,8 @@ class PlanInfo(ctypes.Structure):
                 ("n_u", ctypes.c_int64), ("n_w", ctypes.c_int64), ("n_x", ctypes.c_int64),
                 ("n_p2p_pairs", ctypes.c_int64), ("p_level", ctypes.c_int * MAX_LEVELS),
                 ("extent", ctypes.c_double), ("pmin", ctypes.c_double * 3),
-                ("p_table", ctypes.c_int * MAX_LEVELS)]
+                ("p_table", ctypes.c_int * MAX_LEVELS), ("m2l_fp32_levels", ctypes.c_int),
+                ("n_leaves_phase64", ctypes.c_int)]
 
 
 class HFMMError(RuntimeError):

@@ -18,6 +18,9 @@ import numpy as np
 from . import _lib as L
 
 PRECISIONS = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED}
+# the automatic mixed-precision rule: M2L in FP32 where k D_l <= this (D_l the box
+# diagonal of level l), FP64 above (hfmm.h: HFMM_MIXED_FP32_MAX_KD)
+MIXED_FP32_MAX_KD = 32.0
 
 
 class HFMMState:
@@ -46,6 +49,12 @@ class HFMMState:
     def p_level(self):
         """Degree evaluated at levels 1..depth."""
         return [self.info.p_level[i] for i in range(1, self.info.depth + 1)]
+
+    @property
+    def m2l_fp32_levels(self):
+        """Levels whose M2L runs in FP32 (mixed precision; empty at FP64)."""
+        m = self.info.m2l_fp32_levels
+        return [i for i in range(1, self.info.depth + 1) if (m >> i) & 1]
 
     @property
     def p_table(self):
@@ -84,7 +93,7 @@ class HFMMState:
 
 
 def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, use_graph=True,
-          m2l_kernel="auto", tree_device=False):
+          m2l_kernel="auto", tree_device=False, m2l_fp64_levels=None):
     """Build the octree, lists and operator tables; upload everything to the GPU.
 
     Validation order and messages follow ``HFMM.py:57-64`` and ``HFMM.py:14-16``.
@@ -95,10 +104,13 @@ def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_le
     block kernel for that many pairs per operator application (target boxes times
     directions of equal theta and |d|; used where it fits).
     ``tree_device=True`` builds the octree and lists on the GPU (same arrays).
+    ``m2l_fp64_levels`` (mixed precision only) lists the tree levels whose M2L stays in
+    FP64; ``None`` applies the automatic rule (FP64 where k times the level's box diagonal
+    exceeds ``MIXED_FP32_MAX_KD``).
     """
     x, k = _check_problem(positions, k)
     o = make_opts(p, N_max, max_depth, precision=precision, p_level=p_level, use_graph=use_graph,
-                  m2l_kernel=m2l_kernel, tree_device=tree_device, _k=k)
+                  m2l_kernel=m2l_kernel, tree_device=tree_device, m2l_fp64_levels=m2l_fp64_levels, _k=k)
     h = ctypes.c_void_p()
     L.check(L.lib().hfmm_plan_create(L.ptr(x), len(x), k, ctypes.byref(o), ctypes.byref(h)))
     return HFMMState(h, len(x), k, o)
@@ -112,7 +124,7 @@ def _check_problem(positions, k):
 
 
 def make_opts(p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, use_graph=True,
-              m2l_kernel="auto", tree_device=False, _k=0.0):
+              m2l_kernel="auto", tree_device=False, m2l_fp64_levels=None, _k=0.0):
     """Validated plan options (the C ``hfmm_opts``), in the reference's validation order
     after the positions check (``HFMM.py:57-64``, ``HFMM.py:14-16``)."""
     if p < 2 or int(p) != p or N_max < 1 or int(N_max) != N_max:
@@ -133,6 +145,11 @@ def make_opts(p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, 
         raise ValueError("m2l_kernel must be 'auto', 'generic', 8, 16, 32 or 64")
     o.m2l_kernel = {"auto": 0, "generic": 1}.get(m2l_kernel, m2l_kernel)
     o.tree_device = 1 if tree_device else 0
+    if m2l_fp64_levels is not None:
+        lv = [int(v) for v in m2l_fp64_levels]
+        if any(v < 1 or v >= L.MAX_LEVELS for v in lv):
+            raise ValueError(f"m2l_fp64_levels entries must be tree levels in [1, {L.MAX_LEVELS - 1}]")
+        o.m2l_fp64_levels = sum(1 << v for v in set(lv))
     if p_level is not None:
         for i, v in enumerate(p_level):
             if i + 1 >= L.MAX_LEVELS:

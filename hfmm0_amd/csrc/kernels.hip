@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "common.h"
 #include "kernels.h"
@@ -547,14 +548,10 @@ __global__ __launch_bounds__(kXlateThreads) void xlate_kernel(XlateArgs a) {
 // ---------------------------------------------------------------------------------
 // P2P near field.
 // ---------------------------------------------------------------------------------
-// sin and cos of x in FP32: Cody-Waite reduction by pi/2 with a two-term split of
-// pi/2 (float(pi/2) and its remainder), then the Taylor polynomials of degree 9 (sin)
-// and 10 (cos) on [-pi/4, pi/4] (truncation error below 2e-9, under half an FP32 ulp
-// of the results), and a quadrant swap.
-__device__ inline void sincos_rr(float x, float& s, float& c) {
-    const float kn = rintf(x * 0.636619772367581343f);
-    float y = fmaf(-kn, 1.57079637050628662109375f, x);
-    y = fmaf(-kn, -4.37113900018624283e-8f, y);
+// sin and cos in FP32 of y + q pi/2, y in [-pi/4, pi/4]: the Taylor polynomials of degree
+// 9 (sin) and 10 (cos) on [-pi/4, pi/4] (truncation error below 2e-9, under half an FP32
+// ulp of the results), then the quadrant swap.
+__device__ inline void sincos_quadrant(float y, int q, float& s, float& c) {
     const float y2 = y * y;
     float ps = fmaf(y2, 2.7557319223985893e-6f, -1.9841269841269841e-4f);
     ps = fmaf(y2, ps, 8.3333333333333333e-3f);
@@ -565,11 +562,42 @@ __device__ inline void sincos_rr(float x, float& s, float& c) {
     pc = fmaf(y2, pc, 4.1666666666666667e-2f);
     pc = fmaf(y2, pc, -0.5f);
     const float cs = fmaf(y2, pc, 1.0f);
-    const int q = ((int)kn) & 3;
+    q &= 3;
     const float a = (q & 1) ? cs : sn;
     const float b = (q & 1) ? sn : cs;
     s = (q & 2) ? -a : a;
     c = ((q + 1) & 2) ? -b : b;
+}
+
+// sin and cos of x in FP32: Cody-Waite reduction by pi/2 with a two-term split of pi/2
+// (float(pi/2) and its remainder), then sincos_quadrant.
+__device__ inline void sincos_rr(float x, float& s, float& c) {
+    const float kn = rintf(x * 0.636619772367581343f);
+    float y = fmaf(-kn, 1.57079637050628662109375f, x);
+    y = fmaf(-kn, -4.37113900018624283e-8f, y);
+    sincos_quadrant(y, (int)kn, s, c);
+}
+
+// e^{ikr}/r in FP32 with the phase k r accurate to FP64 rounding, from an FP64 r^2: an FP32
+// phase carries an error of about k r 2^-23 rad, which matters once k r reaches tens (long
+// W / X list pairs at high frequency).  r is the FP32 estimate r2 rsq(r2) refined by one
+// Newton step in FP64 (relative error ~1e-14), k r is reduced by pi/2 (two-term FP64 split)
+// and the reduced argument goes to the FP32 polynomials.  Also returns 1/r and k r (FP32)
+// for the field weights.
+__device__ inline void phase_fp64_weights(double r2, double k, float& wr, float& wi, float& rinv, float& kr) {
+    const float r2f = (float)r2;
+    rinv = __builtin_amdgcn_rsqf(r2f);
+    const double r0 = (double)(r2f * rinv);
+    const double r = fma(0.5 * (double)rinv, fma(-r0, r0, r2), r0);
+    const double ph = k * r;
+    const double kn = rint(ph * 0.63661977236758134308);
+    double y = fma(-kn, 1.5707963267948966192, ph);
+    y = fma(-kn, 6.123233995736766036e-17, y);
+    float s, c;
+    sincos_quadrant((float)y, (int)kn, s, c);
+    wr = c * rinv;
+    wi = s * rinv;
+    kr = (float)ph;
 }
 
 template <typename T>
@@ -615,15 +643,19 @@ __device__ inline void field_weights(T r2, T k, T& wr, T& wi, T& fr, T& fi) {
 // and scans the sources j == l / T' (mod 64/T') of each 64-point source batch staged
 // in LDS (coordinates relative to the target leaf centre, in T).  Lane-group partial
 // sums are combined with wavefront shuffles.  Sources of a leaf are a list of point
-// ranges (adjacent leaves, and the W/X interactions evaluated directly).
-template <typename T>
-__global__ __launch_bounds__(64) void p2p_kernel(const int* leaf_box, DevBoxes bx, const int* rng_ptr,
+// ranges (adjacent leaves, and the W/X interactions evaluated directly).  PH (FP32 only):
+// coordinates and the phase k r in FP64 (phase_fp64_weights), for the leaves whose pairs
+// reach large k r; the workgroup handles leaf leaf0 + blockIdx.x.
+template <typename T, bool PH>
+__global__ __launch_bounds__(64) void p2p_kernel(int leaf0, const int* leaf_box, DevBoxes bx, const int* rng_ptr,
                                                  const int* rng_start, const int* rng_pref,
                                                  const int* leaf_src_total, const double* X, const double* Y,
                                                  const double* Z, const double2* q, double k, const double2* far,
                                                  const int* perm, int out_mode, double2* out) {
-    __shared__ T sx[64], sy[64], sz[64], sqr[64], sqi[64];
-    const int leaf = blockIdx.x;
+    using C = typename std::conditional<PH, double, T>::type;   // coordinate type
+    __shared__ C sx[64], sy[64], sz[64];
+    __shared__ T sqr[64], sqi[64];
+    const int leaf = leaf0 + blockIdx.x;
     const int b = leaf_box[leaf];
     const int tstart = bx.pstart[b], tcnt = bx.count[b];
     const double cx = bx.centre[3 * b], cy = bx.centre[3 * b + 1], cz = bx.centre[3 * b + 2];
@@ -639,9 +671,13 @@ __global__ __launch_bounds__(64) void p2p_kernel(const int* leaf_box, DevBoxes b
         const int ti = lane & (Tp - 1), g = lane / Tp;
         const bool has_t = ti < tc;
         const int it = tstart + t0 + (has_t ? ti : 0);
-        const T xt = (T)(X[it] - cx), yt = (T)(Y[it] - cy), zt = (T)(Z[it] - cz);
-        T ar = 0, ai = 0;
+        const C xt = (C)(X[it] - cx), yt = (C)(Y[it] - cy), zt = (C)(Z[it] - cz);
+        // per-batch partial sums in T (at most 64 terms per lane), added into FP64 sums:
+        // a single FP32 running sum over thousands of near sources (dense clusters) would
+        // lose sqrt(n) ulps
+        double sr = 0.0, si = 0.0;
         for (int base = 0; base < S; base += 64) {
+            T ar = 0, ai = 0;
             const int v = base + lane;
             if (v < S) {
                 int lo = r0, hi = r1 - 1;
@@ -653,9 +689,9 @@ __global__ __launch_bounds__(64) void p2p_kernel(const int* leaf_box, DevBoxes b
                         hi = mid - 1;
                 }
                 const int idx = rng_start[lo] + (v - rng_pref[lo]);
-                sx[lane] = (T)(X[idx] - cx);
-                sy[lane] = (T)(Y[idx] - cy);
-                sz[lane] = (T)(Z[idx] - cz);
+                sx[lane] = (C)(X[idx] - cx);
+                sy[lane] = (C)(Y[idx] - cy);
+                sz[lane] = (C)(Z[idx] - cz);
                 const double2 qq = q[idx];
                 sqr[lane] = (T)qq.x;
                 sqi[lane] = (T)qq.y;
@@ -663,26 +699,33 @@ __global__ __launch_bounds__(64) void p2p_kernel(const int* leaf_box, DevBoxes b
             __syncthreads();
             const int nb = min(64, S - base);
             for (int j = g; j < nb; j += G) {
-                const T dx = xt - sx[j], dy = yt - sy[j], dz = zt - sz[j];
-                const T r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 > (T)0) {
+                const C dx = xt - sx[j], dy = yt - sy[j], dz = zt - sz[j];
+                const C r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > (C)0) {
                     T wr, wi;
-                    P2PMath<T>::kernel(r2, kk, wr, wi);
+                    if constexpr (PH) {
+                        float rinv, kr;
+                        phase_fp64_weights(r2, k, wr, wi, rinv, kr);
+                    } else {
+                        P2PMath<T>::kernel(r2, kk, wr, wi);
+                    }
                     const T qr = sqr[j], qi = sqi[j];
                     ar += wr * qr - wi * qi;
                     ai += wr * qi + wi * qr;
                 }
             }
+            sr += (double)ar;
+            si += (double)ai;
             __syncthreads();
         }
         for (int off = Tp; off < 64; off <<= 1) {
-            ar += __shfl_xor(ar, off);
-            ai += __shfl_xor(ai, off);
+            sr += __shfl_xor(sr, off);
+            si += __shfl_xor(si, off);
         }
         if (g == 0 && has_t) {
             double2 f = far[it];
-            f.x += (double)ar;
-            f.y += (double)ai;
+            f.x += sr;
+            f.y += si;
             if (out_mode & 1) f.y = -f.y;   // k < 0: conjugate of the |k| result
             if (out_mode & 2) f.y = 0.0;    // k == 0 with real charges: the result is real
             out[perm[it]] = f;
@@ -693,15 +736,17 @@ __global__ __launch_bounds__(64) void p2p_kernel(const int* leaf_box, DevBoxes b
 // P2P with the field: the potential and E = -grad phi of the near field (same
 // decomposition and source ranges as p2p_kernel), added to the far-field values.
 // field[3 i + a] holds component a of point i (input order).
-template <typename T>
-__global__ __launch_bounds__(64) void p2p_field_kernel(const int* leaf_box, DevBoxes bx, const int* rng_ptr,
-                                                       const int* rng_start, const int* rng_pref,
+template <typename T, bool PH>
+__global__ __launch_bounds__(64) void p2p_field_kernel(int leaf0, const int* leaf_box, DevBoxes bx,
+                                                       const int* rng_ptr, const int* rng_start, const int* rng_pref,
                                                        const int* leaf_src_total, const double* X, const double* Y,
                                                        const double* Z, const double2* q, double k,
                                                        const double2* far, const double2* far_e, const int* perm,
                                                        int out_mode, double2* out, double2* field) {
-    __shared__ T sx[64], sy[64], sz[64], sqr[64], sqi[64];
-    const int leaf = blockIdx.x;
+    using C = typename std::conditional<PH, double, T>::type;   // coordinate type
+    __shared__ C sx[64], sy[64], sz[64];
+    __shared__ T sqr[64], sqi[64];
+    const int leaf = leaf0 + blockIdx.x;
     const int b = leaf_box[leaf];
     const int tstart = bx.pstart[b], tcnt = bx.count[b];
     const double cx = bx.centre[3 * b], cy = bx.centre[3 * b + 1], cz = bx.centre[3 * b + 2];
@@ -717,9 +762,10 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(const int* leaf_box, DevB
         const int ti = lane & (Tp - 1), g = lane / Tp;
         const bool has_t = ti < tc;
         const int it = tstart + t0 + (has_t ? ti : 0);
-        const T xt = (T)(X[it] - cx), yt = (T)(Y[it] - cy), zt = (T)(Z[it] - cz);
-        T acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // phi, Ex, Ey, Ez (re, im)
+        const C xt = (C)(X[it] - cx), yt = (C)(Y[it] - cy), zt = (C)(Z[it] - cz);
+        double sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // phi, Ex, Ey, Ez (re, im); FP64 over batches
         for (int base = 0; base < S; base += 64) {
+            T acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             const int v = base + lane;
             if (v < S) {
                 int lo = r0, hi = r1 - 1;
@@ -731,9 +777,9 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(const int* leaf_box, DevB
                         hi = mid - 1;
                 }
                 const int idx = rng_start[lo] + (v - rng_pref[lo]);
-                sx[lane] = (T)(X[idx] - cx);
-                sy[lane] = (T)(Y[idx] - cy);
-                sz[lane] = (T)(Z[idx] - cz);
+                sx[lane] = (C)(X[idx] - cx);
+                sy[lane] = (C)(Y[idx] - cy);
+                sz[lane] = (C)(Z[idx] - cz);
                 const double2 qq = q[idx];
                 sqr[lane] = (T)qq.x;
                 sqi[lane] = (T)qq.y;
@@ -741,41 +787,51 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(const int* leaf_box, DevB
             __syncthreads();
             const int nb = min(64, S - base);
             for (int j = g; j < nb; j += G) {
-                const T dx = xt - sx[j], dy = yt - sy[j], dz = zt - sz[j];
-                const T r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 > (T)0) {
+                const C dx = xt - sx[j], dy = yt - sy[j], dz = zt - sz[j];
+                const C r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > (C)0) {
                     T wr, wi, fr, fi;
-                    field_weights<T>(r2, kk, wr, wi, fr, fi);
+                    if constexpr (PH) {
+                        float rinv, kr;
+                        phase_fp64_weights(r2, k, wr, wi, rinv, kr);
+                        const float rinv2 = rinv * rinv;
+                        fr = (wr + kr * wi) * rinv2;
+                        fi = (wi - kr * wr) * rinv2;
+                    } else {
+                        field_weights<T>(r2, kk, wr, wi, fr, fi);
+                    }
                     const T qr = sqr[j], qi = sqi[j];
                     acc[0] += wr * qr - wi * qi;
                     acc[1] += wr * qi + wi * qr;
                     const T gr = fr * qr - fi * qi, gi = fr * qi + fi * qr;
-                    acc[2] += dx * gr;
-                    acc[3] += dx * gi;
-                    acc[4] += dy * gr;
-                    acc[5] += dy * gi;
-                    acc[6] += dz * gr;
-                    acc[7] += dz * gi;
+                    acc[2] += (T)dx * gr;
+                    acc[3] += (T)dx * gi;
+                    acc[4] += (T)dy * gr;
+                    acc[5] += (T)dy * gi;
+                    acc[6] += (T)dz * gr;
+                    acc[7] += (T)dz * gi;
                 }
             }
+#pragma unroll
+            for (int c = 0; c < 8; ++c) sum[c] += (double)acc[c];
             __syncthreads();
         }
         for (int off = Tp; off < 64; off <<= 1)
 #pragma unroll
-            for (int c = 0; c < 8; ++c) acc[c] += __shfl_xor(acc[c], off);
+            for (int c = 0; c < 8; ++c) sum[c] += __shfl_xor(sum[c], off);
         if (g == 0 && has_t) {
             const int o = perm[it];
             double2 f = far[it];
-            f.x += (double)acc[0];
-            f.y += (double)acc[1];
+            f.x += sum[0];
+            f.y += sum[1];
             if (out_mode & 1) f.y = -f.y;   // k < 0: conjugate of the |k| result
             if (out_mode & 2) f.y = 0.0;    // k == 0 with real charges: the result is real
             out[o] = f;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 double2 e = far_e[3 * (int64_t)it + a];
-                e.x += (double)acc[2 + 2 * a];
-                e.y += (double)acc[3 + 2 * a];
+                e.x += sum[2 + 2 * a];
+                e.y += sum[3 + 2 * a];
                 if (out_mode & 1) e.y = -e.y;
                 if (out_mode & 2) e.y = 0.0;
                 field[3 * (int64_t)o + a] = e;
@@ -918,18 +974,26 @@ void launch_l2p_field(const int* leaves, int n_items, DevBoxes bx, DevLevels lv,
     check_launch("l2p_field");
 }
 
-void launch_p2p_field(int precision, int n_leaves, const int* leaf_box, DevBoxes bx, const int* rng_ptr,
-                      const int* rng_start, const int* rng_pref, const int* leaf_src_total, const double* X,
-                      const double* Y, const double* Z, const double2* q, double k, const double2* far,
-                      const double2* far_e, const int* perm, int out_mode, double2* out, double2* field,
-                      hipStream_t s) {
+void launch_p2p_field(int precision, int n_leaves, int n_fp32, const int* leaf_box, DevBoxes bx,
+                      const int* rng_ptr, const int* rng_start, const int* rng_pref, const int* leaf_src_total,
+                      const double* X, const double* Y, const double* Z, const double2* q, double k,
+                      const double2* far, const double2* far_e, const int* perm, int out_mode, double2* out,
+                      double2* field, hipStream_t s) {
     if (n_leaves <= 0) return;
-    if (precision == 0)
-        hipLaunchKernelGGL(p2p_field_kernel<double>, dim3(n_leaves), dim3(64), 0, s, leaf_box, bx, rng_ptr, rng_start,
-                           rng_pref, leaf_src_total, X, Y, Z, q, k, far, far_e, perm, out_mode, out, field);
-    else
-        hipLaunchKernelGGL(p2p_field_kernel<float>, dim3(n_leaves), dim3(64), 0, s, leaf_box, bx, rng_ptr, rng_start,
-                           rng_pref, leaf_src_total, X, Y, Z, q, k, far, far_e, perm, out_mode, out, field);
+    if (precision == 0) {
+        hipLaunchKernelGGL((p2p_field_kernel<double, false>), dim3(n_leaves), dim3(64), 0, s, 0, leaf_box, bx,
+                           rng_ptr, rng_start, rng_pref, leaf_src_total, X, Y, Z, q, k, far, far_e, perm, out_mode,
+                           out, field);
+    } else {
+        if (n_fp32 > 0)
+            hipLaunchKernelGGL((p2p_field_kernel<float, false>), dim3(n_fp32), dim3(64), 0, s, 0, leaf_box, bx,
+                               rng_ptr, rng_start, rng_pref, leaf_src_total, X, Y, Z, q, k, far, far_e, perm,
+                               out_mode, out, field);
+        if (n_leaves > n_fp32)
+            hipLaunchKernelGGL((p2p_field_kernel<float, true>), dim3(n_leaves - n_fp32), dim3(64), 0, s, n_fp32,
+                               leaf_box, bx, rng_ptr, rng_start, rng_pref, leaf_src_total, X, Y, Z, q, k, far, far_e,
+                               perm, out_mode, out, field);
+    }
     check_launch("p2p_field");
 }
 
@@ -941,17 +1005,23 @@ void launch_xlate(const XlateArgs& a, int n_targets, hipStream_t s) {
     check_launch("xlate");
 }
 
-void launch_p2p(int precision, int n_leaves, const int* leaf_box, DevBoxes bx, const int* rng_ptr,
+void launch_p2p(int precision, int n_leaves, int n_fp32, const int* leaf_box, DevBoxes bx, const int* rng_ptr,
                 const int* rng_start, const int* rng_pref, const int* leaf_src_total, const double* X,
                 const double* Y, const double* Z, const double2* q, double k, const double2* far, const int* perm,
                 int out_mode, double2* out, hipStream_t s) {
     if (n_leaves <= 0) return;
-    if (precision == 0)
-        hipLaunchKernelGGL(p2p_kernel<double>, dim3(n_leaves), dim3(64), 0, s, leaf_box, bx, rng_ptr, rng_start,
-                           rng_pref, leaf_src_total, X, Y, Z, q, k, far, perm, out_mode, out);
-    else
-        hipLaunchKernelGGL(p2p_kernel<float>, dim3(n_leaves), dim3(64), 0, s, leaf_box, bx, rng_ptr, rng_start,
-                           rng_pref, leaf_src_total, X, Y, Z, q, k, far, perm, out_mode, out);
+    if (precision == 0) {
+        hipLaunchKernelGGL((p2p_kernel<double, false>), dim3(n_leaves), dim3(64), 0, s, 0, leaf_box, bx, rng_ptr,
+                           rng_start, rng_pref, leaf_src_total, X, Y, Z, q, k, far, perm, out_mode, out);
+    } else {
+        if (n_fp32 > 0)
+            hipLaunchKernelGGL((p2p_kernel<float, false>), dim3(n_fp32), dim3(64), 0, s, 0, leaf_box, bx, rng_ptr,
+                               rng_start, rng_pref, leaf_src_total, X, Y, Z, q, k, far, perm, out_mode, out);
+        if (n_leaves > n_fp32)
+            hipLaunchKernelGGL((p2p_kernel<float, true>), dim3(n_leaves - n_fp32), dim3(64), 0, s, n_fp32, leaf_box,
+                               bx, rng_ptr, rng_start, rng_pref, leaf_src_total, X, Y, Z, q, k, far, perm, out_mode,
+                               out);
+    }
     check_launch("p2p");
 }
 

@@ -44,6 +44,7 @@ inline size_t charge_bytes(int q_type) {
 
 struct M2LBlockLevel {
     bool use = false;
+    bool fp32 = false;             // kernel precision (M2L of a mixed-precision level)
     int tb = 0;                    // target boxes per workgroup
     int grp = 1;                   // directions per group
     int ps = 0, pt = 0;            // source and target table degrees
@@ -100,6 +101,7 @@ struct hfmm_plan {
     int64_t n_coef = 0;
     int64_t n_p2p_pairs = 0;
     int n_leaves = 0, n_far_leaves = 0, p2m_chunk = 8, p_leafmax = 2;
+    int n_p2p_fp32 = 0;             // leading leaves of the FP32 near field (phase in FP32)
 
     std::vector<void*> allocs;
     double *dX = nullptr, *dY = nullptr, *dZ = nullptr;
@@ -117,6 +119,7 @@ struct hfmm_plan {
     DispInfo* d_dinfo = nullptr;
     XlateLaunch m2l[kMaxLevels], m2m[kMaxLevels], l2l[kMaxLevels];
     M2LBlockLevel m2lb[kMaxLevels];
+    bool m2l_fp32[kMaxLevels] = {};  // M2L of the level in FP32 (mixed precision)
     M2LBlockLevel m2mb[kMaxLevels], l2lb[kMaxLevels];   // M2M / L2L on matrix cores (FP64)
     int m2l_kernel = 0;
     size_t max_lds = 65536;
@@ -299,7 +302,7 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     if (P.m2l_kernel == 1) return;
     const int p = P.p_level[L];
     if (p > kM2LBlockMaxP) return;   // higher degrees use the generic kernel (coarse levels only)
-    const int fp32 = P.precision == HFMM_PREC_MIXED;
+    const int fp32 = P.m2l_fp32[L];
     const int eb = fp32 ? 4 : 8;
     const int maxu = m2l_block_maxu(p);   // the instantiation's stage-3 units per wave
     // shape of one operator application: TB target boxes x G directions (of equal theta
@@ -372,6 +375,7 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     const std::vector<int4> items = make_items(t, blk_targets, blk_ptr, tb, true);
     M2LBlockLevel& B = P.m2lb[L];
     B.use = true;
+    B.fp32 = fp32;
     B.tb = tb;
     B.grp = grp;
     B.ps = B.pt = p;
@@ -735,28 +739,61 @@ void build_p2p(hfmm_plan& P) {
     csr(ls.u_tgt, ls.u_src, up, us);
     csr(ls.w_tgt, ls.w_src, wp, ws);
     csr(ls.x_tgt, ls.x_src, xp, xs);
+    // box centre and half diagonal, for the bound on k r over a leaf's pairs
+    auto geom = [&](int b, double c[3]) {
+        int g[3];
+        morton_grid(t.code[b], g[0], g[1], g[2]);
+        const double w = t.extent * std::ldexp(1.0, -t.level[b]);
+        for (int a = 0; a < 3; ++a) c[a] = (g[a] + 0.5) * w + t.pmin[a];
+        return 0.5 * std::sqrt(3.0) * w;
+    };
+    // leaves whose pairs stay within k r <= kP2PFp32MaxKr first (FP32 near field), then the
+    // rest (FP32 with the phase in FP64); Morton order within each class
+    std::vector<int> leaves[2];
+    for (int b = 0; b < nb; ++b) {
+        if (!t.is_leaf[b]) continue;
+        double kr = 0.0;
+        if (P.precision == HFMM_PREC_MIXED) {
+            double cb[3], cs[3];
+            const double hb = geom(b, cb);
+            auto reach = [&](int box) {
+                const double hs = geom(box, cs);
+                const double d = std::sqrt((cb[0] - cs[0]) * (cb[0] - cs[0]) + (cb[1] - cs[1]) * (cb[1] - cs[1]) +
+                                           (cb[2] - cs[2]) * (cb[2] - cs[2]));
+                kr = std::max(kr, P.k * (d + hb + hs));
+            };
+            reach(b);
+            for (int e = up[b]; e < up[b + 1]; ++e) reach(us[e]);
+            for (int e = wp[b]; e < wp[b + 1]; ++e) reach(ws[e]);
+            for (int a = b; a >= 0; a = t.parent[a])
+                for (int e = xp[a]; e < xp[a + 1]; ++e) reach(xs[e]);
+        }
+        leaves[kr > kP2PFp32MaxKr].push_back(b);
+    }
+    P.n_p2p_fp32 = (int)leaves[0].size();
     std::vector<int> leaf_box, rng_ptr{0}, rng_start, rng_pref, total;
     std::vector<int> far_leaves;
     int64_t pairs = 0;
-    for (int b = 0; b < nb; ++b) {
-        if (!t.is_leaf[b]) continue;
-        leaf_box.push_back(b);
-        if (t.level[b] >= 2) far_leaves.push_back(b);
-        int acc = 0;
-        auto add = [&](int box) {
-            rng_start.push_back(t.pstart[box]);
-            rng_pref.push_back(acc);
-            acc += t.count[box];
-        };
-        add(b);
-        for (int e = up[b]; e < up[b + 1]; ++e) add(us[e]);
-        for (int e = wp[b]; e < wp[b + 1]; ++e) add(ws[e]);
-        for (int a = b; a >= 0; a = t.parent[a])
-            for (int e = xp[a]; e < xp[a + 1]; ++e) add(xs[e]);
-        rng_ptr.push_back((int)rng_start.size());
-        total.push_back(acc);
-        pairs += (int64_t)acc * t.count[b];
-    }
+    for (int b = 0; b < nb; ++b)
+        if (t.is_leaf[b] && t.level[b] >= 2) far_leaves.push_back(b);
+    for (int cls = 0; cls < 2; ++cls)
+        for (int b : leaves[cls]) {
+            leaf_box.push_back(b);
+            int acc = 0;
+            auto add = [&](int box) {
+                rng_start.push_back(t.pstart[box]);
+                rng_pref.push_back(acc);
+                acc += t.count[box];
+            };
+            add(b);
+            for (int e = up[b]; e < up[b + 1]; ++e) add(us[e]);
+            for (int e = wp[b]; e < wp[b + 1]; ++e) add(ws[e]);
+            for (int a = b; a >= 0; a = t.parent[a])
+                for (int e = xp[a]; e < xp[a + 1]; ++e) add(xs[e]);
+            rng_ptr.push_back((int)rng_start.size());
+            total.push_back(acc);
+            pairs += (int64_t)acc * t.count[b];
+        }
     P.n_leaves = (int)leaf_box.size();
     P.n_far_leaves = (int)far_leaves.size();
     P.n_p2p_pairs = pairs;
@@ -811,6 +848,9 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
         P.R[L] = std::sqrt(3.0) * P.h[L];
         if (L >= 1 && L <= t.depth) P.pmax = std::max(P.pmax, P.p_level[L]);
         P.p_eff[L] = P.p_level[L];
+        const double kd = P.k * 2.0 * P.R[L];   // k times the box diagonal
+        P.m2l_fp32[L] = o.precision == HFMM_PREC_MIXED &&
+                        (o.m2l_fp64_levels < 0 ? kd <= HFMM_MIXED_FP32_MAX_KD : !((o.m2l_fp64_levels >> L) & 1));
     }
     HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
     // sorted positions (SoA, FP64) and permutation
@@ -861,7 +901,9 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
         if (t.is_leaf[b]) P.p_leafmax = std::max(P.p_leafmax, P.p_level[t.level[b]]);
     P.p2m_chunk = (int)std::max<size_t>(1, std::min<size_t>(64, 48 * 1024 / p2m_bytes_per_point(P.p_leafmax)));
     P.dM = P.alloc<double2>(P.n_coef);
-    if (P.precision == HFMM_PREC_MIXED) P.dM32 = P.alloc<float2>(P.n_coef);
+    bool any_fp32 = false;
+    for (int L = 0; L < kMaxLevels; ++L) any_fp32 = any_fp32 || (P.m2lb[L].use && P.m2lb[L].fp32);
+    if (any_fp32) P.dM32 = P.alloc<float2>(P.n_coef);
     P.dL = P.alloc<double2>(P.n_coef);
     P.d_q = P.alloc<double2>(n);
     P.d_far = P.alloc<double2>(n);
@@ -957,7 +999,6 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
             launch_xlate(xargs(P, P.m2m[L], P.dM, P.dM, 0), P.m2m[L].n_targets, s);
     }
     mark(3);
-    const bool fp32 = P.precision == HFMM_PREC_MIXED;
     bool converted = false;
     for (int L = 2; L <= P.depth; ++L) {
         if (!P.m2l[L].n_targets) continue;
@@ -968,13 +1009,14 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
             if (lev) HIPCHK(hipEventRecord(lev[2 * L + 1], s));
             continue;
         }
-        if (fp32 && !converted) {
+        if (B.fp32 && !converted) {
             launch_to_float(P.dM, P.dM32, P.n_coef, s);
             converted = true;
         }
-        const M2LBlockArgs a = block_args(P, B, fp32, fp32 ? (const void*)P.dM32 : (const void*)P.dM, P.dL, 0);
+        const M2LBlockArgs a =
+            block_args(P, B, B.fp32, B.fp32 ? (const void*)P.dM32 : (const void*)P.dM, P.dL, 0);
         if (lev) HIPCHK(hipEventRecord(lev[2 * L], s));
-        launch_m2l_block(a, B.tb, B.grp, fp32, s);
+        launch_m2l_block(a, B.tb, B.grp, B.fp32, s);
         if (lev) HIPCHK(hipEventRecord(lev[2 * L + 1], s));
     }
     mark(4);
@@ -991,13 +1033,13 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
         launch_l2p_field(P.d_far_leaves, P.n_far_leaves, bx, lv, P.dX, P.dY, P.dZ, P.dL, P.k, P.p_leafmax, P.d_far,
                          P.d_far_e, s);
         mark(6);
-        launch_p2p_field(P.precision, P.n_leaves, P.d_leaf_box, bx, P.d_rng_ptr, P.d_rng_start, P.d_rng_pref,
+        launch_p2p_field(P.precision, P.n_leaves, P.n_p2p_fp32, P.d_leaf_box, bx, P.d_rng_ptr, P.d_rng_start, P.d_rng_pref,
                          P.d_leaf_src_total, P.dX, P.dY, P.dZ, P.d_q, P.k, P.d_far, P.d_far_e, P.d_perm, out_mode,
                          out, field, s);
     } else {
         launch_l2p(P.d_far_leaves, P.n_far_leaves, bx, lv, P.dX, P.dY, P.dZ, P.dL, P.k, P.pmax, P.d_far, s);
         mark(6);
-        launch_p2p(P.precision, P.n_leaves, P.d_leaf_box, bx, P.d_rng_ptr, P.d_rng_start, P.d_rng_pref,
+        launch_p2p(P.precision, P.n_leaves, P.n_p2p_fp32, P.d_leaf_box, bx, P.d_rng_ptr, P.d_rng_start, P.d_rng_pref,
                    P.d_leaf_src_total, P.dX, P.dY, P.dZ, P.d_q, P.k, P.d_far, P.d_perm, out_mode, out, s);
     }
     mark(7);
@@ -1088,6 +1130,7 @@ void hfmm_opts_default(hfmm_opts* o) {
     o->precision = HFMM_PREC_FP64;
     o->use_graph = 1;
     o->m2l_kernel = 0;
+    o->m2l_fp64_levels = -1;
 }
 
 int hfmm_plan_create(const double* xyz, int64_t n, double k, const hfmm_opts* o, hfmm_plan** out) {
@@ -1113,6 +1156,7 @@ int hfmm_plan_info_get(const hfmm_plan* P, hfmm_plan_info* info) {
         info->depth = P->depth;
         info->n_boxes = P->tree.n_boxes();
         info->n_leaves = P->n_leaves;
+        info->n_leaves_phase64 = P->precision == HFMM_PREC_MIXED ? P->n_leaves - P->n_p2p_fp32 : 0;
         info->maxleaf = P->tree.maxleaf;
         info->n_v = (int64_t)P->lists.v_tgt.size();
         info->n_u = (int64_t)P->lists.u_tgt.size();
@@ -1122,6 +1166,7 @@ int hfmm_plan_info_get(const hfmm_plan* P, hfmm_plan_info* info) {
         for (int L = 0; L < HFMM_MAX_LEVELS; ++L) {
             info->p_level[L] = (L >= 1 && L <= P->depth) ? P->p_eff[L] : 0;
             info->p_table[L] = (L >= 1 && L <= P->depth) ? P->p_level[L] : 0;
+            if (P->m2lb[L].use && P->m2lb[L].fp32) info->m2l_fp32_levels |= 1 << L;
         }
         info->extent = P->tree.extent;
         for (int a = 0; a < 3; ++a) info->pmin[a] = P->tree.pmin[a];

@@ -32,7 +32,8 @@ extern "C" {
 /* precision of the evaluate path */
 #define HFMM_PREC_FP64 0       /* FP64 everywhere (the plain FMM) */
 #define HFMM_PREC_MIXED 1      /* FP32 near field on leaf-centred coordinates and FP32 M2L on
-                                  matrix cores; P2M/M2M/L2L/L2P and accumulation in FP64 */
+                                  matrix cores (at the levels hfmm_opts.m2l_fp64_levels leaves
+                                  in FP32); P2M/M2M/L2L/L2P and accumulation in FP64 */
 
 /* pointer kinds for q / out */
 #define HFMM_HOST 0
@@ -59,7 +60,16 @@ typedef struct hfmm_opts {
                                        directions of equal theta and |d|) where it fits */
     int tree_device;                /* 1: build the octree and lists on the GPU (identical result);
                                        0: on the host */
+    int m2l_fp64_levels;            /* HFMM_PREC_MIXED only: bit l set keeps the M2L of level l in
+                                       FP64; -1 (the default): automatic, FP64 at the levels whose
+                                       box diagonal D_l has k D_l > HFMM_MIXED_FP32_MAX_KD */
 } hfmm_opts;
+
+/* Largest k D_l (D_l = 2 sqrt(3) extent 2^-(l+1), the box diagonal of level l) whose M2L the
+ * automatic rule of HFMM_PREC_MIXED runs in FP32.  Above it the translated sums cancel
+ * (terms of |h_n(k r)| j_n growth much larger than their sum) and FP32 rounding reaches the
+ * 1e-6 error budget. */
+#define HFMM_MIXED_FP32_MAX_KD 32.0
 
 typedef struct hfmm_plan hfmm_plan;
 
@@ -75,6 +85,10 @@ typedef struct hfmm_plan_info {
     double extent;
     double pmin[3];
     int p_table[HFMM_MAX_LEVELS];   /* degree the level's tables were built for */
+    int m2l_fp32_levels;            /* bit l set: the M2L of level l runs in FP32 */
+    int n_leaves_phase64;           /* HFMM_PREC_MIXED: leaves whose FP32 near field takes the
+                                       coordinates and the phase k r in FP64 (pairs reaching
+                                       k r > 8, where an FP32 phase errs by ~k r 2^-23 rad) */
 } hfmm_plan_info;
 
 void hfmm_opts_default(hfmm_opts* o);

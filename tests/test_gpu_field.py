@@ -61,3 +61,23 @@ def test_field_converges_with_degree():
         errs.append(_rel(E[idx], ref))
     print(f"  field errors {errs}", flush=True)
     assert errs[1] < errs[0] / 5 and errs[2] < errs[1] / 5 and errs[2] < 5e-7
+
+
+def test_field_mixed_near_field_phase64():
+    """Mixed precision at k r up to 130 between neighbour leaves (k = 300, leaves of width
+    1/8): every leaf takes the FP64 coordinates and phase.  With every M2L level kept in
+    FP64 the near field is the only FP32 part, so phi and E sit within the FP32 rounding of
+    the 1/r weights (~1e-7) of the FP64 plan at the same degrees; an FP32 phase would err
+    by about k r 2^-23 ~ 1e-5 on these pairs."""
+    import hfmm0_amd as hf
+    print("case: field mixed phase64 n=20000 k=300 p=14 leaf=48", flush=True)
+    x, q = D.uniform_problem(20000)
+    ref = hf.setup(x, 300.0, p=14, N_max=48)
+    lv = list(range(2, ref.depth + 1))
+    st = hf.setup(x, 300.0, p=14, N_max=48, precision="mixed", m2l_fp64_levels=lv)
+    assert st.info.n_leaves_phase64 == st.info.n_leaves > 0
+    phi64, E64 = hf.evaluate_field(ref, q)
+    phi, E = hf.evaluate_field(st, q)
+    dp, de = _rel(phi, phi64), _rel(E, E64)
+    print(f"  vs fp64 plan: phi {dp:.2e} E {de:.2e}", flush=True)
+    assert dp < 3e-7 and de < 3e-7

@@ -208,6 +208,42 @@ def test_matrix_core_m2l_fp32(hf):
     assert ea < eb + 3e-6
 
 
+def test_mixed_precision_per_level(hf):
+    """M2L precision per level under precision="mixed": the automatic rule keeps levels with
+    k D_l > MIXED_FP32_MAX_KD (D_l the box diagonal) in FP64; listing every level in
+    m2l_fp64_levels leaves only the FP32 near field, so the result then sits within the
+    FP32 P2P rounding of the FP64 plan, and a single FP32 level sits between the two."""
+    from hfmm0_amd import api
+    say("case: mixed per-level precision N=30000 k=80 p=14 leaf=32")
+    x, q = D.uniform_problem(30000)
+    k = 80.0
+    base = dict(p=14, N_max=32)
+    auto = hf.setup(x, k, precision="mixed", **base)
+    ext = auto.info.extent
+    levels = [lv for lv in range(2, auto.depth + 1)]
+    want = [lv for lv in levels if k * 2 * np.sqrt(3) * ext * 0.5 ** (lv + 1) <= api.MIXED_FP32_MAX_KD]
+    say(f"  depth {auto.depth} fp32 levels {auto.m2l_fp32_levels} want {want}")
+    assert auto.m2l_fp32_levels == want and 2 not in want
+    ref = hf.evaluate(hf.setup(x, k, precision="fp64", **base), q)
+    all64 = hf.setup(x, k, precision="mixed", m2l_fp64_levels=levels, **base)
+    assert all64.m2l_fp32_levels == []
+    # neighbour leaves of width 1/16 reach k r = 80 * 2 sqrt(3) / 16 = 17 > 8: every leaf
+    # takes the FP64 phase
+    assert all64.info.n_leaves_phase64 == all64.info.n_leaves > 0
+    d_all64 = D.relative_error(hf.evaluate(all64, q), ref)
+    one = hf.setup(x, k, precision="mixed", m2l_fp64_levels=[lv for lv in levels if lv != 3], **base)
+    assert one.m2l_fp32_levels == [3]
+    d_one = D.relative_error(hf.evaluate(one, q), ref)
+    none64 = hf.setup(x, k, precision="mixed", m2l_fp64_levels=[], **base)
+    assert none64.m2l_fp32_levels == levels
+    d_none = D.relative_error(hf.evaluate(none64, q), ref)
+    say(f"  vs fp64 plan: all fp64 {d_all64:.2e} level 3 fp32 {d_one:.2e} all fp32 {d_none:.2e}")
+    assert d_all64 < 3e-7
+    assert d_all64 < d_one < d_none < 3e-6
+    with pytest.raises(ValueError):
+        hf.setup(x, k, precision="mixed", m2l_fp64_levels=[0], **base)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_set_degrees_equals_dedicated_plan(hf, precision):
     say(f"case: set_degrees {precision} N=20000 k=12 tables p=14 -> [12, 11, 9]")
