@@ -116,10 +116,15 @@ __host__ __device__ constexpr int m2l_block_maxu(int p) {
 __host__ __device__ constexpr int m2l_acc_regs(int maxu, int tb, int elem_bytes) {
     return maxu * 2 * (2 * tb / 16) * 4 * (elem_bytes / 4);
 }
-// LDS elements of the expansion block: (p+1)^2 rows x ncol + 1 columns, one scratch row
-// that receives the stores of rows outside a block, and 16 zero rows that the reads of
-// the last block's padding run into; padded to 16 bytes
-__host__ __device__ constexpr int m2l_lds_x_rows(int p) { return (p + 1) * (p + 1) + 17; }
+// LDS rows of the expansion block: (p+1)^2 expansion rows, a scratch row that receives
+// the stores of rows outside a block, a zero row (read for orders beyond the evaluated
+// degree), and as many further zero rows as the padded tiles of the last u block read:
+// those reads end at row p^2 + p + 16 m2l_tiles(p+1) - 1.  (At p = 11 in FP32 with 16 x 2
+// pairs per group: 148 rows, 38.5 KB, four workgroups per CU.)
+__host__ __device__ constexpr int m2l_lds_x_rows(int p) {
+    return (p + 1) * (p + 1) + 2 > p * p + p + 16 * m2l_tiles(p + 1) ? (p + 1) * (p + 1) + 2
+                                                                      : p * p + p + 16 * m2l_tiles(p + 1);
+}
 // phase table of the M2L block kernel: entries per (lane parity, direction) for the
 // orders m = 0 .. 16 KT - 1 (KT = m2l_tiles(p + 1))
 __host__ __device__ constexpr int m2l_phase_len(int p) { return 16 * m2l_tiles(p + 1); }

@@ -94,8 +94,18 @@ __device__ inline void decode_q(int q, int& n, int& m) {
 // items (coarse levels, to fill the GPU) the results are added atomically.  KT is the
 // number of 16-row tiles of the largest (u or v) block, m2l_tiles(p_tab + 1) (1 up to
 // p = 15, 2 up to 23); it bounds the A fragments a unit keeps in registers.
+// waves per SIMD the compiler fits the registers to: FP32 with 16 x 2 pairs per group and
+// one tile per block (the finest levels) runs faster at 4 (128 VGPRs, a few spilled values:
+// 5.13 against 5.32 ms at p = 11, N = 1M, k = 40) than at the 3 its 163 VGPRs allow; the
+// other instances keep the compiler's choice (FP64 at 4 spills heavily and runs 35% slower)
+template <typename T, int TB, int G, int KT>
+constexpr int m2l_block_min_waves() {
+    return sizeof(T) == 4 && TB == 16 && G == 2 && KT == 1 ? 4 : 1;
+}
+
 template <typename T, int TB, int G, int KT, int MAXU>
-__global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(m2l_block_min_waves<T, TB, G, KT>())))
+void m2l_block_kernel(M2LBlockArgs a) {
     using V4 = typename Mma<T>::v4;
     using C2 = typename Mma<T>::c2;
     constexpr int kMaxTiles = KT;           // row tiles of a rotation block or an order block
@@ -148,7 +158,7 @@ __global__ __launch_bounds__(256) void m2l_block_kernel(M2LBlockArgs a) {
     // zero the whole block once: rows that the padded tiles read beyond the gathered
     // degrees are then finite (stale results of earlier groups) or zero, and their
     // products with the zero padding of A vanish
-    const int zero_row = spare + 1;   // the 16 rows after the scratch row stay zero
+    const int zero_row = spare + 1;   // the rows after the scratch row stay zero
     for (int i = threadIdx.x; i < m2l_lds_x_rows(ptab) * LDX; i += 256) X[i] = (T)0;
     // with PF: the wave's units of each stage in its lanes (lane i: unit wave + 4 i), as
     // fragment offset << 9 | code, 255 for a unit to skip; read with v_readlane, so stepping
