@@ -95,9 +95,11 @@ __device__ inline void decode_q(int q, int& n, int& m) {
 // number of 16-row tiles of the largest (u or v) block, m2l_tiles(p_tab + 1) (1 up to
 // p = 15, 2 up to 23); it bounds the A fragments a unit keeps in registers.
 // waves per SIMD the compiler fits the registers to: FP32 with 16 x 2 pairs per group and
-// one tile per block (the finest levels) runs faster at 4 (128 VGPRs, a few spilled values:
-// 5.13 against 5.32 ms at p = 11, N = 1M, k = 40) than at the 3 its 163 VGPRs allow; the
-// other instances keep the compiler's choice (FP64 at 4 spills heavily and runs 35% slower)
+// one tile per block (the finest levels) runs faster at 4 (128 VGPRs with 100 bytes per
+// lane spilled to scratch, mostly around the stage-3 accumulation; 5.0 against 5.3 ms at
+// p = 11, N = 1M, k = 40) than at the 3 its 156 VGPRs allow.  The spill stores reach HBM
+// (about 2.5 GB written per such launch), which the traffic counters show.  The other
+// instances keep the compiler's choice (FP64 at 4 spills heavily and runs 35% slower).
 template <typename T, int TB, int G, int KT>
 constexpr int m2l_block_min_waves() {
     return sizeof(T) == 4 && TB == 16 && G == 2 && KT == 1 ? 4 : 1;
@@ -395,12 +397,17 @@ void m2l_block_kernel(M2LBlockArgs a) {
                         rowp[j] = __umul24(ok ? q + am : zero_row, LDX);
                         rowm[j] = __umul24(ok ? q - am : zero_row, LDX);
                     }
-                    V4 c4[2][NCT];
+                    const int nr0 = am + Mma<T>::row(lk, 0);
+                    const int qr = __umul24(nr0, nr0 + 1);
+                    // the order +m (sg 0), then -m: each result is stored before the other
+                    // order's rows are read (the rows n^2+n+m and n^2+n-m are disjoint), so
+                    // only one set of output tiles is live
 #pragma unroll
                     for (int sg = 0; sg < 2; ++sg) {
-#pragma unroll
-                        for (int ct = 0; ct < NCT; ++ct) c4[sg][ct] = V4{0, 0, 0, 0};
                         if (sg < ns) {
+                            V4 c4[NCT];
+#pragma unroll
+                            for (int ct = 0; ct < NCT; ++ct) c4[ct] = V4{0, 0, 0, 0};
                             T b0[4][NCT], b1[4][NCT];
 #pragma unroll
                             for (int j = 0; j < 4; ++j) {
@@ -416,25 +423,19 @@ void m2l_block_kernel(M2LBlockArgs a) {
                                 if (j < nks)
 #pragma unroll
                                     for (int ct = 0; ct < NCT; ++ct) {
-                                        c4[sg][ct] = Mma<T>::mma(are[j], b0[j][ct], c4[sg][ct]);
-                                        c4[sg][ct] = Mma<T>::mma(aim[j], b1[j][ct], c4[sg][ct]);
+                                        c4[ct] = Mma<T>::mma(are[j], b0[j][ct], c4[ct]);
+                                        c4[ct] = Mma<T>::mma(aim[j], b1[j][ct], c4[ct]);
                                     }
-                        }
-                    }
-                    const int nr0 = am + Mma<T>::row(lk, 0);
-                    const int qr = __umul24(nr0, nr0 + 1);
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int sr = r * RSTEP;
-                        const int q = qr + __umul24(2 * sr, nr0) + sr * sr + sr;
-                        const bool ok = Mma<T>::row(lk, r) < lent;
-#pragma unroll
-                        for (int sg = 0; sg < 2; ++sg)
-                            if (sg < ns) {
+                            for (int r = 0; r < 4; ++r) {
+                                const int sr = r * RSTEP;
+                                const int q = qr + __umul24(2 * sr, nr0) + sr * sr + sr;
+                                const bool ok = Mma<T>::row(lk, r) < lent;
                                 const int dst = __umul24(ok ? (sg ? q - am : q + am) : spare, LDX);
 #pragma unroll
-                                for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[sg][ct][r];
+                                for (int ct = 0; ct < NCT; ++ct) X[dst + ct * 16 + li] = c4[ct][r];
                             }
+                        }
                     }
                 }
             } else {
@@ -556,14 +557,6 @@ void m2l_block_kernel(M2LBlockArgs a) {
                     n3 = nx;
                     if (nx < MAXU) pre(nx);
                     V4 cu[NCT], cv[NCT];
-                    T bu[4][NCT], bw[4][NCT];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-#pragma unroll
-                        for (int ct = 0; ct < NCT; ++ct) {
-                            bu[j][ct] = X[(n * n + n + 4 * j + lk) * LDX + ct * 16 + li];
-                            bw[j][ct] = X[(n * n + 4 * j + lk) * LDX + ct * 16 + li];
-                        }
 #pragma unroll
                     for (int ct = 0; ct < NCT; ++ct) {
                         cu[ct] = V4{0, 0, 0, 0};
@@ -571,12 +564,18 @@ void m2l_block_kernel(M2LBlockArgs a) {
                     }
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
+                        T bu[NCT], bw[NCT];
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ++ct) {
+                            bu[ct] = X[(n * n + n + 4 * j + lk) * LDX + ct * 16 + li];
+                            bw[ct] = X[(n * n + 4 * j + lk) * LDX + ct * 16 + li];
+                        }
                         if (j < nku)
 #pragma unroll
-                            for (int ct = 0; ct < NCT; ++ct) cu[ct] = Mma<T>::mma(avu[j], bu[j][ct], cu[ct]);
+                            for (int ct = 0; ct < NCT; ++ct) cu[ct] = Mma<T>::mma(avu[j], bu[ct], cu[ct]);
                         if (j < nkv)
 #pragma unroll
-                            for (int ct = 0; ct < NCT; ++ct) cv[ct] = Mma<T>::mma(avv[j], bw[j][ct], cv[ct]);
+                            for (int ct = 0; ct < NCT; ++ct) cv[ct] = Mma<T>::mma(avv[j], bw[ct], cv[ct]);
                     }
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
