@@ -101,6 +101,34 @@ class Batch:
         L.check(L.lib().hfmm_batch_benchmark(self.handle, qp, qt, op, int(warmup), int(reps), ctypes.byref(med)))
         return med.value
 
+    def certify(self, charges_list, tol=1e-6, n_samp=256, seed=0, max_steps=4):
+        """Evaluate and certify every problem on n_samp sampled targets against the GPU FP64
+        direct sum; a problem that misses tol gets one more degree at every level (within
+        its table degrees, through ``set_degrees``: no new setup) and the batch is evaluated
+        again, at most max_steps times -- the batch counterpart of the planned evaluator's
+        climb (the role of the reference's ``jaxfmm/tune.py:375-516`` ladder).  Returns
+        (phi list, per-problem degrees of levels 1..depth, per-problem errors)."""
+        from .learned import sampled_error
+        plans = [self.plan(i) for i in range(len(self))]
+        phis = self.evaluate(charges_list)
+        errs = [sampled_error(x, q, k, phi, n_samp, seed)
+                for x, q, k, phi in zip(self._xs, charges_list, self.k, phis)]
+        for _ in range(max_steps):
+            bad = [i for i, e in enumerate(errs) if e > tol]
+            raised = []
+            for i in bad:
+                cur, tab = plans[i].p_level, plans[i].p_table
+                new = [min(c + 1, t) for c, t in zip(cur, tab)]
+                if new != cur:
+                    plans[i].set_degrees(new)
+                    raised.append(i)
+            if not raised:
+                break
+            phis = self.evaluate(charges_list)
+            for i in raised:
+                errs[i] = sampled_error(self._xs[i], charges_list[i], self.k[i], phis[i], n_samp, seed)
+        return phis, [pl.p_level for pl in plans], errs
+
     def close(self):
         if self._h is not None:
             L.lib().hfmm_batch_destroy(self._h)

@@ -72,9 +72,10 @@ void launch_xlate(const XlateArgs& a, int n_targets, hipStream_t s);
 void launch_l2p_field(const int* leaves, int n_items, DevBoxes bx, DevLevels lv, const double* X, const double* Y,
                       const double* Z, const double2* L, double k, int pmax, double2* far, double2* far_e,
                       hipStream_t s);
-// Near field over the leaves in leaf_box order.  FP32 (precision != 0): the first n_fp32
-// leaves with FP32 coordinates and phase, the rest with coordinates and the phase k r in
-// FP64 (their pairs reach k r > kP2PFp32MaxKr); FP64: every leaf in FP64.
+// Near field over the leaves in leaf_box order, coordinate differences and r^2 in FP64.
+// FP32 (precision != 0): FP32 weights; the first n_fp32 leaves with an FP32 phase k r, the
+// rest with the phase in FP64 (their pairs reach k r > kP2PFp32MaxKr).  FP64: every leaf
+// in FP64.
 constexpr double kP2PFp32MaxKr = 8.0;
 void launch_p2p_field(int precision, int n_leaves, int n_fp32, const int* leaf_box, DevBoxes bx,
                       const int* rng_ptr, const int* rng_start, const int* rng_pref, const int* leaf_src_total,

@@ -641,18 +641,23 @@ __device__ inline void field_weights(T r2, T k, T& wr, T& wi, T& fr, T& fi) {
 // One wave per target leaf.  Targets of a chunk of up to 64 points are spread over
 // lane groups: with T' = next power of two >= chunk size, lane l holds target l % T'
 // and scans the sources j == l / T' (mod 64/T') of each 64-point source batch staged
-// in LDS (coordinates relative to the target leaf centre, in T).  Lane-group partial
+// in LDS (coordinates relative to the target leaf centre, in FP64).  Lane-group partial
 // sums are combined with wavefront shuffles.  Sources of a leaf are a list of point
-// ranges (adjacent leaves, and the W/X interactions evaluated directly).  PH (FP32 only):
-// coordinates and the phase k r in FP64 (phase_fp64_weights), for the leaves whose pairs
-// reach large k r; the workgroup handles leaf leaf0 + blockIdx.x.
+// ranges (adjacent leaves, and the W/X interactions evaluated directly).  T = float: r^2
+// is rounded to FP32 for the weights; PH: the phase k r in FP64 as well
+// (phase_fp64_weights), for the leaves whose pairs reach large k r.  The workgroup
+// handles leaf leaf0 + blockIdx.x.
 template <typename T, bool PH>
 __global__ __launch_bounds__(64) void p2p_kernel(int leaf0, const int* leaf_box, DevBoxes bx, const int* rng_ptr,
                                                  const int* rng_start, const int* rng_pref,
                                                  const int* leaf_src_total, const double* X, const double* Y,
                                                  const double* Z, const double2* q, double k, const double2* far,
                                                  const int* perm, int out_mode, double2* out) {
-    using C = typename std::conditional<PH, double, T>::type;   // coordinate type
+    // coordinates relative to the leaf centre and r^2 in FP64 at every precision: FP32
+    // coordinates would carry an absolute error of 2^-24 times the leaf size, a large
+    // relative error of r for close pairs (points on surfaces); FP64 adds and FMAs issue
+    // at the rate of the (unpacked) FP32 ones
+    using C = double;
     __shared__ C sx[64], sy[64], sz[64];
     __shared__ T sqr[64], sqi[64];
     const int leaf = leaf0 + blockIdx.x;
@@ -707,7 +712,7 @@ __global__ __launch_bounds__(64) void p2p_kernel(int leaf0, const int* leaf_box,
                         float rinv, kr;
                         phase_fp64_weights(r2, k, wr, wi, rinv, kr);
                     } else {
-                        P2PMath<T>::kernel(r2, kk, wr, wi);
+                        P2PMath<T>::kernel((T)r2, kk, wr, wi);
                     }
                     const T qr = sqr[j], qi = sqi[j];
                     ar += wr * qr - wi * qi;
@@ -743,7 +748,11 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(int leaf0, const int* lea
                                                        const double* Z, const double2* q, double k,
                                                        const double2* far, const double2* far_e, const int* perm,
                                                        int out_mode, double2* out, double2* field) {
-    using C = typename std::conditional<PH, double, T>::type;   // coordinate type
+    // coordinates relative to the leaf centre and r^2 in FP64 at every precision: FP32
+    // coordinates would carry an absolute error of 2^-24 times the leaf size, a large
+    // relative error of r for close pairs (points on surfaces); FP64 adds and FMAs issue
+    // at the rate of the (unpacked) FP32 ones
+    using C = double;
     __shared__ C sx[64], sy[64], sz[64];
     __shared__ T sqr[64], sqi[64];
     const int leaf = leaf0 + blockIdx.x;
@@ -798,7 +807,7 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(int leaf0, const int* lea
                         fr = (wr + kr * wi) * rinv2;
                         fi = (wi - kr * wr) * rinv2;
                     } else {
-                        field_weights<T>(r2, kk, wr, wi, fr, fi);
+                        field_weights<T>((T)r2, kk, wr, wi, fr, fi);
                     }
                     const T qr = sqr[j], qi = sqi[j];
                     acc[0] += wr * qr - wi * qi;

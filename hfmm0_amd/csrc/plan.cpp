@@ -102,6 +102,7 @@ struct hfmm_plan {
     int64_t n_p2p_pairs = 0;
     int n_leaves = 0, n_far_leaves = 0, p2m_chunk = 8, p_leafmax = 2;
     int n_p2p_fp32 = 0;             // leading leaves of the FP32 near field (phase in FP32)
+    unsigned degree_version = 0;    // bumped by set_degrees (graphs that captured this plan are stale)
 
     std::vector<void*> allocs;
     double *dX = nullptr, *dY = nullptr, *dZ = nullptr;
@@ -1201,6 +1202,7 @@ int hfmm_plan_set_degrees(hfmm_plan* P, const int* p_level) {
         }
         HIPCHK(hipStreamSynchronize(P->stream));
         std::copy(eff, eff + kMaxLevels, P->p_eff);
+        ++P->degree_version;
         HIPCHK(hipMemcpy(P->d_plevel, P->p_eff, sizeof(int) * kMaxLevels, hipMemcpyHostToDevice));
         if (P->gexec) {
             HIPCHK(hipGraphExecDestroy(P->gexec));
@@ -1525,6 +1527,7 @@ struct hfmm_batch {
     hipGraphExec_t gexec = nullptr;
     std::vector<const void*> g_q;
     std::vector<void*> g_out;
+    std::vector<unsigned> g_ver;            // degree versions of the plans at capture
     int g_qtype = -1;
     ~hfmm_batch() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -1555,7 +1558,8 @@ void batch_sequence(hfmm_batch& B, const void* const* q, int q_type, void* const
 void batch_evaluate_device(hfmm_batch& B, const void* const* q, int q_type, void* const* out, hipStream_t s) {
     const size_t np = B.plans.size();
     bool same = B.gexec && B.g_qtype == q_type;
-    for (size_t i = 0; same && i < np; ++i) same = B.g_q[i] == q[i] && B.g_out[i] == out[i];
+    for (size_t i = 0; same && i < np; ++i)
+        same = B.g_q[i] == q[i] && B.g_out[i] == out[i] && B.g_ver[i] == B.plans[i]->degree_version;
     if (!same) {
         if (B.gexec) HIPCHK(hipGraphExecDestroy(B.gexec));
         if (B.graph) HIPCHK(hipGraphDestroy(B.graph));
@@ -1574,6 +1578,8 @@ void batch_evaluate_device(hfmm_batch& B, const void* const* q, int q_type, void
         HIPCHK(hipGraphInstantiate(&B.gexec, B.graph, nullptr, nullptr, 0));
         B.g_q.assign(q, q + np);
         B.g_out.assign(out, out + np);
+        B.g_ver.resize(np);
+        for (size_t i = 0; i < np; ++i) B.g_ver[i] = B.plans[i]->degree_version;
         B.g_qtype = q_type;
     }
     HIPCHK(hipGraphLaunch(B.gexec, s));

@@ -31,7 +31,7 @@ extern "C" {
 
 /* precision of the evaluate path */
 #define HFMM_PREC_FP64 0       /* FP64 everywhere (the plain FMM) */
-#define HFMM_PREC_MIXED 1      /* FP32 near field on leaf-centred coordinates and FP32 M2L on
+#define HFMM_PREC_MIXED 1      /* FP32 near-field weights (FP64 coordinate differences) and FP32 M2L on
                                   matrix cores (at the levels hfmm_opts.m2l_fp64_levels leaves
                                   in FP32); P2M/M2M/L2L/L2P and accumulation in FP64 */
 
@@ -89,8 +89,8 @@ typedef struct hfmm_plan_info {
     int p_table[HFMM_MAX_LEVELS];   /* degree the level's tables were built for */
     int m2l_fp32_levels;            /* bit l set: the M2L of level l runs in FP32 */
     int n_leaves_phase64;           /* HFMM_PREC_MIXED: leaves whose FP32 near field takes the
-                                       coordinates and the phase k r in FP64 (pairs reaching
-                                       k r > 8, where an FP32 phase errs by ~k r 2^-23 rad) */
+                                       phase k r in FP64 (pairs reaching k r > 8, where an FP32
+                                       phase errs by ~k r 2^-23 rad) */
 } hfmm_plan_info;
 
 void hfmm_opts_default(hfmm_opts* o);

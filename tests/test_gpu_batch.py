@@ -66,3 +66,51 @@ def test_batch_argument_errors():
     with pytest.raises(ValueError):
         B.evaluate([np.ones(100), np.ones(99)])
     B.close()
+
+
+def test_batch_recaptures_after_set_degrees():
+    """set_degrees on a member plan makes the batch capture its graph again: the next
+    batch evaluate equals a batch built at the new degrees, and the other problem is
+    unchanged (device pointers, so that the stale graph would otherwise be replayed)."""
+    import torch
+    from hfmm0_amd.batch import Batch
+    probs = _problems(2, 900)
+    xs, qs, ks = [x for x, _, _ in probs], [q for _, q, _ in probs], [k for _, _, k in probs]
+    print(f"case: batch recapture, n={[len(x) for x in xs]}", flush=True)
+    B = Batch(xs, ks, p=12, N_max=48)
+    qd = [torch.from_numpy(q).cuda() for q in qs]
+    od = [torch.empty(len(q), dtype=torch.complex128, device="cuda") for q in qs]
+    B.evaluate_device(qd, od)
+    torch.cuda.synchronize()
+    before = [o.cpu().numpy() for o in od]
+    st = B.plan(1)
+    low = [8] * st.depth
+    st.set_degrees(low)
+    B.evaluate_device(qd, od)
+    torch.cuda.synchronize()
+    after = [o.cpu().numpy() for o in od]
+    R = Batch(xs, ks, p=12, N_max=48, p_level=[[12] * B.plan(0).depth, low])
+    ref = R.evaluate(qs)
+    np.testing.assert_allclose(after[0], before[0], rtol=0, atol=1e-12 * np.abs(before[0]).max())
+    np.testing.assert_allclose(after[1], ref[1], rtol=0, atol=1e-12 * np.abs(ref[1]).max())
+    assert np.abs(after[1] - before[1]).max() > 1e-9 * np.abs(before[1]).max()
+    R.close()
+    B.close()
+
+
+def test_batch_certify_raises_the_missing_problem():
+    """Batch.certify raises the degrees of the problem that misses the tolerance (within
+    its table degrees) until it meets it, and leaves the others."""
+    from hfmm0_amd.batch import Batch
+    probs = _problems(3, 950)
+    xs, qs, ks = [x for x, _, _ in probs], [q for _, q, _ in probs], [k for _, _, k in probs]
+    print(f"case: batch certify, n={[len(x) for x in xs]} k={[round(k, 1) for k in ks]}", flush=True)
+    B = Batch(xs, ks, p=14, N_max=48)
+    st = B.plan(2)
+    st.set_degrees([5] * st.depth)
+    _, pls, errs = B.certify(qs, tol=1e-5, n_samp=200)
+    print(f"  degrees {pls} errors {[f'{e:.1e}' for e in errs]}", flush=True)
+    assert all(e <= 1e-5 for e in errs)
+    assert pls[0] == [14] * B.plan(0).depth and pls[1] == [14] * B.plan(1).depth
+    assert 5 < pls[2][0] <= 14
+    B.close()
