@@ -569,13 +569,23 @@ __device__ inline void sincos_quadrant(float y, int q, float& s, float& c) {
     c = ((q + 1) & 2) ? -b : b;
 }
 
-// sin and cos of x in FP32: Cody-Waite reduction by pi/2 with a two-term split of pi/2
-// (float(pi/2) and its remainder), then sincos_quadrant.
-__device__ inline void sincos_rr(float x, float& s, float& c) {
-    const float kn = rintf(x * 0.636619772367581343f);
-    float y = fmaf(-kn, 1.57079637050628662109375f, x);
-    y = fmaf(-kn, -4.37113900018624283e-8f, y);
-    sincos_quadrant(y, (int)kn, s, c);
+// sin and cos in FP32 of y in [-pi/2, pi/2]: the Taylor polynomials of degree 13 (sin) and
+// 14 (cos), whose truncation errors on that interval are below 6.6e-10 and 6.5e-11.
+__device__ inline void sincos_halfpi(float y, float& s, float& c) {
+    const float y2 = y * y;
+    float ps = fmaf(y2, 1.6059043836821615e-10f, -2.5052108385441719e-8f);
+    ps = fmaf(y2, ps, 2.7557319223985893e-6f);
+    ps = fmaf(y2, ps, -1.9841269841269841e-4f);
+    ps = fmaf(y2, ps, 8.3333333333333333e-3f);
+    ps = fmaf(y2, ps, -1.6666666666666667e-1f);
+    s = fmaf(y * y2, ps, y);
+    float pc = fmaf(y2, -1.1470745597729725e-11f, 2.0876756987868099e-9f);
+    pc = fmaf(y2, pc, -2.7557319223985891e-7f);
+    pc = fmaf(y2, pc, 2.4801587301587302e-5f);
+    pc = fmaf(y2, pc, -1.3888888888888889e-3f);
+    pc = fmaf(y2, pc, 4.1666666666666667e-2f);
+    pc = fmaf(y2, pc, -0.5f);
+    c = fmaf(y2, pc, 1.0f);
 }
 
 // e^{ikr}/r in FP32 with the phase k r accurate to FP64 rounding, from an FP64 r^2: an FP32
@@ -603,15 +613,22 @@ __device__ inline void phase_fp64_weights(double r2, double k, float& wr, float&
 template <typename T>
 struct P2PMath;
 
+// e^{ikr}/r in FP32.  k r is reduced by pi (Cody-Waite, two-term split of pi: float(pi)
+// and its remainder) to y in [-pi/2, pi/2], so that e^{ikr} = (-1)^n e^{iy}: the sign goes
+// into 1/r as one bit, with no quadrant selection of sin and cos.
 template <>
 struct P2PMath<float> {
     __device__ static inline void kernel(float r2, float k, float& wr, float& wi) {
         const float rinv = __builtin_amdgcn_rsqf(r2);
-        const float r = r2 * rinv;
+        const float x = k * (r2 * rinv);
+        const float kn = rintf(x * 0.318309886183790672f);
+        float y = fmaf(-kn, 3.14159274101257324219f, x);
+        y = fmaf(-kn, -8.74227800037248e-8f, y);
         float s, c;
-        sincos_rr(k * r, s, c);
-        wr = c * rinv;
-        wi = s * rinv;
+        sincos_halfpi(y, s, c);
+        const float rs = __int_as_float(__float_as_int(rinv) ^ (((int)kn & 1) << 31));
+        wr = c * rs;
+        wi = s * rs;
     }
 };
 

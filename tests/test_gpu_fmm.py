@@ -244,6 +244,20 @@ def test_mixed_precision_per_level(hf):
         hf.setup(x, k, precision="mixed", m2l_fp64_levels=[0], **base)
 
 
+def test_fp32_phase_near_field(hf):
+    """The FP32 near field with an FP32 phase (neighbour leaves of width 1/16 at k = 20: k r
+    at most 20 * 2 sqrt(3) / 16 = 4.3, below the FP64-phase bound of 8): with every M2L level
+    kept in FP64 the result sits within FP32 weight rounding of the FP64 plan."""
+    say("case: fp32-phase near field N=30000 k=20 p=12 leaf=32")
+    x, q = D.uniform_problem(30000)
+    ref = hf.setup(x, 20.0, p=12, N_max=32)
+    st = hf.setup(x, 20.0, p=12, N_max=32, precision="mixed", m2l_fp64_levels=list(range(2, ref.depth + 1)))
+    assert st.info.n_leaves_phase64 == 0 and st.m2l_fp32_levels == []
+    d = D.relative_error(hf.evaluate(st, q), hf.evaluate(ref, q))
+    say(f"  vs fp64 plan {d:.2e}")
+    assert d < 2e-7
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_set_degrees_equals_dedicated_plan(hf, precision):
     say(f"case: set_degrees {precision} N=20000 k=12 tables p=14 -> [12, 11, 9]")
