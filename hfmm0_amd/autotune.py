@@ -179,10 +179,11 @@ def refine_degrees(positions, charges, k, p_level, tol=1e-6, leaf=128, precision
                 up[i - 1] += 1
                 moves.append(up)
         pick = None
+        seen = {tuple(h[0]): (h[1], h[2]) for h in history}
         for pl in moves:
-            if any(h[0] == pl for h in history):
-                continue
-            ms, err = measure(pl)
+            # a configuration measured in an earlier round keeps its measurement (it may be
+            # the best move from the current degrees even though it was not from earlier ones)
+            ms, err = seen[tuple(pl)] if tuple(pl) in seen else measure(pl)
             if err <= safety * tol and ms < 0.99 * best_ms and (pick is None or ms < pick[1]):
                 pick = (pl, ms, err)
         if pick is None:
