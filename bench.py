@@ -84,19 +84,23 @@ def m2l_roofline(form, precision):
     flops = f_m2l(p) * form["v_per_level"][lvl]
     achieved = flops / (lv_ms[lvl] * 1e-3)
     peak = PEAK["fp32"] if precision == "mixed" else PEAK["fp64"]
-    traffic = None
+    traffic = traffic_alg = None
     tf = os.path.join(ROOT, "profiles", "m2l_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             rec = json.load(f)
         if rec.get("level") == lvl and rec.get("p") == p and rec.get("precision") == precision:
             traffic = rec.get("bytes_per_launch")
+            traffic_alg = rec.get("algorithmic_bytes_per_launch")
     total = form["m2l_flops"] / (sum(lv_ms.values()) * 1e-3)
     return {"bound": "mfma",
             "kernel": f"m2l_block_kernel, level {lvl} (p={p}, {form['v_per_level'][lvl]} V pairs), "
                       f"{'FP32' if precision == 'mixed' else 'FP64'} MFMA",
             "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
             "frac": achieved / peak, "traffic": traffic,
+            # compulsory bytes of the launch (multipoles read once, locals read and written);
+            # the measured traffic above them is mostly register spill (profiles/m2l_traffic.json)
+            "traffic_algorithmic": traffic_alg,
             "launch_ms": lv_ms[lvl], "flops_per_launch": flops,
             "all_levels_tflops": total / 1e12, "m2l_level_ms": {str(k): v for k, v in lv_ms.items()}}
 
