@@ -61,6 +61,8 @@ struct XlateArgs {
     int ps, pt, pmax;          // effective source / target degrees (<= table degrees)
     int tps, tpt;              // degrees the coaxial tables were built for
     int accumulate;            // 1: dst += result, 0: dst = result
+    int split;                 // workgroups per target (> 1: contiguous parts of the source
+                               // list, added atomically; needs accumulate = 1)
     const double2* srcbuf;
     double2* dstbuf;
     const int64_t* exp_off;

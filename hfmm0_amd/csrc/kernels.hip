@@ -457,14 +457,17 @@ __global__ __launch_bounds__(kXlateThreads) void xlate_kernel(XlateArgs a) {
             o += (tpt + 1 - m) * (tps + 1 - m);
         }
     }
-    const int t_ord = blockIdx.x;
+    const int t_ord = blockIdx.x / a.split, part = blockIdx.x % a.split;
     const int tbox = a.targets[t_ord];
+    const int e_lo = a.ptr[t_ord], e_n = a.ptr[t_ord + 1] - e_lo;
+    const int e_begin = e_lo + (int)((int64_t)e_n * part / a.split);
+    const int e_end = e_lo + (int)((int64_t)e_n * (part + 1) / a.split);
     const int rsz = (a.pmax + 1) * (2 * a.pmax + 1) * (2 * a.pmax + 3) / 3;
     double2 acc[kAccPerThread];
 #pragma unroll
     for (int t = 0; t < kAccPerThread; ++t) acc[t] = make_double2(0.0, 0.0);
     __syncthreads();
-    for (int e = a.ptr[t_ord]; e < a.ptr[t_ord + 1]; ++e) {
+    for (int e = e_begin; e < e_end; ++e) {
         const int s = a.src[e];
         const int d = a.dir[e];
         const DispInfo di = a.dinfo[d];
@@ -529,6 +532,18 @@ __global__ __launch_bounds__(kXlateThreads) void xlate_kernel(XlateArgs a) {
         __syncthreads();
     }
     double2* dst = a.dstbuf + a.exp_off[tbox];
+    if (a.split > 1) {   // parts of one target's sources: added atomically
+        if (e_begin == e_end) return;
+#pragma unroll
+        for (int t = 0; t < kAccPerThread; ++t) {
+            const int c = threadIdx.x + t * blockDim.x;
+            if (c < nct) {
+                atomicAdd(&dst[c].x, acc[t].x);
+                atomicAdd(&dst[c].y, acc[t].y);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int t = 0; t < kAccPerThread; ++t) {
         const int c = threadIdx.x + t * blockDim.x;
@@ -1025,9 +1040,10 @@ void launch_p2p_field(int precision, int n_leaves, int n_fp32, const int* leaf_b
 
 void launch_xlate(const XlateArgs& a, int n_targets, hipStream_t s) {
     if (n_targets <= 0) return;
+    if (a.split < 1 || (a.split > 1 && !a.accumulate)) fail(-9, "xlate: split needs accumulate");
     const int nmx = std::max((a.ps + 1) * (a.ps + 1), (a.pt + 1) * (a.pt + 1));
     const size_t shm = 2 * (size_t)nmx * sizeof(double2) + (kMaxP + 1) * sizeof(int);
-    hipLaunchKernelGGL(xlate_kernel, dim3(n_targets), dim3(kXlateThreads), shm, s, a);
+    hipLaunchKernelGGL(xlate_kernel, dim3((unsigned)n_targets * (unsigned)a.split), dim3(kXlateThreads), shm, s, a);
     check_launch("xlate");
 }
 

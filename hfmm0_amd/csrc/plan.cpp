@@ -930,6 +930,7 @@ XlateArgs xargs(const hfmm_plan& P, const XlateLaunch& X, const double2* srcbuf,
     a.tpt = X.pt;
     a.pmax = P.pmax;
     a.accumulate = accumulate;
+    a.split = 1;
     a.srcbuf = srcbuf;
     a.dstbuf = dstbuf;
     a.exp_off = P.d_expoff;
@@ -1006,7 +1007,12 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
         const M2LBlockLevel& B = P.m2lb[L];
         if (!B.use) {
             if (lev) HIPCHK(hipEventRecord(lev[2 * L], s));
-            launch_xlate(xargs(P, P.m2l[L], P.dM, P.dL, 1), P.m2l[L].n_targets, s);
+            // coarse levels with few targets (degrees above the matrix-core kernel's): the
+            // source lists split over up to 16 workgroups per target, so that about 2048
+            // workgroups fill the GPU (level 2 of a uniform tree has 64 targets)
+            XlateArgs a = xargs(P, P.m2l[L], P.dM, P.dL, 1);
+            a.split = std::max(1, std::min(16, 2048 / std::max(1, P.m2l[L].n_targets)));
+            launch_xlate(a, P.m2l[L].n_targets, s);
             if (lev) HIPCHK(hipEventRecord(lev[2 * L + 1], s));
             continue;
         }
