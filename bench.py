@@ -175,8 +175,8 @@ def main():
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding); "
-              "run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding is a design "
+              "only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

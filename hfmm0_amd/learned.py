@@ -4,7 +4,7 @@ label generation by the FMM itself, and the certified evaluate with the plain fa
 Role in the reference: order selection belongs to ``jaxfmm/tune.py``'s
 ``AutoEvaluator`` (certify on sampled targets, climb a (p, theta) ladder,
 ``tune.py:375-516``).  Here a small network predicts the degree of every tree level
-from cheap features; the plain fixed-degree FMM is both its fallback and its judge.
+from cheap features; the plain fixed-degree FMM is both its fallback and its check.
 
 Features of level l (normalised to O(1)):
   0 level / 10

@@ -49,7 +49,7 @@ def generate(n_problems, seed0=1000, n_range=(10_000, 200_000), k_range=(1.0, 60
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--problems", type=int, default=200)
-    ap.add_argument("--out", default="gpurun_out")
+    ap.add_argument("--out", default="degree_training")
     ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--budget", type=float, default=6e-7, help="total error budget of the labels")

@@ -101,8 +101,8 @@ def test_tree_and_lists(name, x, nmax, md):
             assert cover == 1, (name, a, b, cover)
 
 
-def test_uniform_counts_match_survey_c1():
-    """C1 (N=20k, leaf 64): depth 3, 512 leaves, 56,448 V pairs, 10,648 U+self pairs."""
+def test_uniform_20k_leaf64_counts():
+    """Uniform cube N=20k, leaf 64: depth 3, 512 leaves, 56,448 V pairs, 10,648 U+self pairs."""
     from oracle import direct
     x, _ = direct.uniform_problem(20000)
     tr = T.build_tree(x, 64)
