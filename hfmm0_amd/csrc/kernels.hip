@@ -700,8 +700,10 @@ __global__ __launch_bounds__(64) void p2p_kernel(int leaf0, const int* leaf_box,
     const int S = leaf_src_total[leaf];
     const int lane = threadIdx.x;
     const T kk = (T)k;
-    for (int t0 = 0; t0 < tcnt; t0 += 64) {
-        const int tc = min(64, tcnt - t0);
+    // target chunks of 64; a remainder of 33..63 targets goes as 32 + the rest, so that the
+    // lanes are not left idle with T' = 64 (each lane then scans half or less of the sources)
+    for (int t0 = 0, tc = 0; t0 < tcnt; t0 += tc) {
+        tc = tcnt - t0 >= 64 ? 64 : min(32, tcnt - t0);
         int Tp = 1;
         while (Tp < tc) Tp <<= 1;
         const int G = 64 / Tp;
@@ -795,8 +797,10 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(int leaf0, const int* lea
     const int S = leaf_src_total[leaf];
     const int lane = threadIdx.x;
     const T kk = (T)k;
-    for (int t0 = 0; t0 < tcnt; t0 += 64) {
-        const int tc = min(64, tcnt - t0);
+    // target chunks of 64; a remainder of 33..63 targets goes as 32 + the rest, so that the
+    // lanes are not left idle with T' = 64 (each lane then scans half or less of the sources)
+    for (int t0 = 0, tc = 0; t0 < tcnt; t0 += tc) {
+        tc = tcnt - t0 >= 64 ? 64 : min(32, tcnt - t0);
         int Tp = 1;
         while (Tp < tc) Tp <<= 1;
         const int G = 64 / Tp;
