@@ -563,27 +563,6 @@ __global__ __launch_bounds__(kXlateThreads) void xlate_kernel(XlateArgs a) {
 // ---------------------------------------------------------------------------------
 // P2P near field.
 // ---------------------------------------------------------------------------------
-// sin and cos in FP32 of y + q pi/2, y in [-pi/4, pi/4]: the Taylor polynomials of degree
-// 9 (sin) and 10 (cos) on [-pi/4, pi/4] (truncation error below 2e-9, under half an FP32
-// ulp of the results), then the quadrant swap.
-__device__ inline void sincos_quadrant(float y, int q, float& s, float& c) {
-    const float y2 = y * y;
-    float ps = fmaf(y2, 2.7557319223985893e-6f, -1.9841269841269841e-4f);
-    ps = fmaf(y2, ps, 8.3333333333333333e-3f);
-    ps = fmaf(y2, ps, -1.6666666666666667e-1f);
-    const float sn = fmaf(y * y2, ps, y);
-    float pc = fmaf(y2, -2.7557319223985891e-7f, 2.4801587301587302e-5f);
-    pc = fmaf(y2, pc, -1.3888888888888889e-3f);
-    pc = fmaf(y2, pc, 4.1666666666666667e-2f);
-    pc = fmaf(y2, pc, -0.5f);
-    const float cs = fmaf(y2, pc, 1.0f);
-    q &= 3;
-    const float a = (q & 1) ? cs : sn;
-    const float b = (q & 1) ? sn : cs;
-    s = (q & 2) ? -a : a;
-    c = ((q + 1) & 2) ? -b : b;
-}
-
 // sin and cos in FP32 of y in [-pi/2, pi/2]: the Taylor polynomials of degree 13 (sin) and
 // 14 (cos), whose truncation errors on that interval are below 6.6e-10 and 6.5e-11.
 __device__ inline void sincos_halfpi(float y, float& s, float& c) {
@@ -606,22 +585,24 @@ __device__ inline void sincos_halfpi(float y, float& s, float& c) {
 // e^{ikr}/r in FP32 with the phase k r accurate to FP64 rounding, from an FP64 r^2: an FP32
 // phase carries an error of about k r 2^-23 rad, which matters once k r reaches tens (long
 // W / X list pairs at high frequency).  r is the FP32 estimate r2 rsq(r2) refined by one
-// Newton step in FP64 (relative error ~1e-14), k r is reduced by pi/2 (two-term FP64 split)
-// and the reduced argument goes to the FP32 polynomials.  Also returns 1/r and k r (FP32)
-// for the field weights.
+// Newton step in FP64 (relative error ~1e-14), k r is reduced by pi (two-term FP64 split:
+// double(pi) and its remainder) to y in [-pi/2, pi/2] and e^{ikr} = (-1)^n e^{iy}, the sign
+// going into 1/r as one bit.  Also returns 1/r (positive) and k r (FP32) for the field
+// weights.
 __device__ inline void phase_fp64_weights(double r2, double k, float& wr, float& wi, float& rinv, float& kr) {
     const float r2f = (float)r2;
     rinv = __builtin_amdgcn_rsqf(r2f);
     const double r0 = (double)(r2f * rinv);
     const double r = fma(0.5 * (double)rinv, fma(-r0, r0, r2), r0);
     const double ph = k * r;
-    const double kn = rint(ph * 0.63661977236758134308);
-    double y = fma(-kn, 1.5707963267948966192, ph);
-    y = fma(-kn, 6.123233995736766036e-17, y);
+    const double kn = rint(ph * 0.31830988618379067154);
+    double y = fma(-kn, 3.1415926535897931160, ph);
+    y = fma(-kn, 1.2246467991473532072e-16, y);
     float s, c;
-    sincos_quadrant((float)y, (int)kn, s, c);
-    wr = c * rinv;
-    wi = s * rinv;
+    sincos_halfpi((float)y, s, c);
+    const float rs = __int_as_float(__float_as_int(rinv) ^ (((int)kn & 1) << 31));
+    wr = c * rs;
+    wi = s * rs;
     kr = (float)ph;
 }
 
