@@ -201,17 +201,38 @@ def evaluate_field(state, charges):
     return phi, field
 
 
-def evaluate_device(state, q_dev, out_dev, stream=None):
-    """Evaluate with device-resident torch tensors: ``q_dev`` (N,) float64 or complex128,
-    ``out_dev`` (N,) complex128, both on the GPU."""
+def torch_charge_code(dtype):
+    """C type code (HFMM_F64 / C128 / F32 / C64) of a torch charge dtype; any other dtype
+    raises ValueError."""
     import torch
-    if q_dev.shape != (state.n,) or out_dev.shape != (state.n,) or out_dev.dtype != torch.complex128:
-        raise ValueError("charges must be a real array of shape (N,)")
-    if not (q_dev.is_cuda and out_dev.is_cuda and q_dev.is_contiguous() and out_dev.is_contiguous()):
-        raise ValueError("device evaluate needs contiguous GPU tensors")
-    qt = {torch.float64: L.F64, torch.complex128: L.C128, torch.float32: L.F32, torch.complex64: L.C64}.get(q_dev.dtype)
+    qt = {torch.float64: L.F64, torch.complex128: L.C128, torch.float32: L.F32,
+          torch.complex64: L.C64}.get(dtype)
     if qt is None:
         raise ValueError("charges must be float32/64 or complex64/128")
+    return qt
+
+
+def device_charge_type(q_dev, out_dev, n):
+    """C type code of a device-resident charge tensor, after checking both tensors: ``q_dev``
+    (n,) float64 / complex128 / float32 / complex64 and ``out_dev`` (n,) complex128, both
+    contiguous GPU tensors.  Every entry point that hands torch pointers to the library
+    goes through this map, so the library never reads a buffer as a wider type than it is."""
+    import torch
+    if q_dev.shape != (n,) or out_dev.shape != (n,):
+        raise ValueError("charges must be a real array of shape (N,)")
+    if out_dev.dtype != torch.complex128:
+        raise ValueError("the device output must be a complex128 tensor")
+    qt = torch_charge_code(q_dev.dtype)
+    if not (q_dev.is_cuda and out_dev.is_cuda and q_dev.is_contiguous() and out_dev.is_contiguous()):
+        raise ValueError("device evaluate needs contiguous GPU tensors")
+    return qt
+
+
+def evaluate_device(state, q_dev, out_dev, stream=None):
+    """Evaluate with device-resident torch tensors: ``q_dev`` (N,) float32/64 or
+    complex64/128, ``out_dev`` (N,) complex128, both on the GPU."""
+    import torch
+    qt = device_charge_type(q_dev, out_dev, state.n)
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
     L.check(L.lib().hfmm_evaluate(state.handle, ctypes.c_void_p(q_dev.data_ptr()), qt,
@@ -222,7 +243,7 @@ def evaluate_device(state, q_dev, out_dev, stream=None):
 def benchmark(state, q_dev, out_dev, warmup=3, reps=20, phases=False):
     """Median milliseconds of one device-resident evaluate (hipEvent timing of graph
     replays) and optionally per-phase milliseconds of one instrumented run."""
-    qt = L.C128 if q_dev.is_complex() else L.F64
+    qt = device_charge_type(q_dev, out_dev, state.n)
     med = ctypes.c_double()
     ph = np.zeros(7)
     L.check(L.lib().hfmm_benchmark(state.handle, ctypes.c_void_p(q_dev.data_ptr()), qt,
@@ -237,7 +258,7 @@ def benchmark(state, q_dev, out_dev, warmup=3, reps=20, phases=False):
 def benchmark_m2l_levels(state, q_dev, out_dev, reps=5):
     """Average milliseconds of each level's M2L launch ({level: ms}), hipEvents around
     every launch on the plan's stream over `reps` instrumented evaluates."""
-    qt = L.C128 if q_dev.is_complex() else L.F64
+    qt = device_charge_type(q_dev, out_dev, state.n)
     ms = np.zeros(L.MAX_LEVELS)
     L.check(L.lib().hfmm_benchmark_m2l_levels(state.handle, ctypes.c_void_p(q_dev.data_ptr()), qt,
                                               ctypes.c_void_p(out_dev.data_ptr()), int(reps), L.ptr(ms)))

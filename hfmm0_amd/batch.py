@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 
 from . import _lib as L
-from .api import HFMMState, _check_problem, make_opts
+from .api import HFMMState, _check_problem, device_charge_type, make_opts
 
 
 class Batch:
@@ -73,18 +73,14 @@ class Batch:
         return outs
 
     def _dev_ptrs(self, q_devs, out_devs):
-        import torch
         if len(q_devs) != len(self) or len(out_devs) != len(self):
             raise ValueError("need one charge vector and one output per problem")
-        cplx = q_devs[0].is_complex()
-        for q, o, n in zip(q_devs, out_devs, self.n):
-            if q.shape != (n,) or o.shape != (n,) or o.dtype != torch.complex128 or q.is_complex() != cplx:
-                raise ValueError("charges must be a real array of shape (N,)")
-            if not (q.is_cuda and o.is_cuda and q.is_contiguous() and o.is_contiguous()):
-                raise ValueError("device evaluate needs contiguous GPU tensors")
+        types = {device_charge_type(q, o, n) for q, o, n in zip(q_devs, out_devs, self.n)}
+        if len(types) != 1:
+            raise ValueError("all charge vectors of a batch must have the same dtype")
         qp = (ctypes.c_void_p * len(self))(*[q.data_ptr() for q in q_devs])
         op = (ctypes.c_void_p * len(self))(*[o.data_ptr() for o in out_devs])
-        return qp, op, (L.C128 if cplx else L.F64)
+        return qp, op, types.pop()
 
     def evaluate_device(self, q_devs, out_devs):
         """All problems with device-resident torch tensors (one graph replay)."""

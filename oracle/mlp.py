@@ -10,8 +10,10 @@ Parameter layout (float32, 8833 values): W1[64][6], b1[64], W2[64][64], b2[64],
 W3[64][64], b3[64], W4[64], b4.
 
 Pinned by ``tests/test_oracle_mlp.py``: gradients against central finite differences
-of the loss (FP64 evaluation of the same BF16-rounded network) and the closed-form GELU
-derivative against finite differences.
+of the loss (FP64 evaluation of the same BF16-rounded network), the closed-form GELU
+derivative against finite differences, GELU values against a 50-digit mpmath evaluation of
+the tanh form (Hendrycks & Gimpel 2016), BF16 rounding cases, and the Adam step against its
+closed form for a constant gradient (every bias-corrected step is lr g / (|g| + eps)).
 """
 import numpy as np
 

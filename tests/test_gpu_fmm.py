@@ -316,3 +316,22 @@ def test_charge_and_output_types(hf):
     st0 = hf.setup(x, 0.0, p=10, N_max=32)
     assert np.all(hf.evaluate(st0, r32).imag == 0)
     assert np.all(hf.evaluate(st0, r32, out_dtype=np.complex64).imag == 0)
+
+
+def test_device_benchmark_narrow_charge_types(hf):
+    """benchmark / benchmark_m2l_levels / evaluate_device read float32 and complex64 device
+    tensors as their own width (they were once sent as float64 / complex128)."""
+    import torch
+    say("case: device benchmark with float32 / complex64 charges N=12000 k=8 p=10")
+    x, q = D.uniform_problem(12000)
+    st = hf.setup(x, 8.0, p=10, N_max=32)
+    od = torch.empty(12000, dtype=torch.complex128, device="cuda")
+    for qn in (q.astype(np.complex64), q.real.astype(np.float32)):
+        ref = hf.evaluate(st, qn.astype(np.complex128 if np.iscomplexobj(qn) else np.float64))
+        qd = torch.from_numpy(qn).cuda()
+        ms = hf.benchmark(st, qd, od, warmup=1, reps=3)
+        lv = hf.benchmark_m2l_levels(st, qd, od, reps=2)
+        hf.evaluate_device(st, qd, od)
+        torch.cuda.synchronize()
+        assert ms > 0 and lv
+        np.testing.assert_allclose(od.cpu().numpy(), ref, rtol=0, atol=1e-13 * np.abs(ref).max())

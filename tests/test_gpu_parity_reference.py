@@ -1,8 +1,17 @@
 """GPU parity with the reference's own evaluate on identical inputs.
 
 phi_ref comes from the committed fixtures (oracle/kifmm.py, a restatement of the
-reference's leveled KIFMM).  Parity rule: ||phi_build - phi_ref|| / ||phi_ref|| must not
-exceed err_build + err_ref, both measured against the FP64 direct sum on the same inputs.
+reference's leveled KIFMM, written by tests/golden/make_kifmm_fixtures.py); phi_direct is
+the FP64 direct sum on all points.  The assertions follow the reference harness
+(test.py:63-72) applied to this build:
+
+* the build's error against the direct sum, on the full result and on its real and
+  imaginary parts, is at most max(err_ref, 1e-6) -- the build is at least as accurate as
+  the reference on the same inputs (test.py:64-66 bounds the reference at 5e-3);
+* the build-vs-reference difference is the reference's own error: with all norms taken
+  relative to ||phi_direct||, | ||phi_b - phi_ref|| - ||phi_ref - phi_direct|| | <= e_b,
+  which with e_b small pins phi_b to phi_direct, not merely to a ball around phi_ref;
+* linearity in the charges (test.py:67) and an exactly real result at k = 0 (test.py:68-69).
 """
 import glob
 import os
@@ -17,19 +26,34 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
+def _errors(actual, reference):
+    return np.array([D.relative_error(a, b) for a, b in
+                     ((actual, reference), (actual.real, reference.real), (actual.imag, reference.imag))])
+
+
 @pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(HERE, "golden", "kifmm_*.npz"))))
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_matches_reference_evaluate(path, precision):
     import hfmm0_amd as hf
     f = np.load(path)
     x, q, k, leaf = f["positions"], f["charges"], float(f["k"]), int(f["leaf"])
+    phi_ref, phi_dir = f["phi_ref"], f["phi_direct"]
     print(f"case: {os.path.basename(path)} precision={precision}", flush=True)
     sys.stdout.flush()
-    phi = hf.evaluate(hf.setup(x, k, p=12, N_max=leaf, precision=precision), q)
-    e_b = D.relative_error(phi, f["phi_direct"])
-    e_r = D.relative_error(f["phi_ref"], f["phi_direct"])
-    d = D.relative_error(phi, f["phi_ref"])
-    print(f"  build err {e_b:.2e} ref err {e_r:.2e} build-vs-ref {d:.2e}", flush=True)
-    assert d <= e_b + e_r + 1e-12
+    st = hf.setup(x, k, p=12, N_max=leaf, precision=precision)
+    phi = hf.evaluate(st, q)
+    assert phi.dtype == np.complex128 and np.isfinite(phi).all()
+    e_b = _errors(phi, phi_dir)
+    e_r = _errors(phi_ref, phi_dir)
+    nd = np.linalg.norm(phi_dir)
+    d = np.linalg.norm(phi - phi_ref) / nd
+    print(f"  build err {e_b} ref err {e_r} build-vs-ref {d:.3e}", flush=True)
+    # the reference's own pass bound at its order (test.py:66) holds for the fixture
+    assert max(e_r) < (5e-2 if int(f["p_ref"]) <= 3 else 5e-3)
+    # the build is at least as accurate, part by part
+    assert np.all(e_b <= np.maximum(e_r, 1e-6)), (e_b, e_r)
+    # the difference from the reference is the reference's own error
+    assert abs(d - e_r[0]) <= e_b[0] + 1e-12
+    np.testing.assert_allclose(hf.evaluate(st, -0.5 * q), -0.5 * phi, rtol=1e-10, atol=1e-10)
     if k == 0:
         assert np.count_nonzero(phi.imag) == 0

@@ -61,3 +61,47 @@ def test_adam_reduces_loss():
         loss, g = mlp.loss_and_grad(flat, X, y)
         flat = mlp.adam_step(flat, m, v, g, t, 3e-3)
     assert loss < 0.2 * l0
+
+
+def _gelu_tanh_mp(z):
+    """tanh-form GELU of Hendrycks & Gimpel (2016), 0.5 z (1 + tanh(sqrt(2/pi)(z + 0.044715 z^3))),
+    at 50 digits."""
+    import mpmath as mp
+    mp.mp.dps = 50
+    z = mp.mpf(float(z))
+    return float(mp.mpf("0.5") * z * (1 + mp.tanh(mp.sqrt(2 / mp.pi) * (z + mp.mpf("0.044715") * z ** 3))))
+
+
+def test_gelu_values_pinned_by_mpmath():
+    """GELU at fixed points against a 50-digit evaluation of the tanh form, in FP64 and in the
+    FP32 the kernels use; the exact (erf) GELU differs by up to ~2e-4 near |z| = 2, so a
+    wrong constant or the erf form fails the FP64 bound."""
+    import mpmath as mp
+    z = np.array([-6.0, -3.0, -2.0, -1.5, -0.7, -0.1, 0.0, 0.05, 0.4, 1.0, 1.7, 2.0, 3.5, 6.0])
+    ref = np.array([_gelu_tanh_mp(v) for v in z])
+    got64 = mlp.gelu(z, np.float64)
+    np.testing.assert_allclose(got64, ref, rtol=1e-14, atol=1e-16)
+    got32 = mlp.gelu(z.astype(np.float32), np.float32).astype(np.float64)
+    np.testing.assert_allclose(got32, ref, rtol=3e-6, atol=1e-7)
+    erf_form = np.array([float(0.5 * v * (1 + mp.erf(v / mp.sqrt(2)))) for v in z])
+    assert np.max(np.abs(erf_form - ref)) > 1e-4
+
+
+def test_adam_first_steps_closed_form():
+    """With a constant gradient every bias-corrected Adam step equals lr g / (|g| + eps)
+    (m_hat = g and v_hat = g^2 at every t), so after t steps the weights moved by
+    t lr g / (|g| + eps); a missing bias correction or a swapped beta moves them by a
+    different amount already at t = 1."""
+    rng = np.random.default_rng(5)
+    flat0 = rng.normal(size=257).astype(np.float32)
+    g = rng.normal(size=257).astype(np.float32)
+    g[:3] = [1e-12, -2.5e-9, 0.0]
+    lr, eps = np.float32(1e-3), 1e-8
+    m = np.zeros(257)
+    v = np.zeros(257)
+    flat = flat0.copy()
+    step = lr * g.astype(np.float64) / (np.abs(g.astype(np.float64)) + eps)
+    for t in range(1, 6):
+        flat = mlp.adam_step(flat, m, v, g.astype(np.float64), t, lr)
+        np.testing.assert_allclose(flat.astype(np.float64), flat0 - t * step, rtol=0, atol=t * 2e-7)
+    assert flat[2] == flat0[2]      # zero gradient: no move
