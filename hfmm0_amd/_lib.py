@@ -30,7 +30,7 @@ class Opts(ctypes.Structure):
     _fields_ = [("p", ctypes.c_int), ("p_level", ctypes.c_int * MAX_LEVELS), ("n_max", ctypes.c_int),
                 ("max_depth", ctypes.c_int), ("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("m2l_kernel", ctypes.c_int), ("tree_device", ctypes.c_int),
-                ("m2l_fp64_levels", ctypes.c_int)]
+                ("m2l_fp64_levels", ctypes.c_int), ("p2p_tile", ctypes.c_int)]
 
 
 class PlanInfo(ctypes.Structure):

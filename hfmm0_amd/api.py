@@ -93,7 +93,7 @@ class HFMMState:
 
 
 def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, use_graph=True,
-          m2l_kernel="auto", tree_device=False, m2l_fp64_levels=None):
+          m2l_kernel="auto", tree_device=False, m2l_fp64_levels=None, p2p_tile=0):
     """Build the octree, lists and operator tables; upload everything to the GPU.
 
     Validation order and messages follow ``HFMM.py:57-64`` and ``HFMM.py:14-16``.
@@ -110,7 +110,8 @@ def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_le
     """
     x, k = _check_problem(positions, k)
     o = make_opts(p, N_max, max_depth, precision=precision, p_level=p_level, use_graph=use_graph,
-                  m2l_kernel=m2l_kernel, tree_device=tree_device, m2l_fp64_levels=m2l_fp64_levels, _k=k)
+                  m2l_kernel=m2l_kernel, tree_device=tree_device, m2l_fp64_levels=m2l_fp64_levels,
+                  p2p_tile=p2p_tile, _k=k)
     h = ctypes.c_void_p()
     L.check(L.lib().hfmm_plan_create(L.ptr(x), len(x), k, ctypes.byref(o), ctypes.byref(h)))
     return HFMMState(h, len(x), k, o)
@@ -124,7 +125,7 @@ def _check_problem(positions, k):
 
 
 def make_opts(p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, use_graph=True,
-              m2l_kernel="auto", tree_device=False, m2l_fp64_levels=None, _k=0.0):
+              m2l_kernel="auto", tree_device=False, m2l_fp64_levels=None, p2p_tile=0, _k=0.0):
     """Validated plan options (the C ``hfmm_opts``), in the reference's validation order
     after the positions check (``HFMM.py:57-64``, ``HFMM.py:14-16``)."""
     if p < 2 or int(p) != p or N_max < 1 or int(N_max) != N_max:
@@ -145,6 +146,9 @@ def make_opts(p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, 
         raise ValueError("m2l_kernel must be 'auto', 'generic', 8, 16, 32 or 64")
     o.m2l_kernel = {"auto": 0, "generic": 1}.get(m2l_kernel, m2l_kernel)
     o.tree_device = 1 if tree_device else 0
+    if p2p_tile not in (0, 1, 2, 4):
+        raise ValueError("p2p_tile must be 0, 1, 2 or 4")
+    o.p2p_tile = int(p2p_tile)
     if m2l_fp64_levels is not None:
         lv = [int(v) for v in m2l_fp64_levels]
         if any(v < 1 or v >= L.MAX_LEVELS for v in lv):

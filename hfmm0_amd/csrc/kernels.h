@@ -77,18 +77,40 @@ void launch_l2p_field(const int* leaves, int n_items, DevBoxes bx, DevLevels lv,
 // Near field over the leaves in leaf_box order, coordinate differences and r^2 in FP64.
 // FP32 (precision != 0): FP32 weights; the first n_fp32 leaves with an FP32 phase k r, the
 // rest with the phase in FP64 (their pairs reach k r > kP2PFp32MaxKr).  FP64: every leaf
-// in FP64.
+// in FP64.  Pairs with r^2 <= floor2[leaf] (the coincidence floor of the leaf's pairs,
+// in unscaled coordinates) contribute nothing.
 constexpr double kP2PFp32MaxKr = 8.0;
+// Largest k r over a plan's near pairs for which the mixed near field takes e^{ikr} as
+// 1 + ikr (relative error (kr)^2 / 2 <= 5e-9, below FP32 rounding).
+constexpr double kP2PLowKr = 1e-4;
 void launch_p2p_field(int precision, int n_leaves, int n_fp32, const int* leaf_box, DevBoxes bx,
                       const int* rng_ptr, const int* rng_start, const int* rng_pref, const int* leaf_src_total,
-                      const double* X, const double* Y, const double* Z, const double2* q, double k,
-                      const double2* far, const double2* far_e, const int* perm, int out_mode, double2* out,
-                      double2* field, hipStream_t s);
+                      const double* floor2, const double* X, const double* Y, const double* Z, const double2* q,
+                      double k, const double2* far, const double2* far_e, const int* perm, int out_mode,
+                      double2* out, double2* field, hipStream_t s);
 
-void launch_p2p(int precision, int n_leaves, int n_fp32, const int* leaf_box, DevBoxes bx, const int* rng_ptr,
-                const int* rng_start, const int* rng_pref, const int* leaf_src_total, const double* X,
-                const double* Y, const double* Z, const double2* q, double k, const double2* far, const int* perm,
-                int out_mode /* bit 0: conjugate, bit 1: zero imaginary part */, double2* out, hipStream_t s);
+struct P2PArgs {
+    int leaf0;                    // first leaf of the launch (set by the launcher)
+    const int* leaf_box;          // leaf -> box
+    DevBoxes bx;
+    const int* rng_ptr;           // per leaf: its source point ranges [rng_ptr[l], rng_ptr[l+1])
+    const int* rng_start;         // first point of each range
+    const int* rng_pref;          // sources of the leaf before each range
+    const int* leaf_src_total;    // sources of each leaf
+    const double* floor2;         // per leaf: coincidence floor on r^2 (unscaled coordinates)
+    const double *X, *Y, *Z;      // sorted points (FP64)
+    const double2* q;             // sorted charges (complex FP64)
+    double k;                     // |k|
+    double scale;                 // coordinate scale of the mixed kernel (set by the launcher)
+    double wscale;                // factor of the mixed kernel's sums (set by the launcher)
+    const double2* far;           // far field per sorted point
+    const int* perm;              // sorted -> input order
+    int out_mode;                 // bit 0: conjugate, bit 1: zero imaginary part
+    double2* out;                 // input order
+};
+// lowk: every near pair of the plan has k r <= kP2PLowKr (mixed form only); tile: targets
+// per lane of the mixed form (1, 2 or 4)
+void launch_p2p(const P2PArgs& a, int precision, int n_leaves, int n_fp32, bool lowk, int tile, hipStream_t s);
 
 void launch_direct(const double* X, const double* Y, const double* Z, const double2* q, int64_t n, double k,
                    const double* tx, const double* ty, const double* tz, int64_t nt, double2* out, hipStream_t s);

@@ -13,6 +13,7 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "p2p_math.h"
 
 namespace hfmm {
 
@@ -563,69 +564,15 @@ __global__ __launch_bounds__(kXlateThreads) void xlate_kernel(XlateArgs a) {
 // ---------------------------------------------------------------------------------
 // P2P near field.
 // ---------------------------------------------------------------------------------
-// sin and cos in FP32 of y in [-pi/2, pi/2]: the Taylor polynomials of degree 13 (sin) and
-// 14 (cos), whose truncation errors on that interval are below 6.6e-10 and 6.5e-11.
-__device__ inline void sincos_halfpi(float y, float& s, float& c) {
-    const float y2 = y * y;
-    float ps = fmaf(y2, 1.6059043836821615e-10f, -2.5052108385441719e-8f);
-    ps = fmaf(y2, ps, 2.7557319223985893e-6f);
-    ps = fmaf(y2, ps, -1.9841269841269841e-4f);
-    ps = fmaf(y2, ps, 8.3333333333333333e-3f);
-    ps = fmaf(y2, ps, -1.6666666666666667e-1f);
-    s = fmaf(y * y2, ps, y);
-    float pc = fmaf(y2, -1.1470745597729725e-11f, 2.0876756987868099e-9f);
-    pc = fmaf(y2, pc, -2.7557319223985891e-7f);
-    pc = fmaf(y2, pc, 2.4801587301587302e-5f);
-    pc = fmaf(y2, pc, -1.3888888888888889e-3f);
-    pc = fmaf(y2, pc, 4.1666666666666667e-2f);
-    pc = fmaf(y2, pc, -0.5f);
-    c = fmaf(y2, pc, 1.0f);
-}
-
-// e^{ikr}/r in FP32 with the phase k r accurate to FP64 rounding, from an FP64 r^2: an FP32
-// phase carries an error of about k r 2^-23 rad, which matters once k r reaches tens (long
-// W / X list pairs at high frequency).  r is the FP32 estimate r2 rsq(r2) refined by one
-// Newton step in FP64 (relative error ~1e-14), k r is reduced by pi (two-term FP64 split:
-// double(pi) and its remainder) to y in [-pi/2, pi/2] and e^{ikr} = (-1)^n e^{iy}, the sign
-// going into 1/r as one bit.  Also returns 1/r (positive) and k r (FP32) for the field
-// weights.
-__device__ inline void phase_fp64_weights(double r2, double k, float& wr, float& wi, float& rinv, float& kr) {
-    const float r2f = (float)r2;
-    rinv = __builtin_amdgcn_rsqf(r2f);
-    const double r0 = (double)(r2f * rinv);
-    const double r = fma(0.5 * (double)rinv, fma(-r0, r0, r2), r0);
-    const double ph = k * r;
-    const double kn = rint(ph * 0.31830988618379067154);
-    double y = fma(-kn, 3.1415926535897931160, ph);
-    y = fma(-kn, 1.2246467991473532072e-16, y);
-    float s, c;
-    sincos_halfpi((float)y, s, c);
-    const float rs = __int_as_float(__float_as_int(rinv) ^ (((int)kn & 1) << 31));
-    wr = c * rs;
-    wi = s * rs;
-    kr = (float)ph;
-}
+using nearfield::radian_weight;
+using nearfield::radian_weight_phase64;
 
 template <typename T>
 struct P2PMath;
 
-// e^{ikr}/r in FP32.  k r is reduced by pi (Cody-Waite, two-term split of pi: float(pi)
-// and its remainder) to y in [-pi/2, pi/2], so that e^{ikr} = (-1)^n e^{iy}: the sign goes
-// into 1/r as one bit, with no quadrant selection of sin and cos.
 template <>
 struct P2PMath<float> {
-    __device__ static inline void kernel(float r2, float k, float& wr, float& wi) {
-        const float rinv = __builtin_amdgcn_rsqf(r2);
-        const float x = k * (r2 * rinv);
-        const float kn = rintf(x * 0.318309886183790672f);
-        float y = fmaf(-kn, 3.14159274101257324219f, x);
-        y = fmaf(-kn, -8.74227800037248e-8f, y);
-        float s, c;
-        sincos_halfpi(y, s, c);
-        const float rs = __int_as_float(__float_as_int(rinv) ^ (((int)kn & 1) << 31));
-        wr = c * rs;
-        wi = s * rs;
-    }
+    __device__ static inline void kernel(float r2, float k, float& wr, float& wi) { radian_weight(r2, k, wr, wi); }
 };
 
 template <>
@@ -651,36 +598,157 @@ __device__ inline void field_weights(T r2, T k, T& wr, T& wi, T& fr, T& fi) {
     fi = (wi - kr * wr) * rinv2;
 }
 
-// One wave per target leaf.  Targets of a chunk of up to 64 points are spread over
-// lane groups: with T' = next power of two >= chunk size, lane l holds target l % T'
-// and scans the sources j == l / T' (mod 64/T') of each 64-point source batch staged
-// in LDS (coordinates relative to the target leaf centre, in FP64).  Lane-group partial
-// sums are combined with wavefront shuffles.  Sources of a leaf are a list of point
-// ranges (adjacent leaves, and the W/X interactions evaluated directly).  T = float: r^2
-// is rounded to FP32 for the weights; PH: the phase k r in FP64 as well
-// (phase_fp64_weights), for the leaves whose pairs reach large k r.  The workgroup
-// handles leaf leaf0 + blockIdx.x.
-template <typename T, bool PH>
-__global__ __launch_bounds__(64) void p2p_kernel(int leaf0, const int* leaf_box, DevBoxes bx, const int* rng_ptr,
-                                                 const int* rng_start, const int* rng_pref,
-                                                 const int* leaf_src_total, const double* X, const double* Y,
-                                                 const double* Z, const double2* q, double k, const double2* far,
-                                                 const int* perm, int out_mode, double2* out) {
-    // coordinates relative to the leaf centre and r^2 in FP64 at every precision: FP32
-    // coordinates would carry an absolute error of 2^-24 times the leaf size, a large
-    // relative error of r for close pairs (points on surfaces); FP64 adds and FMAs issue
-    // at the rate of the (unpacked) FP32 ones
-    using C = double;
-    __shared__ C sx[64], sy[64], sz[64];
-    __shared__ T sqr[64], sqi[64];
-    const int leaf = leaf0 + blockIdx.x;
-    const int b = leaf_box[leaf];
-    const int tstart = bx.pstart[b], tcnt = bx.count[b];
-    const double cx = bx.centre[3 * b], cy = bx.centre[3 * b + 1], cz = bx.centre[3 * b + 2];
-    const int r0 = rng_ptr[leaf], r1 = rng_ptr[leaf + 1];
-    const int S = leaf_src_total[leaf];
+// Source staging of the near-field kernels: lane `lane` of a batch starting at `base`
+// walks the leaf's list of point ranges (prefix sums rng_pref over [r0, r1)) to the range
+// holding source v = base + lane; `ri` carries the range index from batch to batch (it
+// only grows), so the walk is a step or two per batch.
+__device__ inline int p2p_source_index(int v, int r1, const int* rng_start, const int* rng_pref, int& ri) {
+    while (ri + 1 < r1 && rng_pref[ri + 1] <= v) ++ri;
+    return rng_start[ri] + (v - rng_pref[ri]);
+}
+
+// Target chunk of a leaf: up to 64 R targets; a remainder that is not a power of two
+// (and above 4 R) goes as its largest power of two first, so that few target slots of
+// the R x L lanes are idle (each chunk stages the sources once more).
+template <int R>
+__device__ inline int p2p_chunk(int remaining) {
+    if (remaining >= 64 * R) return 64 * R;
+    if (remaining <= 4 * R) return remaining;
+    int p = 1;
+    while (2 * p <= remaining) p <<= 1;
+    return p;
+}
+
+// Mixed-precision near field: one wave per target leaf.  Each lane holds R targets of the
+// chunk (li, li + L, ..., with L lanes per group, L = next power of two >= chunk / R) and
+// scans every G-th source (G = 64 / L groups) of each 64-point batch staged in LDS, so
+// that every source read from LDS feeds R independent pair evaluations.  Coordinates are
+// relative to the target leaf centre and scaled by a.scale = k / pi, in FP64: the
+// coordinate differences and r^2 are FP64 (exact to rounding for close pairs), the
+// weights e^{i pi u}/u (u = k r / pi) FP32 (halfturn_weight; PH: phase in FP64), the
+// 64-source batch sums FP32, added into FP64 sums per target.  LOWK (k r below 1e-4 over
+// every pair of the plan, k = 0 included): coordinates unscaled and the weights
+// 1/r + i k, exact to FP32 rounding.  Pairs with r^2 at or below the leaf's coincidence
+// floor contribute nothing.  The sums times a.wscale (k / pi; 1 for LOWK) are added to the
+// far field and written in input order.  The workgroup handles leaf a.leaf0 + blockIdx.x.
+template <bool PH, bool LOWK, int R>
+__global__ __launch_bounds__(64) void p2p_mixed_kernel(P2PArgs a) {
+    __shared__ double2 sxy[64];
+    __shared__ double2 szq[64];     // (z, charge as two FP32 bit-cast into one double)
+    const int leaf = a.leaf0 + blockIdx.x;
+    const int b = a.leaf_box[leaf];
+    const int tstart = a.bx.pstart[b], tcnt = a.bx.count[b];
+    const double sc = a.scale;
+    const double cx = a.bx.centre[3 * b], cy = a.bx.centre[3 * b + 1], cz = a.bx.centre[3 * b + 2];
+    const int r0 = a.rng_ptr[leaf], r1 = a.rng_ptr[leaf + 1];
+    const int S = a.leaf_src_total[leaf];
+    // the floor in scaled coordinates, and at least 2^-120 so that an FP32 r^2 above it is
+    // a normal number
+    const double fl2 = fmax(a.floor2[leaf] * sc * sc, 0x1p-120);
+    const float kf = (float)a.k;
     const int lane = threadIdx.x;
-    const T kk = (T)k;
+    for (int t0 = 0, tc = 0; t0 < tcnt; t0 += tc) {
+        tc = p2p_chunk<R>(tcnt - t0);
+        const int need = (tc + R - 1) / R;
+        int L = 1;
+        while (L < need) L <<= 1;
+        const int G = 64 / L;
+        const int li = lane & (L - 1), g = lane / L;
+        double xt[R], yt[R], zt[R], sr[R], si[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int it = tstart + t0 + min(li + r * L, tc - 1);
+            xt[r] = (a.X[it] - cx) * sc;
+            yt[r] = (a.Y[it] - cy) * sc;
+            zt[r] = (a.Z[it] - cz) * sc;
+            sr[r] = 0.0;
+            si[r] = 0.0;
+        }
+        int ri = r0;
+        for (int base = 0; base < S; base += 64) {
+            const int v = base + lane;
+            if (v < S) {
+                const int idx = p2p_source_index(v, r1, a.rng_start, a.rng_pref, ri);
+                const double2 qq = a.q[idx];
+                const float2 qf = make_float2((float)qq.x, (float)qq.y);
+                sxy[lane] = make_double2((a.X[idx] - cx) * sc, (a.Y[idx] - cy) * sc);
+                szq[lane] = make_double2((a.Z[idx] - cz) * sc, __builtin_bit_cast(double, qf));
+            }
+            __syncthreads();
+            const int nb = min(64, S - base);
+            float ar[R], ai[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) ar[r] = ai[r] = 0.0f;
+            for (int j = g; j < nb; j += G) {
+                const double2 pxy = sxy[j], pzq = szq[j];
+                const float2 qj = __builtin_bit_cast(float2, pzq.y);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const double dx = xt[r] - pxy.x, dy = yt[r] - pxy.y, dz = zt[r] - pzq.x;
+                    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                    float wr, wi;
+                    if constexpr (LOWK) {
+                        const bool ok = r2 > fl2;
+                        const float rs = nearfield::rsq((float)r2);
+                        wr = ok ? rs : 0.0f;
+                        wi = ok ? kf : 0.0f;
+                    } else {
+                        nearfield::pair_weight<PH>(r2, fl2, wr, wi);
+                    }
+                    ar[r] = fmaf(wr, qj.x, ar[r]);
+                    ar[r] = fmaf(-wi, qj.y, ar[r]);
+                    ai[r] = fmaf(wr, qj.y, ai[r]);
+                    ai[r] = fmaf(wi, qj.x, ai[r]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                sr[r] += (double)ar[r];
+                si[r] += (double)ai[r];
+            }
+            __syncthreads();
+        }
+        for (int off = L; off < 64; off <<= 1)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                sr[r] += __shfl_xor(sr[r], off);
+                si[r] += __shfl_xor(si[r], off);
+            }
+        if (g == 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int ti = li + r * L;
+                if (ti < tc) {
+                    const int it = tstart + t0 + ti;
+                    double2 f = a.far[it];
+                    f.x = fma(sr[r], a.wscale, f.x);
+                    f.y = fma(si[r], a.wscale, f.y);
+                    if (a.out_mode & 1) f.y = -f.y;   // k < 0: conjugate of the |k| result
+                    if (a.out_mode & 2) f.y = 0.0;    // k == 0 with real charges: the result is real
+                    a.out[a.perm[it]] = f;
+                }
+            }
+        }
+    }
+}
+
+// FP64 near field (the plain form): one wave per target leaf; targets of a chunk of up to
+// 64 points are spread over lane groups: with T' = next power of two >= chunk size, lane
+// l holds target l % T' and scans the sources j == l / T' (mod 64/T') of each 64-point
+// source batch staged in LDS (coordinates relative to the target leaf centre).  Lane-group
+// partial sums are combined with wavefront shuffles.  Sources of a leaf are a list of
+// point ranges (adjacent leaves, and the W/X interactions evaluated directly); pairs at or
+// below the leaf's coincidence floor are skipped.
+__global__ __launch_bounds__(64) void p2p_fp64_kernel(P2PArgs a) {
+    __shared__ double sx[64], sy[64], sz[64], sqr[64], sqi[64];
+    const int leaf = a.leaf0 + blockIdx.x;
+    const int b = a.leaf_box[leaf];
+    const int tstart = a.bx.pstart[b], tcnt = a.bx.count[b];
+    const double cx = a.bx.centre[3 * b], cy = a.bx.centre[3 * b + 1], cz = a.bx.centre[3 * b + 2];
+    const int r0 = a.rng_ptr[leaf], r1 = a.rng_ptr[leaf + 1];
+    const int S = a.leaf_src_total[leaf];
+    const double fl2 = a.floor2[leaf];
+    const int lane = threadIdx.x;
     // target chunks of 64; a remainder of 33..63 targets goes as 32 + the rest, so that the
     // lanes are not left idle with T' = 64 (each lane then scans half or less of the sources)
     for (int t0 = 0, tc = 0; t0 < tcnt; t0 += tc) {
@@ -691,51 +759,33 @@ __global__ __launch_bounds__(64) void p2p_kernel(int leaf0, const int* leaf_box,
         const int ti = lane & (Tp - 1), g = lane / Tp;
         const bool has_t = ti < tc;
         const int it = tstart + t0 + (has_t ? ti : 0);
-        const C xt = (C)(X[it] - cx), yt = (C)(Y[it] - cy), zt = (C)(Z[it] - cz);
-        // per-batch partial sums in T (at most 64 terms per lane), added into FP64 sums:
-        // a single FP32 running sum over thousands of near sources (dense clusters) would
-        // lose sqrt(n) ulps
+        const double xt = a.X[it] - cx, yt = a.Y[it] - cy, zt = a.Z[it] - cz;
         double sr = 0.0, si = 0.0;
+        int ri = r0;
         for (int base = 0; base < S; base += 64) {
-            T ar = 0, ai = 0;
             const int v = base + lane;
             if (v < S) {
-                int lo = r0, hi = r1 - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (rng_pref[mid] <= v)
-                        lo = mid;
-                    else
-                        hi = mid - 1;
-                }
-                const int idx = rng_start[lo] + (v - rng_pref[lo]);
-                sx[lane] = (C)(X[idx] - cx);
-                sy[lane] = (C)(Y[idx] - cy);
-                sz[lane] = (C)(Z[idx] - cz);
-                const double2 qq = q[idx];
-                sqr[lane] = (T)qq.x;
-                sqi[lane] = (T)qq.y;
+                const int idx = p2p_source_index(v, r1, a.rng_start, a.rng_pref, ri);
+                sx[lane] = a.X[idx] - cx;
+                sy[lane] = a.Y[idx] - cy;
+                sz[lane] = a.Z[idx] - cz;
+                const double2 qq = a.q[idx];
+                sqr[lane] = qq.x;
+                sqi[lane] = qq.y;
             }
             __syncthreads();
             const int nb = min(64, S - base);
             for (int j = g; j < nb; j += G) {
-                const C dx = xt - sx[j], dy = yt - sy[j], dz = zt - sz[j];
-                const C r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 > (C)0) {
-                    T wr, wi;
-                    if constexpr (PH) {
-                        float rinv, kr;
-                        phase_fp64_weights(r2, k, wr, wi, rinv, kr);
-                    } else {
-                        P2PMath<T>::kernel((T)r2, kk, wr, wi);
-                    }
-                    const T qr = sqr[j], qi = sqi[j];
-                    ar += wr * qr - wi * qi;
-                    ai += wr * qi + wi * qr;
+                const double dx = xt - sx[j], dy = yt - sy[j], dz = zt - sz[j];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > fl2) {
+                    double wr, wi;
+                    P2PMath<double>::kernel(r2, a.k, wr, wi);
+                    const double qr = sqr[j], qi = sqi[j];
+                    sr += wr * qr - wi * qi;
+                    si += wr * qi + wi * qr;
                 }
             }
-            sr += (double)ar;
-            si += (double)ai;
             __syncthreads();
         }
         for (int off = Tp; off < 64; off <<= 1) {
@@ -743,12 +793,12 @@ __global__ __launch_bounds__(64) void p2p_kernel(int leaf0, const int* leaf_box,
             si += __shfl_xor(si, off);
         }
         if (g == 0 && has_t) {
-            double2 f = far[it];
+            double2 f = a.far[it];
             f.x += sr;
             f.y += si;
-            if (out_mode & 1) f.y = -f.y;   // k < 0: conjugate of the |k| result
-            if (out_mode & 2) f.y = 0.0;    // k == 0 with real charges: the result is real
-            out[perm[it]] = f;
+            if (a.out_mode & 1) f.y = -f.y;   // k < 0: conjugate of the |k| result
+            if (a.out_mode & 2) f.y = 0.0;    // k == 0 with real charges: the result is real
+            a.out[a.perm[it]] = f;
         }
     }
 }
@@ -759,10 +809,11 @@ __global__ __launch_bounds__(64) void p2p_kernel(int leaf0, const int* leaf_box,
 template <typename T, bool PH>
 __global__ __launch_bounds__(64) void p2p_field_kernel(int leaf0, const int* leaf_box, DevBoxes bx,
                                                        const int* rng_ptr, const int* rng_start, const int* rng_pref,
-                                                       const int* leaf_src_total, const double* X, const double* Y,
-                                                       const double* Z, const double2* q, double k,
-                                                       const double2* far, const double2* far_e, const int* perm,
-                                                       int out_mode, double2* out, double2* field) {
+                                                       const int* leaf_src_total, const double* floor2,
+                                                       const double* X, const double* Y, const double* Z,
+                                                       const double2* q, double k, const double2* far,
+                                                       const double2* far_e, const int* perm, int out_mode,
+                                                       double2* out, double2* field) {
     // coordinates relative to the leaf centre and r^2 in FP64 at every precision: FP32
     // coordinates would carry an absolute error of 2^-24 times the leaf size, a large
     // relative error of r for close pairs (points on surfaces); FP64 adds and FMAs issue
@@ -776,6 +827,7 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(int leaf0, const int* lea
     const double cx = bx.centre[3 * b], cy = bx.centre[3 * b + 1], cz = bx.centre[3 * b + 2];
     const int r0 = rng_ptr[leaf], r1 = rng_ptr[leaf + 1];
     const int S = leaf_src_total[leaf];
+    const double fl2 = floor2[leaf];
     const int lane = threadIdx.x;
     const T kk = (T)k;
     // target chunks of 64; a remainder of 33..63 targets goes as 32 + the rest, so that the
@@ -815,11 +867,11 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(int leaf0, const int* lea
             for (int j = g; j < nb; j += G) {
                 const C dx = xt - sx[j], dy = yt - sy[j], dz = zt - sz[j];
                 const C r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 > (C)0) {
+                if (r2 > fl2) {
                     T wr, wi, fr, fi;
                     if constexpr (PH) {
                         float rinv, kr;
-                        phase_fp64_weights(r2, k, wr, wi, rinv, kr);
+                        radian_weight_phase64(r2, k, wr, wi, rinv, kr);
                         const float rinv2 = rinv * rinv;
                         fr = (wr + kr * wi) * rinv2;
                         fi = (wi - kr * wr) * rinv2;
@@ -1002,23 +1054,23 @@ void launch_l2p_field(const int* leaves, int n_items, DevBoxes bx, DevLevels lv,
 
 void launch_p2p_field(int precision, int n_leaves, int n_fp32, const int* leaf_box, DevBoxes bx,
                       const int* rng_ptr, const int* rng_start, const int* rng_pref, const int* leaf_src_total,
-                      const double* X, const double* Y, const double* Z, const double2* q, double k,
-                      const double2* far, const double2* far_e, const int* perm, int out_mode, double2* out,
-                      double2* field, hipStream_t s) {
+                      const double* floor2, const double* X, const double* Y, const double* Z, const double2* q,
+                      double k, const double2* far, const double2* far_e, const int* perm, int out_mode,
+                      double2* out, double2* field, hipStream_t s) {
     if (n_leaves <= 0) return;
     if (precision == 0) {
         hipLaunchKernelGGL((p2p_field_kernel<double, false>), dim3(n_leaves), dim3(64), 0, s, 0, leaf_box, bx,
-                           rng_ptr, rng_start, rng_pref, leaf_src_total, X, Y, Z, q, k, far, far_e, perm, out_mode,
-                           out, field);
+                           rng_ptr, rng_start, rng_pref, leaf_src_total, floor2, X, Y, Z, q, k, far, far_e, perm,
+                           out_mode, out, field);
     } else {
         if (n_fp32 > 0)
             hipLaunchKernelGGL((p2p_field_kernel<float, false>), dim3(n_fp32), dim3(64), 0, s, 0, leaf_box, bx,
-                               rng_ptr, rng_start, rng_pref, leaf_src_total, X, Y, Z, q, k, far, far_e, perm,
-                               out_mode, out, field);
+                               rng_ptr, rng_start, rng_pref, leaf_src_total, floor2, X, Y, Z, q, k, far, far_e,
+                               perm, out_mode, out, field);
         if (n_leaves > n_fp32)
             hipLaunchKernelGGL((p2p_field_kernel<float, true>), dim3(n_leaves - n_fp32), dim3(64), 0, s, n_fp32,
-                               leaf_box, bx, rng_ptr, rng_start, rng_pref, leaf_src_total, X, Y, Z, q, k, far, far_e,
-                               perm, out_mode, out, field);
+                               leaf_box, bx, rng_ptr, rng_start, rng_pref, leaf_src_total, floor2, X, Y, Z, q, k,
+                               far, far_e, perm, out_mode, out, field);
     }
     check_launch("p2p_field");
 }
@@ -1032,23 +1084,36 @@ void launch_xlate(const XlateArgs& a, int n_targets, hipStream_t s) {
     check_launch("xlate");
 }
 
-void launch_p2p(int precision, int n_leaves, int n_fp32, const int* leaf_box, DevBoxes bx, const int* rng_ptr,
-                const int* rng_start, const int* rng_pref, const int* leaf_src_total, const double* X,
-                const double* Y, const double* Z, const double2* q, double k, const double2* far, const int* perm,
-                int out_mode, double2* out, hipStream_t s) {
-    if (n_leaves <= 0) return;
-    if (precision == 0) {
-        hipLaunchKernelGGL((p2p_kernel<double, false>), dim3(n_leaves), dim3(64), 0, s, 0, leaf_box, bx, rng_ptr,
-                           rng_start, rng_pref, leaf_src_total, X, Y, Z, q, k, far, perm, out_mode, out);
-    } else {
-        if (n_fp32 > 0)
-            hipLaunchKernelGGL((p2p_kernel<float, false>), dim3(n_fp32), dim3(64), 0, s, 0, leaf_box, bx, rng_ptr,
-                               rng_start, rng_pref, leaf_src_total, X, Y, Z, q, k, far, perm, out_mode, out);
-        if (n_leaves > n_fp32)
-            hipLaunchKernelGGL((p2p_kernel<float, true>), dim3(n_leaves - n_fp32), dim3(64), 0, s, n_fp32, leaf_box,
-                               bx, rng_ptr, rng_start, rng_pref, leaf_src_total, X, Y, Z, q, k, far, perm, out_mode,
-                               out);
+template <int R>
+static void launch_p2p_mixed(P2PArgs a, int n_leaves, int n_fp32, bool lowk, hipStream_t s) {
+    if (lowk) {
+        a.scale = 1.0;
+        a.wscale = 1.0;
+        hipLaunchKernelGGL((p2p_mixed_kernel<false, true, R>), dim3(n_leaves), dim3(64), 0, s, a);
+        return;
     }
+    // coordinates in units of pi / k: the weights are e^{i pi u}/u with u = k r / pi
+    a.scale = a.k / M_PI;
+    a.wscale = a.k / M_PI;
+    if (n_fp32 > 0) hipLaunchKernelGGL((p2p_mixed_kernel<false, false, R>), dim3(n_fp32), dim3(64), 0, s, a);
+    if (n_leaves > n_fp32) {
+        a.leaf0 = n_fp32;
+        hipLaunchKernelGGL((p2p_mixed_kernel<true, false, R>), dim3(n_leaves - n_fp32), dim3(64), 0, s, a);
+    }
+}
+
+void launch_p2p(const P2PArgs& a0, int precision, int n_leaves, int n_fp32, bool lowk, int tile, hipStream_t s) {
+    if (n_leaves <= 0) return;
+    P2PArgs a = a0;
+    a.leaf0 = 0;
+    if (precision == 0)
+        hipLaunchKernelGGL(p2p_fp64_kernel, dim3(n_leaves), dim3(64), 0, s, a);
+    else if (tile == 1)
+        launch_p2p_mixed<1>(a, n_leaves, n_fp32, lowk, s);
+    else if (tile == 4)
+        launch_p2p_mixed<4>(a, n_leaves, n_fp32, lowk, s);
+    else
+        launch_p2p_mixed<2>(a, n_leaves, n_fp32, lowk, s);
     check_launch("p2p");
 }
 

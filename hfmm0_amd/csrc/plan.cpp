@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <map>
 #include <memory>
 #include <tuple>
@@ -102,6 +103,9 @@ struct hfmm_plan {
     int64_t n_p2p_pairs = 0;
     int n_leaves = 0, n_far_leaves = 0, p2m_chunk = 8, p_leafmax = 2;
     int n_p2p_fp32 = 0;             // leading leaves of the FP32 near field (phase in FP32)
+    bool p2p_lowk = false;          // every near pair has k r <= kP2PLowKr (mixed form)
+    std::vector<double> box_maxabs; // setup only: largest |x| of the points of each box
+    double* d_floor2 = nullptr;     // per leaf (leaf_box order): coincidence floor on r^2
     unsigned degree_version = 0;    // bumped by set_degrees (graphs that captured this plan are stale)
 
     std::vector<void*> allocs;
@@ -123,6 +127,7 @@ struct hfmm_plan {
     bool m2l_fp32[kMaxLevels] = {};  // M2L of the level in FP32 (mixed precision)
     M2LBlockLevel m2mb[kMaxLevels], l2lb[kMaxLevels];   // M2M / L2L on matrix cores (FP64)
     int m2l_kernel = 0;
+    int p2p_tile = 2;               // targets per lane of the mixed near field
     size_t max_lds = 65536;
     // rotation blocks as MFMA A fragments (E^T for the forward, E for the back rotation)
     double *d_rf1 = nullptr, *d_rf3 = nullptr;
@@ -751,6 +756,7 @@ void build_p2p(hfmm_plan& P) {
     // leaves whose pairs stay within k r <= kP2PFp32MaxKr first (FP32 near field), then the
     // rest (FP32 with the phase in FP64); Morton order within each class
     std::vector<int> leaves[2];
+    double kr_max = 0.0;
     for (int b = 0; b < nb; ++b) {
         if (!t.is_leaf[b]) continue;
         double kr = 0.0;
@@ -770,9 +776,12 @@ void build_p2p(hfmm_plan& P) {
                 for (int e = xp[a]; e < xp[a + 1]; ++e) reach(xs[e]);
         }
         leaves[kr > kP2PFp32MaxKr].push_back(b);
+        kr_max = std::max(kr_max, kr);
     }
     P.n_p2p_fp32 = (int)leaves[0].size();
+    P.p2p_lowk = P.precision == HFMM_PREC_MIXED && kr_max <= kP2PLowKr;
     std::vector<int> leaf_box, rng_ptr{0}, rng_start, rng_pref, total;
+    std::vector<double> floor2;
     std::vector<int> far_leaves;
     int64_t pairs = 0;
     for (int b = 0; b < nb; ++b)
@@ -781,10 +790,12 @@ void build_p2p(hfmm_plan& P) {
         for (int b : leaves[cls]) {
             leaf_box.push_back(b);
             int acc = 0;
+            double src_maxabs = 0.0;
             auto add = [&](int box) {
                 rng_start.push_back(t.pstart[box]);
                 rng_pref.push_back(acc);
                 acc += t.count[box];
+                src_maxabs = std::max(src_maxabs, P.box_maxabs[box]);
             };
             add(b);
             for (int e = up[b]; e < up[b + 1]; ++e) add(us[e]);
@@ -793,6 +804,10 @@ void build_p2p(hfmm_plan& P) {
                 for (int e = xp[a]; e < xp[a + 1]; ++e) add(xs[e]);
             rng_ptr.push_back((int)rng_start.size());
             total.push_back(acc);
+            // the reference's coincidence floor (jaxfmm/near.py:252-261, "pair" form in FP64):
+            // pairs with r <= 16 eps (max |x_t| + max |x_s|) contribute nothing
+            const double fl = 16.0 * std::numeric_limits<double>::epsilon() * (P.box_maxabs[b] + src_maxabs);
+            floor2.push_back(fl * fl);
             pairs += (int64_t)acc * t.count[b];
         }
     P.n_leaves = (int)leaf_box.size();
@@ -804,6 +819,8 @@ void build_p2p(hfmm_plan& P) {
     P.d_rng_start = P.upload(rng_start);
     P.d_rng_pref = P.upload(rng_pref);
     P.d_leaf_src_total = P.upload(total);
+    P.d_floor2 = P.upload(floor2);
+    P.box_maxabs = std::vector<double>();
 }
 
 void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm_opts& o) {
@@ -820,6 +837,8 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
     if (o.m2l_kernel != 0 && o.m2l_kernel != 1 && o.m2l_kernel != 8 && o.m2l_kernel != 16 && o.m2l_kernel != 32 &&
         o.m2l_kernel != 64)
         fail(HFMM_ERR_ARG, "m2l_kernel must be 0, 1, 8, 16, 32 or 64");
+    if (o.p2p_tile != 0 && o.p2p_tile != 1 && o.p2p_tile != 2 && o.p2p_tile != 4)
+        fail(HFMM_ERR_ARG, "p2p_tile must be 0, 1, 2 or 4");
     check_device();
     upload_legendre_coefficients();
     P.n = n;
@@ -828,6 +847,7 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
     P.precision = o.precision;
     P.use_graph = o.use_graph;
     P.m2l_kernel = o.m2l_kernel;
+    P.p2p_tile = o.p2p_tile == 0 ? 2 : o.p2p_tile;
     {
         int dev = 0, lds = 65536;
         HIPCHK(hipGetDevice(&dev));
@@ -870,6 +890,15 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
     P.d_perm = P.upload(perm);
     // boxes
     const int nb = t.n_boxes();
+    // largest |x| per box (points of leaves, then children into parents: boxes are in level
+    // order, so children come after their parents), for the near field's coincidence floor
+    P.box_maxabs.assign(nb, 0.0);
+    for (int b = nb - 1; b >= 0; --b) {
+        if (t.is_leaf[b])
+            for (int i = t.pstart[b]; i < t.pstart[b] + t.count[b]; ++i)
+                P.box_maxabs[b] = std::max(P.box_maxabs[b], std::sqrt(X[i] * X[i] + Y[i] * Y[i] + Z[i] * Z[i]));
+        if (t.parent[b] >= 0) P.box_maxabs[t.parent[b]] = std::max(P.box_maxabs[t.parent[b]], P.box_maxabs[b]);
+    }
     std::vector<double> centre(3 * (size_t)nb);
     P.exp_off.assign(nb, 0);
     int64_t off = 0;
@@ -1041,13 +1070,29 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
                          P.d_far_e, s);
         mark(6);
         launch_p2p_field(P.precision, P.n_leaves, P.n_p2p_fp32, P.d_leaf_box, bx, P.d_rng_ptr, P.d_rng_start, P.d_rng_pref,
-                         P.d_leaf_src_total, P.dX, P.dY, P.dZ, P.d_q, P.k, P.d_far, P.d_far_e, P.d_perm, out_mode,
+                         P.d_leaf_src_total, P.d_floor2, P.dX, P.dY, P.dZ, P.d_q, P.k, P.d_far, P.d_far_e, P.d_perm, out_mode,
                          out, field, s);
     } else {
         launch_l2p(P.d_far_leaves, P.n_far_leaves, bx, lv, P.dX, P.dY, P.dZ, P.dL, P.k, P.pmax, P.d_far, s);
         mark(6);
-        launch_p2p(P.precision, P.n_leaves, P.n_p2p_fp32, P.d_leaf_box, bx, P.d_rng_ptr, P.d_rng_start, P.d_rng_pref,
-                   P.d_leaf_src_total, P.dX, P.dY, P.dZ, P.d_q, P.k, P.d_far, P.d_perm, out_mode, out, s);
+        P2PArgs pa{};
+        pa.leaf_box = P.d_leaf_box;
+        pa.bx = bx;
+        pa.rng_ptr = P.d_rng_ptr;
+        pa.rng_start = P.d_rng_start;
+        pa.rng_pref = P.d_rng_pref;
+        pa.leaf_src_total = P.d_leaf_src_total;
+        pa.floor2 = P.d_floor2;
+        pa.X = P.dX;
+        pa.Y = P.dY;
+        pa.Z = P.dZ;
+        pa.q = P.d_q;
+        pa.k = P.k;
+        pa.far = P.d_far;
+        pa.perm = P.d_perm;
+        pa.out_mode = out_mode;
+        pa.out = out;
+        launch_p2p(pa, P.precision, P.n_leaves, P.n_p2p_fp32, P.p2p_lowk, P.p2p_tile, s);
     }
     mark(7);
 }
@@ -1138,6 +1183,7 @@ void hfmm_opts_default(hfmm_opts* o) {
     o->use_graph = 1;
     o->m2l_kernel = 0;
     o->m2l_fp64_levels = -1;
+    o->p2p_tile = 0;
 }
 
 int hfmm_plan_create(const double* xyz, int64_t n, double k, const hfmm_opts* o, hfmm_plan** out) {

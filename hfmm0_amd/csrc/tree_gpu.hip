@@ -2,7 +2,8 @@
 // arrays of the host builder (tree.cpp, which follows jaxfmm/trees/cubic.py:316-695 and
 // jaxfmm/trees/lists.py:619-931); the results are identical, element for element.
 //
-// Points: Morton codes at the finest level and a stable radix sort of (code, index).
+// Points: Morton codes at the finest level and a stable radix sort of (code, index)
+// (rocPRIM device primitives: radix sort, run-length encoding, reduce-by-key, select).
 // Boxes: run-length encoding of the sorted codes gives the finest boxes; then, level by
 // level from the finest, parents by reduce-by-key on code >> 3, the merge test (count at
 // most N_max and the 2:1 balance check over the 26 neighbours) one thread per parent,
@@ -11,7 +12,10 @@
 // Lists: one thread per box counts its V / U / W / X entries, a scan gives offsets, the
 // same thread fills them, and a radix sort on (target, source) fixes the order.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce_by_key.hpp>
+#include <rocprim/device/device_run_length_encode.hpp>
+#include <rocprim/device/device_select.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -318,21 +322,17 @@ void build_levels(const double* d_xyz, int64_t n, int md, int n_max, const doubl
                        grid, (1ull << md) - 1, codes.p, idx.p);
     tchk(hipGetLastError(), "codes kernel");
     size_t bytes = 0;
-    tchk(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, codes.p, scodes.p, idx.p, sidx.p, (int)n, 0, 3 * md),
-         "sort size");
-    tchk(hipcub::DeviceRadixSort::SortPairs(tmp.get(bytes), bytes, codes.p, scodes.p, idx.p, sidx.p, (int)n, 0, 3 * md),
-         "sort");
+    const unsigned un = (unsigned)n, mbits = (unsigned)(3 * md);
+    tchk(rocprim::radix_sort_pairs(nullptr, bytes, codes.p, scodes.p, idx.p, sidx.p, un, 0u, mbits), "sort size");
+    tchk(rocprim::radix_sort_pairs(tmp.get(bytes), bytes, codes.p, scodes.p, idx.p, sidx.p, un, 0u, mbits), "sort");
     lv.clear();
     lv.resize(md + 2);
     DBuf<int> nruns(1);
     DLevel& fin = lv[md];
     fin.code.resize(n);
     fin.count.resize(n);
-    tchk(hipcub::DeviceRunLengthEncode::Encode(nullptr, bytes, scodes.p, fin.code.p, fin.count.p, nruns.p, (int)n),
-         "rle size");
-    tchk(hipcub::DeviceRunLengthEncode::Encode(tmp.get(bytes), bytes, scodes.p, fin.code.p, fin.count.p, nruns.p,
-                                                (int)n),
-         "rle");
+    tchk(rocprim::run_length_encode(nullptr, bytes, scodes.p, un, fin.code.p, fin.count.p, nruns.p), "rle size");
+    tchk(rocprim::run_length_encode(tmp.get(bytes), bytes, scodes.p, un, fin.code.p, fin.count.p, nruns.p), "rle");
     fin.n = read1(nruns.p);
     fin.leaf.resize(fin.n);
     hipLaunchKernelGGL(fill_u8_kernel, dim3(blocks(fin.n)), dim3(256), 0, 0, fin.leaf.p, fin.n, (uint8_t)1);
@@ -343,11 +343,12 @@ void build_levels(const double* d_xyz, int64_t n, int md, int n_max, const doubl
         DLevel par;
         par.code.resize(cur.n);
         par.count.resize(cur.n);
-        tchk(hipcub::DeviceReduce::ReduceByKey(nullptr, bytes, keys.p, par.code.p, cur.count.p, par.count.p, nruns.p,
-                                               hipcub::Sum(), cur.n),
+        const size_t cn = (size_t)cur.n;
+        tchk(rocprim::reduce_by_key(nullptr, bytes, keys.p, cur.count.p, cn, par.code.p, par.count.p, nruns.p,
+                                    rocprim::plus<int>()),
              "reduce size");
-        tchk(hipcub::DeviceReduce::ReduceByKey(tmp.get(bytes), bytes, keys.p, par.code.p, cur.count.p, par.count.p,
-                                               nruns.p, hipcub::Sum(), cur.n),
+        tchk(rocprim::reduce_by_key(tmp.get(bytes), bytes, keys.p, cur.count.p, cn, par.code.p, par.count.p, nruns.p,
+                                    rocprim::plus<int>()),
              "reduce");
         par.n = read1(nruns.p);
         const uint64_t last_code = read1(cur.code.p + cur.n - 1);
@@ -364,18 +365,12 @@ void build_levels(const double* d_xyz, int64_t n, int md, int n_max, const doubl
         kept.code.resize(cur.n);
         kept.count.resize(cur.n);
         kept.leaf.resize(cur.n);
-        tchk(hipcub::DeviceSelect::Flagged(nullptr, bytes, cur.code.p, keep.p, kept.code.p, nruns.p, cur.n),
-             "select size");
-        tchk(hipcub::DeviceSelect::Flagged(tmp.get(bytes), bytes, cur.code.p, keep.p, kept.code.p, nruns.p, cur.n),
-             "select");
-        tchk(hipcub::DeviceSelect::Flagged(nullptr, bytes, cur.count.p, keep.p, kept.count.p, nruns.p, cur.n),
-             "select size");
-        tchk(hipcub::DeviceSelect::Flagged(tmp.get(bytes), bytes, cur.count.p, keep.p, kept.count.p, nruns.p, cur.n),
-             "select");
-        tchk(hipcub::DeviceSelect::Flagged(nullptr, bytes, cur.leaf.p, keep.p, kept.leaf.p, nruns.p, cur.n),
-             "select size");
-        tchk(hipcub::DeviceSelect::Flagged(tmp.get(bytes), bytes, cur.leaf.p, keep.p, kept.leaf.p, nruns.p, cur.n),
-             "select");
+        tchk(rocprim::select(nullptr, bytes, cur.code.p, keep.p, kept.code.p, nruns.p, cn), "select size");
+        tchk(rocprim::select(tmp.get(bytes), bytes, cur.code.p, keep.p, kept.code.p, nruns.p, cn), "select");
+        tchk(rocprim::select(nullptr, bytes, cur.count.p, keep.p, kept.count.p, nruns.p, cn), "select size");
+        tchk(rocprim::select(tmp.get(bytes), bytes, cur.count.p, keep.p, kept.count.p, nruns.p, cn), "select");
+        tchk(rocprim::select(nullptr, bytes, cur.leaf.p, keep.p, kept.leaf.p, nruns.p, cn), "select size");
+        tchk(rocprim::select(tmp.get(bytes), bytes, cur.leaf.p, keep.p, kept.leaf.p, nruns.p, cn), "select");
         kept.n = read1(nruns.p);
         cur = std::move(kept);
         par.leaf = std::move(merge);
@@ -391,8 +386,9 @@ void finish_list(DBuf<uint64_t>& keys, int64_t m, int nb, Temp& tmp, std::vector
     DBuf<uint64_t> sorted(m);
     size_t bytes = 0;
     const int end_bit = 32 + bits_for((uint64_t)nb);
-    tchk(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, keys.p, sorted.p, (int)m, 0, end_bit), "sort size");
-    tchk(hipcub::DeviceRadixSort::SortKeys(tmp.get(bytes), bytes, keys.p, sorted.p, (int)m, 0, end_bit), "sort");
+    const unsigned um = (unsigned)m, ebits = (unsigned)end_bit;
+    tchk(rocprim::radix_sort_keys(nullptr, bytes, keys.p, sorted.p, um, 0u, ebits), "sort size");
+    tchk(rocprim::radix_sort_keys(tmp.get(bytes), bytes, keys.p, sorted.p, um, 0u, ebits), "sort");
     const std::vector<uint64_t> h = sorted.host(m);
     tgt.resize(m);
     src.resize(m);

@@ -63,6 +63,8 @@ typedef struct hfmm_opts {
     int m2l_fp64_levels;            /* HFMM_PREC_MIXED only: bit l set keeps the M2L of level l in
                                        FP64; -1 (the default): automatic, FP64 at the levels whose
                                        box diagonal D_l has k D_l > HFMM_MIXED_FP32_MAX_KD */
+    int p2p_tile;                   /* HFMM_PREC_MIXED near field: targets per lane (1, 2 or 4);
+                                       0: the default (2) */
 } hfmm_opts;
 
 /* Largest k D_l (D_l = 2 sqrt(3) extent 2^-(l+1), the box diagonal of level l) whose M2L the

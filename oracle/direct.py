@@ -4,7 +4,9 @@ Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
 leg may import this module, and only as the checker or the timed CPU baseline.
 
 * :func:`direct_numpy` restates ``test.py:11-18`` (256-row chunks, complex128,
-  ``exp(ikr)/r`` with the r = 0 term excluded, no 1/(4 pi)).
+  ``exp(ikr)/r`` with the r = 0 term excluded, no 1/(4 pi)); with ``floor`` > 0 it
+  excludes every pair with r <= floor instead, the rule of the reference evaluate's near
+  field for near-coincident points (``jaxfmm/near.py:252-261``).
 * :func:`direct_c` is the same sum in C with OpenMP (``oracle/direct.c``), used for
   larger sample sets.
 * :func:`direct_field_numpy` / :func:`direct_field_c`: the field E = -grad phi of the
@@ -29,8 +31,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 
-def direct_numpy(positions, charges, k, targets=None):
-    """phi at ``positions[targets]`` (all points if None) from all sources."""
+def direct_numpy(positions, charges, k, targets=None, floor=0.0):
+    """phi at ``positions[targets]`` (all points if None) from all sources; pairs with
+    r <= floor contribute nothing (floor 0: only r = 0 is excluded, as ``test.py:16``)."""
     positions = np.asarray(positions, dtype=np.float64)
     charges = np.asarray(charges).astype(np.complex128)
     tgt = positions if targets is None else positions[np.asarray(targets)]
@@ -38,7 +41,8 @@ def direct_numpy(positions, charges, k, targets=None):
     for start in range(0, len(tgt), 256):
         x = tgt[start:start + 256]
         r = np.linalg.norm(x[:, None, :] - positions[None, :, :], axis=-1)
-        w = np.where(r > 0, np.exp(1j * k * r) / np.where(r > 0, r, 1.0), 0.0)
+        ok = r > floor
+        w = np.where(ok, np.exp(1j * k * r) / np.where(ok, r, 1.0), 0.0)
         result[start:start + len(x)] = w @ charges
     return result
 

@@ -28,7 +28,10 @@ def test_field_uniform(k, p, precision, tol):
     e = _rel(E[idx], ref)
     print(f"  field rel err {e:.2e}", flush=True)
     assert e < tol
-    np.testing.assert_allclose(phi, hf.evaluate(st, q), rtol=0, atol=1e-12 * np.abs(phi).max())
+    # FP64: the same sums; mixed: the field kernel's FP32 near-field weights (radian form)
+    # and the evaluate's (half-turn form, other lane grouping) agree to FP32 rounding
+    atol = (1e-12 if precision == "fp64" else 1e-6) * np.abs(phi).max()
+    np.testing.assert_allclose(phi, hf.evaluate(st, q), rtol=0, atol=atol)
     if k == 0.0:
         assert np.all(E.imag == 0)
 

@@ -335,3 +335,47 @@ def test_device_benchmark_narrow_charge_types(hf):
         torch.cuda.synchronize()
         assert ms > 0 and lv
         np.testing.assert_allclose(od.cpu().numpy(), ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_near_coincident_points_follow_the_reference_floor(hf, precision):
+    # pairs 1e-15 apart lie below the reference's coincidence floor (16 eps (max|x_t| +
+    # max|x_s|), jaxfmm/near.py:252-261) and contribute nothing; without the floor their
+    # 1e15 terms would swamp the result
+    say(f"case: near-coincident pairs N=4000 k=5 p=14 N_max=32 {precision}")
+    x, q = D.uniform_problem(4000)
+    x = x.copy()
+    x[1::40] = x[0::40][: len(x[1::40])] + np.array([1e-15, 0.0, 0.0])
+    st = hf.setup(x, 5.0, p=14, N_max=32, precision=precision)
+    out = hf.evaluate(st, q)
+    idx = np.unique(np.concatenate([np.arange(0, 400, 1), D.sample_targets(len(x), 200)]))
+    ref = D.direct_numpy(x, q, 5.0, idx, floor=1e-12)
+    e = D.relative_error(out[idx], ref)
+    say(f"  rel err {e:.3e} against the direct sum with the floor")
+    assert np.isfinite(out).all()
+    assert e < (2e-7 if precision == "fp64" else 2e-6)
+    st.close()
+
+
+@pytest.mark.parametrize("k", [0.0, 1e-7, 30.0])
+def test_mixed_near_field_tiles_agree(hf, k):
+    # targets per lane 1, 2 and 4 evaluate the same sums in another order; k = 0 and
+    # k = 1e-7 take the low-frequency weights 1/r + ik
+    say(f"case: mixed near-field tiles N=20000 k={k} p=12 N_max=64")
+    x, q = D.uniform_problem(20000)
+    idx = D.sample_targets(len(x), 300)
+    ref = D.direct_c(x, q, k, idx)
+    outs = []
+    for tile in (1, 2, 4):
+        st = hf.setup(x, k, p=14, N_max=64, precision="mixed", p2p_tile=tile)
+        outs.append(hf.evaluate(st, q))
+        st.close()
+        e = D.relative_error(outs[-1][idx], ref)
+        say(f"  tile {tile}: rel err {e:.3e}")
+        assert e < 2e-6
+    for o in outs[1:]:
+        assert D.relative_error(o, outs[0]) < 1e-6
+    if k == 0.0:
+        st = hf.setup(x, k, p=14, N_max=64, precision="mixed")
+        assert np.count_nonzero(hf.evaluate(st, q.real).imag) == 0
+        st.close()

@@ -47,3 +47,21 @@ def test_validation_messages():
     from hfmm0_amd.tree import build_tree
     with pytest.raises(ValueError, match="nonempty finite"):
         build_tree(np.array([[0.0, np.nan, 1.0]]), 16)
+
+
+def test_host_tree_matches_oracle_digests_300k():
+    # the oracle's tree and lists of a 300k-point sphere + cluster input, as SHA-256
+    # digests (tests/golden/make_tree_fixtures.py)
+    import json
+    import os
+    import sys
+    from hfmm0_amd.tree import build_tree
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    from make_tree_fixtures import digests, sphere_clusters_300k
+    with open(os.path.join(here, "golden", "tree_sphere_clusters_300k.json")) as f:
+        rec = json.load(f)
+    x, nmax = sphere_clusters_300k()
+    got = build_tree(x, nmax)
+    assert got["depth"] == rec["depth"] and got["extent"] == rec["extent"]
+    assert digests(got) == rec["digests"]

@@ -26,6 +26,20 @@ def test_coincident_points_excluded():
         np.testing.assert_allclose(got, [g, g, 6 * g], rtol=1e-15)
 
 
+def test_near_coincident_points_below_the_floor_excluded():
+    # a pair 1e-15 apart: test.py's rule keeps its 1e15 term, the evaluate's near-field
+    # floor (r <= floor) drops it; pairs above the floor are unchanged
+    x = np.array([[0.1, 0.2, 0.3], [0.1 + 1e-15, 0.2, 0.3], [1.1, 0.2, 0.3]])
+    q = np.array([1.0, 5.0, 1.0])
+    k = 2.0
+    g = np.exp(2j) / 1.0
+    got = direct.direct_numpy(x, q, k, floor=1e-12)
+    np.testing.assert_allclose(got, [g, g, 6 * g], rtol=1e-14)
+    raw = direct.direct_numpy(x, q, k)
+    r01 = np.linalg.norm(x[0] - x[1])
+    np.testing.assert_allclose(raw[0] - got[0], 5.0 * np.exp(1j * k * r01) / r01, rtol=1e-12)
+
+
 def test_polygon_coulomb_k0():
     # n unit charges on a unit circle: every point sees sum_{j=1}^{n-1} 1/(2 sin(pi j/n))
     n = 12
