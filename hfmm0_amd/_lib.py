@@ -72,6 +72,7 @@ def _declare(lib):
         "hfmm_batch_benchmark": (I, [vp, vp, I, vp, I, I, ctypes.POINTER(D)]),
         "hfmm_batch_destroy": (I, [vp]),
         "hfmm_direct": (I, [vp, L, vp, D, vp, L, vp]),
+        "hfmm_direct_noalloc": (I, [vp, L, vp, D, vp, L, vp, vp]),
         "hfmm_tree_create": (I, [vp, L, I, I, ctypes.POINTER(vp)]),
         "hfmm_tree_create_device": (I, [vp, L, I, I, ctypes.POINTER(vp)]),
         "hfmm_tree_info_get": (I, [vp, ctypes.POINTER(PlanInfo)]),
@@ -100,7 +101,7 @@ def _declare(lib):
 EXPORTED = [
     "hfmm_opts_default", "hfmm_plan_create", "hfmm_plan_info_get", "hfmm_plan_export_tree", "hfmm_plan_set_degrees",
     "hfmm_plan_export_list", "hfmm_evaluate", "hfmm_evaluate_typed", "hfmm_evaluate_field", "hfmm_benchmark", "hfmm_benchmark_m2l_levels",
-    "hfmm_plan_destroy", "hfmm_direct",
+    "hfmm_plan_destroy", "hfmm_direct", "hfmm_direct_noalloc",
     "hfmm_tree_create", "hfmm_tree_create_device", "hfmm_tree_info_get", "hfmm_tree_export", "hfmm_tree_export_list",
     "hfmm_tree_destroy", "hfmm_host_jhat", "hfmm_host_hhat", "hfmm_host_rotation", "hfmm_host_coax",
     "hfmm_mlp_create", "hfmm_mlp_params", "hfmm_mlp_forward", "hfmm_mlp_gradient", "hfmm_mlp_train",

@@ -158,6 +158,19 @@ int hfmm_plan_destroy(hfmm_plan* plan);
 int hfmm_direct(const double* xyz, int64_t n, const double* q, double k, const int64_t* targets,
                 int64_t n_targets, double* out);
 
+/* The same sum on device buffers with no device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under hipGraph capture, serving
+ * loops, harnesses that forbid hipMalloc).  xyz: n x 3 FP64, q: n complex128, targets:
+ * n_targets int64 (an index outside [0, n) gives NaN), out: n_targets complex128, all in
+ * device memory; the kernels run on `stream` (a hipStream_t, NULL for the default stream)
+ * and the call returns without waiting.  The partial sums live in a fixed device array of
+ * HFMM_DIRECT_NOALLOC_MAX_TARGETS complex128 accumulators (1 MiB) per GPU, held until the
+ * process ends and cleared by each call: calls that overlap on one GPU race.  Returns
+ * HFMM_ERR_ARG if n_targets exceeds the maximum. */
+#define HFMM_DIRECT_NOALLOC_MAX_TARGETS 65536
+int hfmm_direct_noalloc(const double* xyz, int64_t n, const double* q, double k, const int64_t* targets,
+                        int64_t n_targets, double* out, void* stream);
+
 /* Host-only octree and lists (no device use).  Replaces the reference's tree build
  * and list builders: build_fmm_tree_auto + assign_particles (jaxfmm/trees/cubic.py:530-695),
  * build_v_eager / build_ulist_eager / build_wx_eager (jaxfmm/trees/lists.py:818-931).

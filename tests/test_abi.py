@@ -44,3 +44,13 @@ def test_argument_errors_without_device():
     assert b"m2l_kernel" in lib.hfmm_last_error()
     assert lib.hfmm_batch_create(0, None, None, None, None, ctypes.byref(h)) == L.HFMM_ERR_ARG
     assert lib.hfmm_evaluate(None, None, 0, None, 0, None) == L.HFMM_ERR_ARG
+
+
+def test_noalloc_direct_rejects_sizes_beyond_its_scratch():
+    # the limit is checked before any device call (no GPU needed)
+    from hfmm0_amd import _lib as L
+    lib = L.lib()
+    dummy = ctypes.c_void_p(8)   # never dereferenced: the size check fails first
+    assert lib.hfmm_direct_noalloc(dummy, 10, dummy, 1.0, dummy, 65537, dummy, None) == L.HFMM_ERR_ARG
+    assert b"HFMM_DIRECT_NOALLOC_MAX_TARGETS" in lib.hfmm_last_error()
+    assert lib.hfmm_direct_noalloc(None, 10, dummy, 1.0, dummy, 5, dummy, None) == L.HFMM_ERR_ARG
