@@ -70,6 +70,7 @@ def _declare(lib):
         "hfmm_batch_plan": (I, [vp, I, ctypes.POINTER(vp)]),
         "hfmm_batch_evaluate": (I, [vp, vp, I, vp, I, vp]),
         "hfmm_batch_benchmark": (I, [vp, vp, I, vp, I, I, ctypes.POINTER(D)]),
+        "hfmm_batch_benchmark_phases": (I, [vp, vp, I, vp, I, vp]),
         "hfmm_batch_destroy": (I, [vp]),
         "hfmm_direct": (I, [vp, L, vp, D, vp, L, vp]),
         "hfmm_direct_noalloc": (I, [vp, L, vp, D, vp, L, vp, vp]),
@@ -105,7 +106,7 @@ EXPORTED = [
     "hfmm_tree_create", "hfmm_tree_create_device", "hfmm_tree_info_get", "hfmm_tree_export", "hfmm_tree_export_list",
     "hfmm_tree_destroy", "hfmm_host_jhat", "hfmm_host_hhat", "hfmm_host_rotation", "hfmm_host_coax",
     "hfmm_mlp_create", "hfmm_mlp_params", "hfmm_mlp_forward", "hfmm_mlp_gradient", "hfmm_mlp_train",
-    "hfmm_mlp_destroy", "hfmm_batch_create", "hfmm_batch_plan", "hfmm_batch_evaluate", "hfmm_batch_benchmark",
+    "hfmm_mlp_destroy", "hfmm_batch_create", "hfmm_batch_plan", "hfmm_batch_evaluate", "hfmm_batch_benchmark", "hfmm_batch_benchmark_phases",
     "hfmm_batch_destroy", "hfmm_last_error",
 ]
 

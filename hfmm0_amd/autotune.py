@@ -254,3 +254,48 @@ class PlannedEvaluator:
 def plan(positions, k, tol=1e-6, precision="mixed", wisdom=True, n_samp=512, verbose=False):
     return PlannedEvaluator(positions, k, tol=tol, precision=precision, wisdom=wisdom, n_samp=n_samp,
                             verbose=verbose)
+
+
+def batch_degrees(positions, k, p=12, depth_cap=10):
+    """Per-level degrees of a batch problem (index 0 = level 1): p at every level, raised to
+    the high-frequency floor (learned.degree_floor) where k times the box diameter needs
+    more.  Level 1 has no far field; it takes the level-2 degree."""
+    from .learned import degree_floor
+    x = np.asarray(positions)
+    extent = float((x.max(0) - x.min(0)).max())
+    d = [max(p, degree_floor(k, extent, lv)) for lv in range(2, depth_cap + 1)]
+    return [d[0]] + d
+
+
+def tune_batch(positions_list, k_list, charges_list, p=12, precision="mixed", leaves=(32, 64, 128), tiles=(2, 4),
+               subsample=24, reps=5, seed=0):
+    """Leaf size and near-field tile of a batch: the evaluate of a batch of `subsample`
+    problems drawn from the list (seeded) is timed for each leaf size at the default tile,
+    then for each tile at the fastest leaf size.  Returns (leaf, tile, timings)."""
+    import torch
+    from .batch import Batch
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(len(positions_list), min(subsample, len(positions_list)), replace=False))
+    xs = [positions_list[i] for i in idx]
+    ks = [k_list[i] for i in idx]
+    qd = [torch.from_numpy(np.asarray(charges_list[i])).cuda() for i in idx]
+    od = [torch.empty(len(x), dtype=torch.complex128, device="cuda") for x in xs]
+    pls = [batch_degrees(x, k, p) for x, k in zip(xs, ks)]
+
+    def timed(leaf, tile):
+        B = Batch(xs, ks, p=max(max(pl) for pl in pls), N_max=leaf, p_level=pls, precision=precision,
+                  p2p_tile=tile)
+        try:
+            return B.benchmark(qd, od, warmup=1, reps=reps)
+        finally:
+            B.close()
+
+    timings = {}
+    for leaf in leaves:
+        timings[(leaf, 0)] = timed(leaf, 0)
+    best_leaf = min(leaves, key=lambda lf: timings[(lf, 0)])
+    for tile in tiles:
+        timings[(best_leaf, tile)] = timed(best_leaf, tile)
+    best = min(((lf, t) for (lf, t) in timings if lf == best_leaf), key=lambda c: timings[c])
+    return best[0], best[1], {f"leaf{lf}_tile{t}": ms for (lf, t), ms in timings.items()}
+

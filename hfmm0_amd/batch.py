@@ -1,8 +1,9 @@
 """Batches of independent problems evaluated together (libhfmm ``hfmm_batch_*``).
 
-One plan per problem; an evaluate forks the problems over a few streams balanced by
-point count and joins them inside one captured hipGraph, so that many small problems
-share the GPU instead of running one after another.  The reference has no batch API:
+One plan per problem.  An evaluate is one captured hipGraph: the upward and downward
+passes of the problems forked over a few streams, and the matrix-core M2L and the near
+field of all problems as persistent launches whose resident workgroups take (problem,
+item) work from device queues, so that many small problems fill the GPU together.  The reference has no batch API:
 its tuner driver (``jaxfmm/tune.py:32-172``) loops over problems.
 """
 import ctypes
@@ -19,14 +20,15 @@ class Batch:
     a list of per-problem lists)."""
 
     def __init__(self, positions_list, k_list, p=4, N_max=16, max_depth=None, *, precision="fp64",
-                 p_level=None, m2l_kernel="auto"):
+                 p_level=None, m2l_kernel="auto", p2p_tile=0):
         if len(positions_list) != len(k_list) or not positions_list:
             raise ValueError("need one k per problem and at least one problem")
         xs, ks, opts = [], [], (L.Opts * len(positions_list))()
         for i, (x, k) in enumerate(zip(positions_list, k_list)):
             x, k = _check_problem(x, k)
             pl = p_level[i] if (p_level is not None and np.ndim(p_level[0]) > 0) else p_level
-            opts[i] = make_opts(p, N_max, max_depth, precision=precision, p_level=pl, m2l_kernel=m2l_kernel, _k=k)
+            opts[i] = make_opts(p, N_max, max_depth, precision=precision, p_level=pl, m2l_kernel=m2l_kernel,
+                                p2p_tile=p2p_tile, _k=k)
             xs.append(x)
             ks.append(float(k))
         self._xs = xs                      # keep the position arrays alive during create
@@ -96,6 +98,14 @@ class Batch:
         med = ctypes.c_double()
         L.check(L.lib().hfmm_batch_benchmark(self.handle, qp, qt, op, int(warmup), int(reps), ctypes.byref(med)))
         return med.value
+
+    def benchmark_phases(self, q_devs, out_devs, reps=5):
+        """Mean milliseconds of the parts of one batch evaluate run eagerly: upward passes,
+        persistent M2L, downward passes, persistent near field."""
+        qp, op, qt = self._dev_ptrs(q_devs, out_devs)
+        ph = (ctypes.c_double * 4)()
+        L.check(L.lib().hfmm_batch_benchmark_phases(self.handle, qp, qt, op, int(reps), ph))
+        return dict(zip(("up", "m2l", "down", "p2p"), list(ph)))
 
     def certify(self, charges_list, tol=1e-6, n_samp=256, seed=0, max_steps=4):
         """Evaluate and certify every problem on n_samp sampled targets against the GPU FP64

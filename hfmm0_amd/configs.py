@@ -5,7 +5,7 @@ same degrees.
 
     python -m hfmm0_amd.configs C3      # 2M sphere + clusters, k = 100
     python -m hfmm0_amd.configs C5      # 4M uniform + sphere + clusters, k = 60
-    python -m hfmm0_amd.configs C4      # 256 problems, N in [2k, 200k], k in [1, 50]: one batch
+    python -m hfmm0_amd.configs C4      # 256 ellipsoids, N in [2k, 200k], k in [1, 50]: one batch
 """
 import argparse
 import json
@@ -15,7 +15,7 @@ import numpy as np
 
 from . import api
 from .learned import DegreeMLP, predict_p_level
-from .workloads import c3_problem, c5_problem, mixed_problem, sample_targets
+from .workloads import c3_problem, c5_problem, sample_targets
 
 CONFIGS = {"C3": (c3_problem, 128, 10), "C5": (c5_problem, 128, None)}   # (problem, leaf, max_depth)
 
@@ -60,64 +60,57 @@ def run(name, n=None, refine=False, tol=1e-6, samples=4096, reps=10):
     return out
 
 
-def c4_problems(n_problems=256, n_range=(2_000, 200_000), k_range=(1.0, 50.0), seed=4000):
-    """The batched configuration: problem i has N log-uniform in n_range (default_rng(seed)
-    draws all N first) and the geometry, k and charges of mixed_problem(N, k_range,
-    seed + 1 + i)."""
-    rng = np.random.default_rng(seed)
-    ns = np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), n_problems)).astype(int)
-    return [mixed_problem(int(n), k_range, seed + 1 + i) for i, n in enumerate(ns)]
-
-
-def run_c4(n_problems=256, leaf=64, tol=1e-6, samples=256, reps=10, headroom=2):
-    """All problems in one batch (mixed precision, learned degrees raised to the
-    high-frequency floor).  The tables are built `headroom` degrees above the prediction;
-    Batch.certify evaluates at the predicted degrees, checks every problem on `samples`
-    targets against the FP64 direct sum and raises the degrees of those that miss `tol`.
-    The batch is then built again at the certified degrees and its evaluate timed against
-    the sum of the problems evaluated one by one."""
+def run_c4(n_problems=256, tol=1e-6, samples=512, reps=10, tune=True, headroom=4):
+    """All problems in one batch (workloads.batch_problems: ellipsoids, N log-uniform in
+    [2k, 200k], k in [1, 50]; complex64 charges; mixed precision).  The leaf size and the
+    near-field tile come from the batch autotuner (autotune.tune_batch, on 24 of the
+    problems).  Degrees: 12, raised to the high-frequency floor where k D needs more
+    (autotune.batch_degrees); tables `headroom` degrees higher, and Batch.certify raises
+    the degrees of every problem that misses `tol` on `samples` targets against the FP64
+    direct sum.  The batch is then built again at the certified degrees and its evaluate
+    timed against the problems evaluated one by one."""
     import torch
+    from .autotune import batch_degrees, tune_batch
     from .batch import Batch
+    from .learned import sampled_error
+    from .workloads import batch_problems
     t0 = time.perf_counter()
-    probs = c4_problems(n_problems)
+    probs = batch_problems(n_problems)
     gen_s = time.perf_counter() - t0
     xs, qs, ks = [x for x, _, _, _ in probs], [q for _, q, _, _ in probs], [k for _, _, k, _ in probs]
     t0 = time.perf_counter()
-    net = DegreeMLP.load()
-    pls = [predict_p_level(net, x, q, k, n_max=leaf, margin=0.0) for x, q, k in zip(xs, qs, ks)]
-    net.close()
-    predict_s = time.perf_counter() - t0
+    leaf, tile, tune_ms = tune_batch(xs, ks, qs) if tune else (128, 0, {})
+    tune_s = time.perf_counter() - t0
+    pls = [batch_degrees(x, k) for x, k in zip(xs, ks)]
     t0 = time.perf_counter()
     tabs = [[min(40, v + headroom) for v in pl] for pl in pls]
-    batch = Batch(xs, ks, p=max(max(t) for t in tabs), N_max=leaf, p_level=tabs, precision="mixed")
+    batch = Batch(xs, ks, p=max(max(t) for t in tabs), N_max=leaf, p_level=tabs, precision="mixed", p2p_tile=tile)
     for i, pl in enumerate(pls):
         st = batch.plan(i)
-        st.set_degrees(list(pl)[:st.depth])
-    _, certified_pl, errs = batch.certify(qs, tol=tol, n_samp=samples)
+        st.set_degrees(pl[:st.depth])
+    _, cert, errs0 = batch.certify(qs, tol=tol, n_samp=samples, max_steps=6)
     certify_s = time.perf_counter() - t0
-    raised = sum(list(c) != list(p)[:len(c)] for c, p in zip(certified_pl, pls))
+    raised = sum(list(c) != list(p)[:len(c)] for c, p in zip(cert, pls))
     batch.close()
     t0 = time.perf_counter()
-    batch = Batch(xs, ks, p=max(max(pl) for pl in certified_pl), N_max=leaf, p_level=certified_pl,
-                  precision="mixed")
+    batch = Batch(xs, ks, p=max(max(c) for c in cert), N_max=leaf, p_level=cert, precision="mixed", p2p_tile=tile)
     setup_s = time.perf_counter() - t0
     q_dev = [torch.from_numpy(q).cuda() for q in qs]
     o_dev = [torch.empty(len(q), dtype=torch.complex128, device="cuda") for q in qs]
     ms = batch.benchmark(q_dev, o_dev, warmup=2, reps=reps)
+    phases = batch.benchmark_phases(q_dev, o_dev, reps=3)
     torch.cuda.synchronize()
     phi = [o.cpu().numpy() for o in o_dev]
-    from .learned import sampled_error
-    errs2 = [sampled_error(x, q, k, f, samples, 0) for x, q, k, f in zip(xs, qs, ks, phi)]
+    errs = [sampled_error(x, q, k, f, samples, 0) for x, q, k, f in zip(xs, qs, ks, phi)]
     seq = sum(api.benchmark(batch.plan(i), q_dev[i], o_dev[i], warmup=1, reps=3) for i in range(len(probs)))
     n_total = sum(len(q) for q in qs)
-    kinds = [kind for _, _, _, kind in probs]
-    out = {"config": "C4", "problems": len(probs), "points": n_total, "leaf": leaf,
+    out = {"config": "C4", "problems": len(probs), "points": n_total, "leaf": leaf, "p2p_tile": tile,
            "n_min": min(len(q) for q in qs), "n_max": max(len(q) for q in qs),
-           "k_min": min(ks), "k_max": max(ks), "kinds": {kd: kinds.count(kd) for kd in sorted(set(kinds))},
-           "batch_ms": ms, "evals_per_s": n_total / (ms * 1e-3), "sequential_ms": seq, "batch_speedup": seq / ms,
-           "raised": raised, "max_rel_err": max(errs2), "median_rel_err": float(np.median(errs2)),
-           "certified": sum(e <= tol for e in errs2), "certify_max_err": max(errs), "gen_s": gen_s,
-           "predict_s": predict_s, "certify_s": certify_s, "setup_s": setup_s}
+           "k_min": min(ks), "k_max": max(ks), "degree_max": max(max(c) for c in cert), "raised": raised,
+           "batch_ms": ms, "evals_per_s": n_total / (ms * 1e-3), "phases_ms": phases, "sequential_ms": seq,
+           "batch_speedup": seq / ms, "max_rel_err": max(errs), "median_rel_err": float(np.median(errs)),
+           "within_tol": sum(e <= tol for e in errs), "tol": tol, "certify_max_err": max(errs0), "gen_s": gen_s,
+           "tune_s": tune_s, "tune_ms": tune_ms, "certify_s": certify_s, "setup_s": setup_s}
     batch.close()
     return out
 

@@ -129,6 +129,32 @@ void launch_direct(const double* X, const double* Y, const double* Z, const doub
 // n^2 .. n^2+n-1 (m = n down to 1).
 constexpr int kM2LBlockMaxP = 23;  // largest degree handled by the block kernel
 constexpr int kXcds = 8;            // accelerator dies; workgroups are dealt to them round-robin
+
+// Work queues of a persistent launch (batches of problems): work items [start[x],
+// start[x+1]) form queue x, one per XCD, each in spatial order; count[16 x] is the next
+// index of queue x (one 64-byte line per queue), cleared before every launch.
+struct WorkQueues {
+    const int2* work;             // (descriptor, item) pairs
+    const int* start;             // kXcds + 1 entries
+    int* count;                   // 16 kXcds ints
+};
+#ifdef __HIP__
+// The next work item for a workgroup on XCD `home`: its own queue first, then the others'
+// (so that the last items are shared out); -1 when every queue is empty.  Indices only
+// grow, so every caller returns after at most one pass over the queues.
+__device__ inline int work_dequeue(const WorkQueues& q, int home) {
+    for (int t = 0; t < kXcds; ++t) {
+        const int x = (home + t) % kXcds;
+        const int n = q.start[x + 1] - q.start[x];
+        int* c = q.count + 16 * x;
+        if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n) continue;
+        const int i = atomicAdd(c, 1);
+        if (i < n) return q.start[x] + i;
+    }
+    return -1;
+}
+#endif
+void launch_clear_ints(int* p, int n, hipStream_t s);
 constexpr int kM2LMaxAccRegs = 128; // VGPRs per lane for the stage-3 accumulators (block-size choice)
 __host__ __device__ constexpr int m2l_tiles(int len) { return (len + 15) / 16; }
 // stage-3 units per wave the M2L block kernel is instantiated for at table degree p: the
@@ -191,6 +217,23 @@ struct M2LBlockArgs {
 size_t m2l_block_lds_bytes(int p, int tb, int g, int precision_bytes);
 void prepare_m2l_block(int p, int tb, int g, int fp32);
 void launch_m2l_block(const M2LBlockArgs& a, int tb, int g, int fp32, hipStream_t s);
+// Persistent form over many translations (batches): the resident grid of the instance for
+// (p = larger table degree, tb, g, fp32) with `shm` bytes of LDS, and the launch over the
+// device descriptors d_args and the queues q of (descriptor, item) work.
+int m2l_block_persistent_grid(int p, int tb, int g, int fp32, size_t shm);
+// near field of one problem in a persistent batch launch
+constexpr int kP2PBatchTargetsPerLane = 2;
+struct P2PBatchArgs {
+    P2PArgs a;                    // scale and wscale set (p2p_scaled)
+};
+// leaf classes of the persistent near field: FP64 plans; mixed plans with k r <= kP2PLowKr;
+// mixed, FP32 phase; mixed, FP64 phase
+constexpr int kP2PClassFp64 = 0, kP2PClassLowK = 1, kP2PClassPhase32 = 2, kP2PClassPhase64 = 3;
+P2PArgs p2p_scaled(P2PArgs a, int precision, bool lowk);
+int p2p_persistent_grid(int cls);
+void launch_p2p_persistent(int cls, const P2PBatchArgs* d_args, const WorkQueues& q, int grid, hipStream_t s);
+void launch_m2l_block_persistent(const M2LBlockArgs* d_args, const WorkQueues& q, int grid, int p, int tb, int g,
+                                 int fp32, size_t shm, hipStream_t s);
 void launch_to_float(const double2* in, float2* out, int64_t n, hipStream_t s);
 // zero the first ncoef coefficients of the expansions of the given boxes
 void launch_zero_boxes(const int* boxes, int n, const int64_t* exp_off, int ncoef, double2* E, hipStream_t s);

@@ -631,11 +631,21 @@ __device__ inline int p2p_chunk(int remaining) {
 // 1/r + i k, exact to FP32 rounding.  Pairs with r^2 at or below the leaf's coincidence
 // floor contribute nothing.  The sums times a.wscale (k / pi; 1 for LOWK) are added to the
 // far field and written in input order.  The workgroup handles leaf a.leaf0 + blockIdx.x.
+// LDS of the near-field kernels: 64 staged sources
+union P2PStage {
+    struct {
+        double2 xy[64];
+        double2 zq[64];   // (z, charge as two FP32 bit-cast into one double)
+    } mixed;
+    struct {
+        double x[64], y[64], z[64], qr[64], qi[64];
+    } fp64;
+};
+
 template <bool PH, bool LOWK, int R>
-__global__ __launch_bounds__(64) void p2p_mixed_kernel(P2PArgs a) {
-    __shared__ double2 sxy[64];
-    __shared__ double2 szq[64];     // (z, charge as two FP32 bit-cast into one double)
-    const int leaf = a.leaf0 + blockIdx.x;
+__device__ __forceinline__ void p2p_mixed_body(const P2PArgs& a, const int leaf, P2PStage& st) {
+    double2* sxy = st.mixed.xy;
+    double2* szq = st.mixed.zq;
     const int b = a.leaf_box[leaf];
     const int tstart = a.bx.pstart[b], tcnt = a.bx.count[b];
     const double sc = a.scale;
@@ -732,6 +742,12 @@ __global__ __launch_bounds__(64) void p2p_mixed_kernel(P2PArgs a) {
     }
 }
 
+template <bool PH, bool LOWK, int R>
+__global__ __launch_bounds__(64) void p2p_mixed_kernel(P2PArgs a) {
+    __shared__ P2PStage st;
+    p2p_mixed_body<PH, LOWK, R>(a, a.leaf0 + blockIdx.x, st);
+}
+
 // FP64 near field (the plain form): one wave per target leaf; targets of a chunk of up to
 // 64 points are spread over lane groups: with T' = next power of two >= chunk size, lane
 // l holds target l % T' and scans the sources j == l / T' (mod 64/T') of each 64-point
@@ -739,9 +755,8 @@ __global__ __launch_bounds__(64) void p2p_mixed_kernel(P2PArgs a) {
 // partial sums are combined with wavefront shuffles.  Sources of a leaf are a list of
 // point ranges (adjacent leaves, and the W/X interactions evaluated directly); pairs at or
 // below the leaf's coincidence floor are skipped.
-__global__ __launch_bounds__(64) void p2p_fp64_kernel(P2PArgs a) {
-    __shared__ double sx[64], sy[64], sz[64], sqr[64], sqi[64];
-    const int leaf = a.leaf0 + blockIdx.x;
+__device__ __forceinline__ void p2p_fp64_body(const P2PArgs& a, const int leaf, P2PStage& st) {
+    double *sx = st.fp64.x, *sy = st.fp64.y, *sz = st.fp64.z, *sqr = st.fp64.qr, *sqi = st.fp64.qi;
     const int b = a.leaf_box[leaf];
     const int tstart = a.bx.pstart[b], tcnt = a.bx.count[b];
     const double cx = a.bx.centre[3 * b], cy = a.bx.centre[3 * b + 1], cz = a.bx.centre[3 * b + 2];
@@ -800,6 +815,38 @@ __global__ __launch_bounds__(64) void p2p_fp64_kernel(P2PArgs a) {
             if (a.out_mode & 2) f.y = 0.0;    // k == 0 with real charges: the result is real
             a.out[a.perm[it]] = f;
         }
+    }
+}
+
+__global__ __launch_bounds__(64) void p2p_fp64_kernel(P2PArgs a) {
+    __shared__ P2PStage st;
+    p2p_fp64_body(a, a.leaf0 + blockIdx.x, st);
+}
+
+// Persistent near field over the leaves of many problems (batches), one launch per leaf
+// class CLS (kP2PClass*): one wave per workgroup takes (problem, leaf) work from the
+// queues q until all are empty; args[w.x] is the problem's near-field description.
+template <int CLS>
+__global__ __launch_bounds__(64) void p2p_persistent_kernel(const P2PBatchArgs* __restrict__ args, WorkQueues q) {
+    __shared__ P2PStage st;
+    const int home = blockIdx.x % kXcds;
+    for (;;) {
+        int w = 0;
+        if (threadIdx.x == 0) w = work_dequeue(q, home);
+        w = __builtin_amdgcn_readfirstlane(w);   // uniform: the descriptor is read with scalar loads
+        if (w < 0) break;
+        const int2 wk = q.work[w];
+        const int prob = __builtin_amdgcn_readfirstlane(wk.x), leaf = __builtin_amdgcn_readfirstlane(wk.y);
+        const P2PArgs a = args[prob].a;
+        if constexpr (CLS == kP2PClassFp64)
+            p2p_fp64_body(a, leaf, st);
+        else if constexpr (CLS == kP2PClassLowK)
+            p2p_mixed_body<false, true, kP2PBatchTargetsPerLane>(a, leaf, st);
+        else if constexpr (CLS == kP2PClassPhase32)
+            p2p_mixed_body<false, false, kP2PBatchTargetsPerLane>(a, leaf, st);
+        else
+            p2p_mixed_body<true, false, kP2PBatchTargetsPerLane>(a, leaf, st);
+        __syncthreads();
     }
 }
 
@@ -1084,17 +1131,73 @@ void launch_xlate(const XlateArgs& a, int n_targets, hipStream_t s) {
     check_launch("xlate");
 }
 
+P2PArgs p2p_scaled(P2PArgs a, int precision, bool lowk) {
+    if (precision != 0) {
+        // coordinates in units of pi / k: the weights are e^{i pi u}/u with u = k r / pi
+        a.scale = lowk ? 1.0 : a.k / M_PI;
+        a.wscale = lowk ? 1.0 : a.k / M_PI;
+    }
+    return a;
+}
+
+template <int CLS>
+static const void* p2p_persistent_fn() {
+    return (const void*)p2p_persistent_kernel<CLS>;
+}
+
+static const void* p2p_persistent_class_fn(int cls) {
+    switch (cls) {
+        case kP2PClassFp64: return p2p_persistent_fn<kP2PClassFp64>();
+        case kP2PClassLowK: return p2p_persistent_fn<kP2PClassLowK>();
+        case kP2PClassPhase32: return p2p_persistent_fn<kP2PClassPhase32>();
+        default: return p2p_persistent_fn<kP2PClassPhase64>();
+    }
+}
+
+int p2p_persistent_grid(int cls) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p2p_persistent_class_fn(cls), 64, 0))
+        fail(-3, "occupancy query (p2p persistent) failed");
+    return std::max(kXcds, (std::max(1, per_cu) * cus) / kXcds * kXcds);
+}
+
+void launch_p2p_persistent(int cls, const P2PBatchArgs* d_args, const WorkQueues& q, int grid, hipStream_t s) {
+    if (grid <= 0) return;
+    switch (cls) {
+        case kP2PClassFp64:
+            hipLaunchKernelGGL(p2p_persistent_kernel<kP2PClassFp64>, dim3(grid), dim3(64), 0, s, d_args, q);
+            break;
+        case kP2PClassLowK:
+            hipLaunchKernelGGL(p2p_persistent_kernel<kP2PClassLowK>, dim3(grid), dim3(64), 0, s, d_args, q);
+            break;
+        case kP2PClassPhase32:
+            hipLaunchKernelGGL(p2p_persistent_kernel<kP2PClassPhase32>, dim3(grid), dim3(64), 0, s, d_args, q);
+            break;
+        default:
+            hipLaunchKernelGGL(p2p_persistent_kernel<kP2PClassPhase64>, dim3(grid), dim3(64), 0, s, d_args, q);
+    }
+    check_launch("p2p persistent");
+}
+
+__global__ void clear_ints_kernel(int* p, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+void launch_clear_ints(int* p, int n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(clear_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+    check_launch("clear");
+}
+
 template <int R>
 static void launch_p2p_mixed(P2PArgs a, int n_leaves, int n_fp32, bool lowk, hipStream_t s) {
+    a = p2p_scaled(a, 1, lowk);
     if (lowk) {
-        a.scale = 1.0;
-        a.wscale = 1.0;
         hipLaunchKernelGGL((p2p_mixed_kernel<false, true, R>), dim3(n_leaves), dim3(64), 0, s, a);
         return;
     }
-    // coordinates in units of pi / k: the weights are e^{i pi u}/u with u = k r / pi
-    a.scale = a.k / M_PI;
-    a.wscale = a.k / M_PI;
     if (n_fp32 > 0) hipLaunchKernelGGL((p2p_mixed_kernel<false, false, R>), dim3(n_fp32), dim3(64), 0, s, a);
     if (n_leaves > n_fp32) {
         a.leaf0 = n_fp32;

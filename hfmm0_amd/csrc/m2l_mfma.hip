@@ -105,9 +105,9 @@ constexpr int m2l_block_min_waves() {
     return sizeof(T) == 4 && TB == 16 && G == 2 && KT == 1 ? 4 : 1;
 }
 
+// One work item (block, group range) of the translation described by a.
 template <typename T, int TB, int G, int KT, int MAXU>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(m2l_block_min_waves<T, TB, G, KT>())))
-void m2l_block_kernel(M2LBlockArgs a) {
+__device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4 item) {
     using V4 = typename Mma<T>::v4;
     using C2 = typename Mma<T>::c2;
     constexpr int kMaxTiles = KT;           // row tiles of a rotation block or an order block
@@ -140,10 +140,6 @@ void m2l_block_kernel(M2LBlockArgs a) {
     // this lane's phase entries of stage 3: add 16 mt for a unit's row tile mt and
     // g TL + r RSTEP for direction g and output register r
     const C2* ps_lane = Ps + (li & 1) * G * TL + Mma<T>::row(lk, 0);
-    // workgroup i runs on XCD i % 8: give XCD x the contiguous item range [x n/8, (x+1) n/8)
-    // (the host pads the item list to a multiple of 8 and orders it in space)
-    const int per_xcd = a.n_items / kXcds;
-    const int4 item = a.items[(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds];   // (block, g0, g1, atomic)
     if (item.y >= item.z) return;   // padding item
     const int* tg = a.blk_targets + (size_t)item.x * TB;
     const C2* Mexp = (const C2*)a.M;
@@ -704,6 +700,45 @@ void m2l_block_kernel(M2LBlockArgs a) {
     }
 }
 
+// One launch per translation: workgroup i runs on XCD i % 8, and XCD x takes the contiguous
+// item range [x n/8, (x+1) n/8) (the host pads the item list to a multiple of 8 and orders
+// it in space, so each XCD's L2 keeps one region's multipoles).
+template <typename T, int TB, int G, int KT, int MAXU>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(m2l_block_min_waves<T, TB, G, KT>())))
+void m2l_block_kernel(M2LBlockArgs a) {
+    const int per_xcd = a.n_items / kXcds;
+    m2l_block_body<T, TB, G, KT, MAXU>(a, a.items[(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds]);
+}
+
+// Persistent form for batches: resident workgroups take (translation, item) work from the
+// queues q (one per XCD, spatially ordered, stealing from the others when their own runs
+// dry) until all are empty; args[w.x] describes translation w.x.
+// The loop keeps a few lane values live across items; FP32 instances with one tile per
+// block are held to the 4 waves per SIMD of the one-launch form.
+template <typename T, int TB, int G, int KT>
+constexpr int m2l_persistent_min_waves() {
+    return sizeof(T) == 4 && KT == 1 && TB * G <= 32 ? 4 : m2l_block_min_waves<T, TB, G, KT>();
+}
+
+template <typename T, int TB, int G, int KT, int MAXU>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(m2l_persistent_min_waves<T, TB, G, KT>())))
+void m2l_block_persistent(const M2LBlockArgs* __restrict__ args, WorkQueues q) {
+    __shared__ int s_w;
+    const int home = blockIdx.x % kXcds;
+    for (;;) {
+        if (threadIdx.x == 0) s_w = work_dequeue(q, home);
+        __syncthreads();
+        // uniform across the workgroup: the descriptor is read with scalar loads
+        const int w = __builtin_amdgcn_readfirstlane(s_w);
+        __syncthreads();
+        if (w < 0) break;
+        const int2 wk = q.work[w];
+        const M2LBlockArgs a = args[__builtin_amdgcn_readfirstlane(wk.x)];
+        m2l_block_body<T, TB, G, KT, MAXU>(a, a.items[__builtin_amdgcn_readfirstlane(wk.y)]);
+        __syncthreads();
+    }
+}
+
 __global__ void to_float_kernel(const double2* in, float2* out, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -715,8 +750,34 @@ __global__ void to_c64_kernel(const double2* in, float2* out, int64_t n) {
     if (i < n) out[i] = make_float2((float)in[i].x, (float)in[i].y);
 }
 
+// a launch of the persistent form: device args, queues and the resident grid
+struct PersistentLaunch {
+    const M2LBlockArgs* d_args;
+    WorkQueues q;
+    int grid;            // 0: return the occupancy-limited resident grid in *grid_out
+    int* grid_out;
+};
+
 template <typename T, int TB, int G, int KT, int MAXU>
-void launch_one(const M2LBlockArgs* a, size_t shm, hipStream_t s) {
+void launch_one(const M2LBlockArgs* a, size_t shm, hipStream_t s, const PersistentLaunch* pl = nullptr) {
+    if (pl) {
+        const void* fn = (const void*)m2l_block_persistent<T, TB, G, KT, MAXU>;
+        if (pl->grid == 0) {
+            int dev = 0, cus = 0, per_cu = 0;
+            (void)hipGetDevice(&dev);
+            // the launch's own dynamic LDS (the kernel also holds a few static bytes)
+            if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) != hipSuccess)
+                fail(-3, "hipFuncSetAttribute(m2l persistent) failed");
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, shm) != hipSuccess)
+                fail(-3, "occupancy query (m2l persistent) failed");
+            *pl->grid_out = std::max(kXcds, (std::max(1, per_cu) * cus) / kXcds * kXcds);
+            return;
+        }
+        hipLaunchKernelGGL((m2l_block_persistent<T, TB, G, KT, MAXU>), dim3(pl->grid), dim3(256), shm, s,
+                           pl->d_args, pl->q);
+        return;
+    }
     if (!a) {   // prepare: allow the dynamic LDS this instantiation may use
         int dev = 0, lds = 0;
         (void)hipGetDevice(&dev);
@@ -731,49 +792,50 @@ void launch_one(const M2LBlockArgs* a, size_t shm, hipStream_t s) {
 
 // (KT, MAXU) follow from the table degree: KT = m2l_tiles(p + 1), MAXU = m2l_block_maxu(p)
 template <typename T, int TB, int G>
-void launch_p(const M2LBlockArgs* a, size_t shm, int p, hipStream_t s) {
+void launch_p(const M2LBlockArgs* a, size_t shm, int p, hipStream_t s, const PersistentLaunch* pl) {
     if (p <= 7)
-        launch_one<T, TB, G, 1, m2l_block_maxu(7)>(a, shm, s);
+        launch_one<T, TB, G, 1, m2l_block_maxu(7)>(a, shm, s, pl);
     else if (p <= 11)
-        launch_one<T, TB, G, 1, m2l_block_maxu(11)>(a, shm, s);
+        launch_one<T, TB, G, 1, m2l_block_maxu(11)>(a, shm, s, pl);
     else if (p <= 15)
-        launch_one<T, TB, G, 1, m2l_block_maxu(15)>(a, shm, s);
+        launch_one<T, TB, G, 1, m2l_block_maxu(15)>(a, shm, s, pl);
     else if (p <= 17)
-        launch_one<T, TB, G, 2, m2l_block_maxu(17)>(a, shm, s);
+        launch_one<T, TB, G, 2, m2l_block_maxu(17)>(a, shm, s, pl);
     else if (p <= 19)
-        launch_one<T, TB, G, 2, m2l_block_maxu(19)>(a, shm, s);
+        launch_one<T, TB, G, 2, m2l_block_maxu(19)>(a, shm, s, pl);
     else if (p <= kM2LBlockMaxP)
-        launch_one<T, TB, G, 2, m2l_block_maxu(23)>(a, shm, s);
+        launch_one<T, TB, G, 2, m2l_block_maxu(23)>(a, shm, s, pl);
     else
         fail(-9, "internal: degree above the block kernel's range");
 }
 
 template <typename T, int TB>
-void launch_g(const M2LBlockArgs* a, size_t shm, int g, int p, hipStream_t s) {
+void launch_g(const M2LBlockArgs* a, size_t shm, int g, int p, hipStream_t s, const PersistentLaunch* pl) {
     if (g == 1)
-        launch_p<T, TB, 1>(a, shm, p, s);
+        launch_p<T, TB, 1>(a, shm, p, s, pl);
     else if (g == 2)
-        launch_p<T, TB, 2>(a, shm, p, s);
+        launch_p<T, TB, 2>(a, shm, p, s, pl);
     else if (g == 4)
-        launch_p<T, TB, 4>(a, shm, p, s);
+        launch_p<T, TB, 4>(a, shm, p, s, pl);
     else
         fail(-9, "internal: unsupported M2L group size");
 }
 
-void dispatch(const M2LBlockArgs* a, int p, int tb, int g, int fp32, hipStream_t s) {
-    const size_t shm = m2l_block_lds_bytes(p, tb, g, fp32 ? 4 : 8);
+void dispatch(const M2LBlockArgs* a, int p, int tb, int g, int fp32, hipStream_t s,
+              const PersistentLaunch* pl = nullptr, size_t shm_min = 0) {
+    const size_t shm = std::max(shm_min, m2l_block_lds_bytes(p, tb, g, fp32 ? 4 : 8));
     if (fp32) {
         if (tb == 16)
-            launch_g<float, 16>(a, shm, g, p, s);
+            launch_g<float, 16>(a, shm, g, p, s, pl);
         else if (tb == 8)
-            launch_g<float, 8>(a, shm, g, p, s);
+            launch_g<float, 8>(a, shm, g, p, s, pl);
         else
             fail(-9, "internal: unsupported M2L block size");
     } else {
         if (tb == 8)
-            launch_g<double, 8>(a, shm, g, p, s);
+            launch_g<double, 8>(a, shm, g, p, s, pl);
         else if (tb == 16 && g == 1)
-            launch_p<double, 16, 1>(a, shm, p, s);
+            launch_p<double, 16, 1>(a, shm, p, s, pl);
         else
             fail(-9, "internal: unsupported M2L block size");
     }
@@ -788,6 +850,22 @@ void launch_m2l_block(const M2LBlockArgs& a, int tb, int g, int fp32, hipStream_
     dispatch(&a, std::max(a.p_tab, a.pt_tab), tb, g, fp32, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) fail(-3, std::string("kernel launch failed (m2l_block): ") + hipGetErrorString(e));
+}
+
+int m2l_block_persistent_grid(int p, int tb, int g, int fp32, size_t shm) {
+    int grid = 0;
+    const PersistentLaunch pl{nullptr, WorkQueues{}, 0, &grid};
+    dispatch(nullptr, p, tb, g, fp32, nullptr, &pl, shm);
+    return grid;
+}
+
+void launch_m2l_block_persistent(const M2LBlockArgs* d_args, const WorkQueues& q, int grid, int p, int tb, int g,
+                                 int fp32, size_t shm, hipStream_t s) {
+    if (grid <= 0) return;
+    const PersistentLaunch pl{d_args, q, grid, nullptr};
+    dispatch(nullptr, p, tb, g, fp32, s, &pl, shm);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) fail(-3, std::string("kernel launch failed (m2l_block persistent): ") + hipGetErrorString(e));
 }
 
 void launch_to_float(const double2* in, float2* out, int64_t n, hipStream_t s) {

@@ -1003,38 +1003,79 @@ M2LBlockArgs block_args(const hfmm_plan& P, const M2LBlockLevel& B, bool fp32, c
     return a;
 }
 
-// The device sequence of one evaluate.  ev (optional, 8 entries) brackets the phases;
-// lev (optional, 2 per level) brackets the M2L launch of every level.
+// parts of one evaluate's device sequence (a batch runs them for all problems, with the
+// block M2L and the near field as persistent launches over every problem)
+constexpr int kSeqUp = 1;          // charges, clears, P2M, M2M
+constexpr int kSeqM2LGeneric = 2;  // M2L levels on the generic kernel
+constexpr int kSeqM2LBlock = 4;    // M2L levels on the matrix-core block kernel
+constexpr int kSeqDown = 8;        // L2L, L2P
+constexpr int kSeqP2P = 16;        // near field and output
+constexpr int kSeqAll = 31;
+
+int out_mode_of(const hfmm_plan& P, int q_type) {
+    return (P.conj ? 1 : 0) | ((P.k == 0.0 && real_charges(q_type)) ? 2 : 0);
+}
+
+P2PArgs p2p_args(const hfmm_plan& P, int out_mode, double2* out) {
+    P2PArgs pa{};
+    pa.leaf_box = P.d_leaf_box;
+    pa.bx = P.boxes();
+    pa.rng_ptr = P.d_rng_ptr;
+    pa.rng_start = P.d_rng_start;
+    pa.rng_pref = P.d_rng_pref;
+    pa.leaf_src_total = P.d_leaf_src_total;
+    pa.floor2 = P.d_floor2;
+    pa.X = P.dX;
+    pa.Y = P.dY;
+    pa.Z = P.dZ;
+    pa.q = P.d_q;
+    pa.k = P.k;
+    pa.far = P.d_far;
+    pa.perm = P.d_perm;
+    pa.out_mode = out_mode;
+    pa.out = out;
+    return pa;
+}
+
+// The device sequence of one evaluate (the parts selected).  ev (optional, 8 entries)
+// brackets the phases; lev (optional, 2 per level) brackets the M2L launch of every level.
 // field (optional): E = -grad phi, 3 complex values per point in input order.
 void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStream_t s, hipEvent_t* ev,
-                  hipEvent_t* lev = nullptr, double2* field = nullptr) {
+                  hipEvent_t* lev = nullptr, double2* field = nullptr, int parts = kSeqAll) {
     auto mark = [&](int i) {
         if (ev) HIPCHK(hipEventRecord(ev[i], s));
     };
-    mark(0);
-    launch_gather_charges(q, q_type, P.d_perm, P.n, P.conj, P.d_q, s);
-    launch_fill_zero(P.dL, P.n_coef, s);
-    launch_fill_zero(P.d_far, P.n, s);
-    mark(1);
     const DevBoxes bx = P.boxes();
     const DevLevels lv = P.levels();
-    launch_p2m(P.d_far_leaves, P.n_far_leaves, P.p2m_chunk, bx, lv, P.dX, P.dY, P.dZ, P.d_q, P.k, P.p_leafmax, P.dM, s);
-    mark(2);
-    for (int L = P.depth - 1; L >= 2; --L) {
-        if (P.m2mb[L].use) {
-            const M2LBlockLevel& B = P.m2mb[L];
-            if (B.split) launch_zero_boxes(B.zero, B.n_zero, P.d_expoff, ncoef(P.p_eff[L]), P.dM, s);
-            launch_m2l_block(block_args(P, B, false, P.dM, P.dM, B.split ? 0 : 1), B.tb, B.grp, 0, s);
+    if (parts & kSeqUp) {
+        mark(0);
+        launch_gather_charges(q, q_type, P.d_perm, P.n, P.conj, P.d_q, s);
+        launch_fill_zero(P.dL, P.n_coef, s);
+        launch_fill_zero(P.d_far, P.n, s);
+        mark(1);
+        launch_p2m(P.d_far_leaves, P.n_far_leaves, P.p2m_chunk, bx, lv, P.dX, P.dY, P.dZ, P.d_q, P.k, P.p_leafmax,
+                   P.dM, s);
+        mark(2);
+        for (int L = P.depth - 1; L >= 2; --L) {
+            if (P.m2mb[L].use) {
+                const M2LBlockLevel& B = P.m2mb[L];
+                if (B.split) launch_zero_boxes(B.zero, B.n_zero, P.d_expoff, ncoef(P.p_eff[L]), P.dM, s);
+                launch_m2l_block(block_args(P, B, false, P.dM, P.dM, B.split ? 0 : 1), B.tb, B.grp, 0, s);
+            }
+            else if (P.m2m[L].n_targets)
+                launch_xlate(xargs(P, P.m2m[L], P.dM, P.dM, 0), P.m2m[L].n_targets, s);
         }
-        else if (P.m2m[L].n_targets)
-            launch_xlate(xargs(P, P.m2m[L], P.dM, P.dM, 0), P.m2m[L].n_targets, s);
+        mark(3);
+        // multipoles in FP32 for the FP32 block levels
+        bool any_fp32 = false;
+        for (int L = 2; L <= P.depth; ++L) any_fp32 |= P.m2l[L].n_targets && P.m2lb[L].use && P.m2lb[L].fp32;
+        if (any_fp32) launch_to_float(P.dM, P.dM32, P.n_coef, s);
     }
-    mark(3);
-    bool converted = false;
     for (int L = 2; L <= P.depth; ++L) {
         if (!P.m2l[L].n_targets) continue;
         const M2LBlockLevel& B = P.m2lb[L];
         if (!B.use) {
+            if (!(parts & kSeqM2LGeneric)) continue;
             if (lev) HIPCHK(hipEventRecord(lev[2 * L], s));
             // coarse levels with few targets (degrees above the matrix-core kernel's): the
             // source lists split over up to 16 workgroups per target, so that about 2048
@@ -1045,56 +1086,41 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
             if (lev) HIPCHK(hipEventRecord(lev[2 * L + 1], s));
             continue;
         }
-        if (B.fp32 && !converted) {
-            launch_to_float(P.dM, P.dM32, P.n_coef, s);
-            converted = true;
-        }
+        if (!(parts & kSeqM2LBlock)) continue;
         const M2LBlockArgs a =
             block_args(P, B, B.fp32, B.fp32 ? (const void*)P.dM32 : (const void*)P.dM, P.dL, 0);
         if (lev) HIPCHK(hipEventRecord(lev[2 * L], s));
         launch_m2l_block(a, B.tb, B.grp, B.fp32, s);
         if (lev) HIPCHK(hipEventRecord(lev[2 * L + 1], s));
     }
-    mark(4);
-    for (int L = 2; L < P.depth; ++L) {
-        if (P.l2lb[L].use)
-            launch_m2l_block(block_args(P, P.l2lb[L], false, P.dL, P.dL, 0), P.l2lb[L].tb, P.l2lb[L].grp, 0, s);
-        else if (P.l2l[L].n_targets)
-            launch_xlate(xargs(P, P.l2l[L], P.dL, P.dL, 1), P.l2l[L].n_targets, s);
-    }
-    mark(5);
-    const int out_mode = (P.conj ? 1 : 0) | ((P.k == 0.0 && real_charges(q_type)) ? 2 : 0);
-    if (field) {
-        launch_fill_zero(P.d_far_e, 3 * P.n, s);
-        launch_l2p_field(P.d_far_leaves, P.n_far_leaves, bx, lv, P.dX, P.dY, P.dZ, P.dL, P.k, P.p_leafmax, P.d_far,
-                         P.d_far_e, s);
+    if (parts & (kSeqM2LGeneric | kSeqM2LBlock)) mark(4);
+    const int out_mode = out_mode_of(P, q_type);
+    if (parts & kSeqDown) {
+        for (int L = 2; L < P.depth; ++L) {
+            if (P.l2lb[L].use)
+                launch_m2l_block(block_args(P, P.l2lb[L], false, P.dL, P.dL, 0), P.l2lb[L].tb, P.l2lb[L].grp, 0, s);
+            else if (P.l2l[L].n_targets)
+                launch_xlate(xargs(P, P.l2l[L], P.dL, P.dL, 1), P.l2l[L].n_targets, s);
+        }
+        mark(5);
+        if (field) {
+            launch_fill_zero(P.d_far_e, 3 * P.n, s);
+            launch_l2p_field(P.d_far_leaves, P.n_far_leaves, bx, lv, P.dX, P.dY, P.dZ, P.dL, P.k, P.p_leafmax,
+                             P.d_far, P.d_far_e, s);
+        } else {
+            launch_l2p(P.d_far_leaves, P.n_far_leaves, bx, lv, P.dX, P.dY, P.dZ, P.dL, P.k, P.pmax, P.d_far, s);
+        }
         mark(6);
-        launch_p2p_field(P.precision, P.n_leaves, P.n_p2p_fp32, P.d_leaf_box, bx, P.d_rng_ptr, P.d_rng_start, P.d_rng_pref,
-                         P.d_leaf_src_total, P.d_floor2, P.dX, P.dY, P.dZ, P.d_q, P.k, P.d_far, P.d_far_e, P.d_perm, out_mode,
-                         out, field, s);
-    } else {
-        launch_l2p(P.d_far_leaves, P.n_far_leaves, bx, lv, P.dX, P.dY, P.dZ, P.dL, P.k, P.pmax, P.d_far, s);
-        mark(6);
-        P2PArgs pa{};
-        pa.leaf_box = P.d_leaf_box;
-        pa.bx = bx;
-        pa.rng_ptr = P.d_rng_ptr;
-        pa.rng_start = P.d_rng_start;
-        pa.rng_pref = P.d_rng_pref;
-        pa.leaf_src_total = P.d_leaf_src_total;
-        pa.floor2 = P.d_floor2;
-        pa.X = P.dX;
-        pa.Y = P.dY;
-        pa.Z = P.dZ;
-        pa.q = P.d_q;
-        pa.k = P.k;
-        pa.far = P.d_far;
-        pa.perm = P.d_perm;
-        pa.out_mode = out_mode;
-        pa.out = out;
-        launch_p2p(pa, P.precision, P.n_leaves, P.n_p2p_fp32, P.p2p_lowk, P.p2p_tile, s);
     }
-    mark(7);
+    if (parts & kSeqP2P) {
+        if (field)
+            launch_p2p_field(P.precision, P.n_leaves, P.n_p2p_fp32, P.d_leaf_box, bx, P.d_rng_ptr, P.d_rng_start,
+                             P.d_rng_pref, P.d_leaf_src_total, P.d_floor2, P.dX, P.dY, P.dZ, P.d_q, P.k, P.d_far,
+                             P.d_far_e, P.d_perm, out_mode, out, field, s);
+        else
+            launch_p2p(p2p_args(P, out_mode, out), P.precision, P.n_leaves, P.n_p2p_fp32, P.p2p_lowk, P.p2p_tile, s);
+        mark(7);
+    }
 }
 
 // one evaluate into `out` of out_type (HFMM_C64: the FP64 result lands in d_out and is
@@ -1572,8 +1598,8 @@ struct hfmm_batch {
     std::vector<hfmm_plan*> plans;
     std::vector<std::vector<int>> branch;   // problems per stream, largest first
     std::vector<hipStream_t> streams;
-    std::vector<hipEvent_t> join;
-    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join, join2;
+    hipEvent_t fork = nullptr, fork2 = nullptr;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1581,11 +1607,46 @@ struct hfmm_batch {
     std::vector<void*> g_out;
     std::vector<unsigned> g_ver;            // degree versions of the plans at capture
     int g_qtype = -1;
+    // persistent launches over all problems (rebuilt with the graph): the block M2L, one
+    // launch per kernel instance, and the near field
+    struct M2LInstance {
+        int p = 0, tb = 0, g = 0, fp32 = 0, grid = 0;
+        size_t shm = 0;
+        WorkQueues q{};
+    };
+    std::vector<M2LInstance> m2l_inst;
+    std::vector<const M2LBlockArgs*> m2l_args;   // device descriptors per instance
+    const P2PBatchArgs* d_p2p_args = nullptr;
+    WorkQueues p2p_q[4]{};                 // per near-field leaf class (kP2PClass*)
+    int p2p_grid[4] = {0, 0, 0, 0};
+    int* d_counters = nullptr;
+    int n_counters = 0;
+    std::vector<void*> pallocs;             // device buffers of the persistent launches
+    void free_persistent() {
+        for (void* p : pallocs) (void)hipFree(p);
+        pallocs.clear();
+        m2l_inst.clear();
+        m2l_args.clear();
+        d_p2p_args = nullptr;
+        d_counters = nullptr;
+    }
+    template <typename T>
+    T* upload(const std::vector<T>& v) {
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(1, v.size()) * sizeof(T)) != hipSuccess)
+            fail(HFMM_ERR_MEMORY, "hipMalloc failed (batch work lists)");
+        pallocs.push_back(p);
+        if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return (T*)p;
+    }
     ~hfmm_batch() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
+        free_persistent();
         for (auto e : join) (void)hipEventDestroy(e);
+        for (auto e : join2) (void)hipEventDestroy(e);
         if (fork) (void)hipEventDestroy(fork);
+        if (fork2) (void)hipEventDestroy(fork2);
         for (auto st : streams) (void)hipStreamDestroy(st);
         if (stream) (void)hipStreamDestroy(stream);
         for (auto P : plans) delete P;
@@ -1596,15 +1657,118 @@ namespace {
 
 constexpr int kBatchStreams = 4;
 
-// fork on s, run every problem of a branch on its stream, join back on s
-void batch_sequence(hfmm_batch& B, const void* const* q, int q_type, void* const* out, hipStream_t s) {
+// kernel instance of the block M2L (the table-degree range selects KT and MAXU)
+int m2l_degree_range(int p) { return p <= 7 ? 0 : p <= 11 ? 1 : p <= 15 ? 2 : p <= 17 ? 3 : p <= 19 ? 4 : 5; }
+
+// queues of a work list: kXcds contiguous ranges of equal length
+WorkQueues make_queues(hfmm_batch& B, const std::vector<int2>& work, int* counters) {
+    std::vector<int> start(kXcds + 1);
+    for (int x = 0; x <= kXcds; ++x) start[x] = (int)((int64_t)work.size() * x / kXcds);
+    WorkQueues q;
+    q.work = B.upload(work);
+    q.start = B.upload(start);
+    q.count = counters;
+    return q;
+}
+
+// Device descriptors and work lists of the persistent launches for the current degrees
+// and pointers: the block M2L levels of every problem grouped by kernel instance, and the
+// leaves of every problem for the near field.
+void build_persistent(hfmm_batch& B, void* const* out, int q_type) {
+    B.free_persistent();
+    const int np = (int)B.plans.size();
+    std::map<std::array<int, 4>, std::vector<M2LBlockArgs>> groups;
+    std::map<std::array<int, 4>, int> pmax;
+    for (int i = 0; i < np; ++i) {
+        hfmm_plan& P = *B.plans[i];
+        for (int L = 2; L <= P.depth; ++L) {
+            const M2LBlockLevel& lb = P.m2lb[L];
+            if (!P.m2l[L].n_targets || !lb.use || lb.n_items <= 0) continue;
+            const int pk = std::max(lb.ps, lb.pt);
+            const std::array<int, 4> key{m2l_degree_range(pk), lb.tb, lb.grp, lb.fp32 ? 1 : 0};
+            groups[key].push_back(block_args(P, lb, lb.fp32, lb.fp32 ? (const void*)P.dM32 : (const void*)P.dM, P.dL, 0));
+            pmax[key] = std::max(pmax[key], pk);
+        }
+    }
+    B.n_counters = 16 * kXcds * ((int)groups.size() + 4);
+    std::vector<int> zeros(B.n_counters, 0);
+    B.d_counters = B.upload(zeros);
+    int qi = 0;
+    for (auto& kv : groups) {
+        hfmm_batch::M2LInstance in;
+        in.p = pmax[kv.first];
+        in.tb = kv.first[1];
+        in.g = kv.first[2];
+        in.fp32 = kv.first[3];
+        in.shm = m2l_block_lds_bytes(in.p, in.tb, in.g, in.fp32 ? 4 : 8);
+        std::vector<int2> work;
+        for (int j = 0; j < (int)kv.second.size(); ++j)
+            for (int t = 0; t < kv.second[j].n_items; ++t) work.push_back(make_int2(j, t));
+        const M2LBlockArgs* d_args = B.upload(kv.second);
+        in.q = make_queues(B, work, B.d_counters + 16 * kXcds * qi++);
+        in.grid = m2l_block_persistent_grid(in.p, in.tb, in.g, in.fp32, in.shm);
+        B.m2l_inst.push_back(in);
+        B.m2l_args.push_back(d_args);
+    }
+    std::vector<P2PBatchArgs> pa(np);
+    std::vector<int2> work[4];
+    for (int i = 0; i < np; ++i) {
+        hfmm_plan& P = *B.plans[i];
+        pa[i].a = p2p_scaled(p2p_args(P, out_mode_of(P, q_type), (double2*)out[i]), P.precision, P.p2p_lowk);
+        for (int l = 0; l < P.n_leaves; ++l) {
+            const int cls = P.precision == HFMM_PREC_FP64 ? kP2PClassFp64
+                            : P.p2p_lowk                  ? kP2PClassLowK
+                            : l < P.n_p2p_fp32            ? kP2PClassPhase32
+                                                          : kP2PClassPhase64;
+            work[cls].push_back(make_int2(i, l));
+        }
+    }
+    B.d_p2p_args = B.upload(pa);
+    for (int c = 0; c < 4; ++c) {
+        B.p2p_q[c] = make_queues(B, work[c], B.d_counters + 16 * kXcds * (qi + c));
+        B.p2p_grid[c] = work[c].empty() ? 0 : p2p_persistent_grid(c);
+    }
+}
+
+// ev (optional, 5 events): brackets of the upward part, the persistent M2L, the downward
+// part and the persistent near field
+void batch_sequence(hfmm_batch& B, const void* const* q, int q_type, void* const* out, hipStream_t s,
+                    hipEvent_t* ev = nullptr) {
+    auto mark = [&](int i) {
+        if (ev) HIPCHK(hipEventRecord(ev[i], s));
+    };
+    mark(0);
+    launch_clear_ints(B.d_counters, B.n_counters, s);
+    // upward passes and the generic-kernel M2L levels, problems forked over the streams
     HIPCHK(hipEventRecord(B.fork, s));
     for (size_t b = 0; b < B.streams.size(); ++b) {
         HIPCHK(hipStreamWaitEvent(B.streams[b], B.fork, 0));
-        for (int i : B.branch[b]) run_sequence(*B.plans[i], q[i], q_type, (double2*)out[i], B.streams[b], nullptr);
+        for (int i : B.branch[b])
+            run_sequence(*B.plans[i], q[i], q_type, (double2*)out[i], B.streams[b], nullptr, nullptr, nullptr,
+                         kSeqUp | kSeqM2LGeneric);
         HIPCHK(hipEventRecord(B.join[b], B.streams[b]));
         HIPCHK(hipStreamWaitEvent(s, B.join[b], 0));
     }
+    mark(1);
+    // the block M2L of every problem and level: persistent launches
+    for (size_t j = 0; j < B.m2l_inst.size(); ++j) {
+        const auto& in = B.m2l_inst[j];
+        launch_m2l_block_persistent(B.m2l_args[j], in.q, in.grid, in.p, in.tb, in.g, in.fp32, in.shm, s);
+    }
+    mark(2);
+    // downward passes
+    HIPCHK(hipEventRecord(B.fork2, s));
+    for (size_t b = 0; b < B.streams.size(); ++b) {
+        HIPCHK(hipStreamWaitEvent(B.streams[b], B.fork2, 0));
+        for (int i : B.branch[b])
+            run_sequence(*B.plans[i], q[i], q_type, (double2*)out[i], B.streams[b], nullptr, nullptr, nullptr, kSeqDown);
+        HIPCHK(hipEventRecord(B.join2[b], B.streams[b]));
+        HIPCHK(hipStreamWaitEvent(s, B.join2[b], 0));
+    }
+    mark(3);
+    // the near field of every problem: one persistent launch per leaf class
+    for (int c = 0; c < 4; ++c) launch_p2p_persistent(c, B.d_p2p_args, B.p2p_q[c], B.p2p_grid[c], s);
+    mark(4);
 }
 
 void batch_evaluate_device(hfmm_batch& B, const void* const* q, int q_type, void* const* out, hipStream_t s) {
@@ -1617,6 +1781,7 @@ void batch_evaluate_device(hfmm_batch& B, const void* const* q, int q_type, void
         if (B.graph) HIPCHK(hipGraphDestroy(B.graph));
         B.gexec = nullptr;
         B.graph = nullptr;
+        build_persistent(B, out, q_type);
         HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
         try {
             batch_sequence(B, q, q_type, out, B.stream);
@@ -1672,11 +1837,14 @@ int hfmm_batch_create(int n_problems, const double* const* xyz, const int64_t* n
         HIPCHK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
         B->streams.resize(ns);
         B->join.resize(ns);
+        B->join2.resize(ns);
         for (int b = 0; b < ns; ++b) {
             HIPCHK(hipStreamCreateWithFlags(&B->streams[b], hipStreamNonBlocking));
             HIPCHK(hipEventCreateWithFlags(&B->join[b], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&B->join2[b], hipEventDisableTiming));
         }
         HIPCHK(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&B->fork2, hipEventDisableTiming));
         *out = B.release();
     });
 }
@@ -1743,6 +1911,29 @@ int hfmm_batch_benchmark(hfmm_batch* B, const void* const* q_dev, int q_type, vo
         for (auto& e : ev) (void)hipEventDestroy(e);
         std::sort(ms.begin(), ms.end());
         *median_ms = ms[reps / 2];
+    });
+}
+
+int hfmm_batch_benchmark_phases(hfmm_batch* B, const void* const* q_dev, int q_type, void* const* out_dev, int reps,
+                                double* phase_ms) {
+    return guarded([&] {
+        if (!B || !q_dev || !out_dev || reps < 1 || !phase_ms) fail(HFMM_ERR_ARG, "bad argument");
+        hipStream_t s = B->stream;
+        batch_evaluate_device(*B, q_dev, q_type, out_dev, s);   // descriptors for these pointers
+        HIPCHK(hipStreamSynchronize(s));
+        hipEvent_t ev[5];
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        for (int i = 0; i < 4; ++i) phase_ms[i] = 0.0;
+        for (int r = 0; r < reps; ++r) {
+            batch_sequence(*B, q_dev, q_type, out_dev, s, ev);
+            HIPCHK(hipStreamSynchronize(s));
+            for (int i = 0; i < 4; ++i) {
+                float f = 0;
+                HIPCHK(hipEventElapsedTime(&f, ev[i], ev[i + 1]));
+                phase_ms[i] += f / reps;
+            }
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
     });
 }
 

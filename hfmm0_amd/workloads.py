@@ -60,6 +60,30 @@ def mixed_problem(n, k_range=(1.0, 60.0), seed=0):
     return x, q, k, ("uniform", "sphere", "clusters", "mixed")[kind]
 
 
+def ellipsoid_problem(n, k_range=(1.0, 50.0), seed=0):
+    """One problem of the batched configuration: N points uniform in the solid
+    axis-aligned ellipsoid centred at (0.5, 0.5, 0.5) with semi-axes uniform in
+    [0.2, 0.5]; k uniform in k_range; complex N(0,1) charges rounded to complex64.
+    default_rng(seed) draws the semi-axes, k, the points, then the charges."""
+    rng = np.random.default_rng(seed)
+    axes = rng.uniform(0.2, 0.5, 3)
+    k = float(rng.uniform(*k_range))
+    v = rng.normal(size=(n, 3))
+    v *= (rng.uniform(size=n) ** (1.0 / 3.0) / np.linalg.norm(v, axis=1))[:, None]
+    x = 0.5 + v * axes
+    q = (rng.normal(size=n) + 1j * rng.normal(size=n)).astype(np.complex64).astype(np.complex128)
+    return x, q, k, axes
+
+
+def batch_problems(n_problems=256, n_range=(2_000, 200_000), k_range=(1.0, 50.0), seed=4000):
+    """The batched configuration: problem i has N log-uniform in n_range (default_rng(seed)
+    draws all N first) and the ellipsoid, k and charges of ellipsoid_problem(N, k_range,
+    seed + 1 + i)."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]), n_problems)).astype(int)
+    return [ellipsoid_problem(int(n), k_range, seed + 1 + i) for i, n in enumerate(ns)]
+
+
 def sphere_clusters(n_sphere, n_clusters, seed):
     """Points on the sphere |x - c| = 0.45, c = (0.5, 0.5, 0.5), then points in 8 Gaussian
     clusters (sigma 0.02, centres uniform in [0.1, 0.9]^3), from default_rng(seed)."""

@@ -217,9 +217,12 @@ int hfmm_mlp_destroy(hfmm_mlp* mlp);
  * Batches of independent problems (many uneven problems, e.g. 256 problems of
  * 2k-200k points).  No counterpart in the reference, whose setup/evaluate handle one
  * problem; the role of its tuner driver (jaxfmm/tune.py:32-172) is to run many.
- * One plan per problem (opts[i], k[i]); hfmm_batch_evaluate runs all problems, forked
- * over a few streams balanced by point count and joined, captured once as one
- * hipGraph per set of pointers and replayed.  q[i] / out[i] as in hfmm_evaluate.
+ * One plan per problem (opts[i], k[i]).  hfmm_batch_evaluate runs all problems as one
+ * hipGraph (captured once per set of pointers and replayed): the upward and downward
+ * passes forked over a few streams balanced by point count, the matrix-core M2L of every
+ * problem and level and the near field of every problem as persistent launches whose
+ * resident workgroups take (problem, item) work from device queues, one per XCD.
+ * q[i] / out[i] as in hfmm_evaluate.
  * --------------------------------------------------------------------------------- */
 typedef struct hfmm_batch hfmm_batch;
 int hfmm_batch_create(int n_problems, const double* const* xyz, const int64_t* n, const double* k,
@@ -231,6 +234,11 @@ int hfmm_batch_evaluate(hfmm_batch* batch, const void* const* q, int q_type, voi
 /* median milliseconds of one device-resident batch evaluate (hipEvents, graph replays) */
 int hfmm_batch_benchmark(hfmm_batch* batch, const void* const* q_dev, int q_type, void* const* out_dev,
                          int warmup, int reps, double* median_ms);
+/* mean milliseconds of the four parts of a device-resident batch evaluate, run eagerly
+ * with hipEvents between them: phase_ms[0] upward passes (and generic-kernel M2L levels),
+ * [1] the persistent matrix-core M2L, [2] downward passes, [3] the persistent near field */
+int hfmm_batch_benchmark_phases(hfmm_batch* batch, const void* const* q_dev, int q_type, void* const* out_dev,
+                                int reps, double* phase_ms);
 int hfmm_batch_destroy(hfmm_batch* batch);
 
 const char* hfmm_last_error(void);

@@ -53,3 +53,15 @@ def test_refine_degrees_certifies_and_is_not_slower():
     st.close()
     print(f"  seed {seed} -> {pl}: {hist[0][1]:.3f} -> {ms:.3f} ms, held-out err {held_out:.2e}", flush=True)
     assert held_out <= 1e-6
+
+
+def test_tune_batch_picks_leaf_and_tile():
+    from hfmm0_amd.autotune import tune_batch
+    from hfmm0_amd.workloads import batch_problems
+    probs = batch_problems(10, n_range=(2_000, 20_000), seed=4100)
+    print(f"case: batch autotune, {len(probs)} problems", flush=True)
+    leaf, tile, t = tune_batch([p[0] for p in probs], [p[2] for p in probs], [p[1] for p in probs],
+                               leaves=(32, 64), tiles=(2, 4), subsample=6, reps=2)
+    print(f"  leaf {leaf} tile {tile} timings {t}", flush=True)
+    assert leaf in (32, 64) and tile in (0, 2, 4)
+    assert t[f"leaf{leaf}_tile{tile}"] == min(v for kk, v in t.items() if kk.startswith(f"leaf{leaf}_"))

@@ -6,7 +6,9 @@
   the smallest fixed degree reaching 1e-6), and the mixed form at fixed per-level degrees;
 * sphere + clusters N=2M, k=100, depth <= 10 (adaptive tree with W and X lists): the
   learned degrees with the high-frequency floor, mixed and FP64, both within 1e-6;
-* uniform + sphere + clusters N=4M, k=60: the learned degrees, mixed and FP64, within 1e-6.
+* uniform + sphere + clusters N=4M, k=60: the learned degrees, mixed and FP64, within 1e-6;
+* 256 problems in ellipsoids (N log-uniform in [2k, 200k], k in [1, 50]) in one batch of
+  persistent launches, each problem on 512 sampled targets.
 
 Each case prints its name and sizes before touching the GPU.
 """
@@ -113,3 +115,45 @@ def test_mixed_distribution_4m_k60():
     pl, res = _learned_config(c5_problem, 128, None, "uniform + sphere + clusters")
     for prec, (e, depth, nw, nx) in res.items():
         assert e <= TOL, (prec, e)
+
+
+def test_batched_256_ellipsoids():
+    """256 problems (uniform in ellipsoids with semi-axes in [0.2, 0.5], N log-uniform in
+    [2k, 200k], k in [1, 50], complex64 charges) in one batch, leaf 128, mixed precision:
+    degree 12 raised to the high-frequency floor, tables built 4 degrees higher, and
+    Batch.certify raising the degrees of every problem that misses 1e-6 on its 512
+    sampled targets against the FP64 direct sum.  All 256 must certify; the batch of
+    persistent launches equals its own plans evaluated one by one."""
+    import torch
+    import hfmm0_amd as hf
+    from hfmm0_amd.autotune import batch_degrees
+    from hfmm0_amd.batch import Batch
+    from hfmm0_amd.workloads import batch_problems
+    probs = batch_problems(256)
+    xs, qs, ks = [x for x, _, _, _ in probs], [q for _, q, _, _ in probs], [k for _, _, k, _ in probs]
+    say(f"case: batch of {len(probs)} ellipsoid problems, {sum(len(x) for x in xs)} points, leaf 128, mixed")
+    pls = [batch_degrees(x, k) for x, k in zip(xs, ks)]
+    tabs = [[min(40, v + 4) for v in pl] for pl in pls]
+    B = Batch(xs, ks, p=max(max(t) for t in tabs), N_max=128, p_level=tabs, precision="mixed")
+    for i, pl in enumerate(pls):
+        st = B.plan(i)
+        st.set_degrees(pl[:st.depth])
+    phis, cert, errs = B.certify(qs, tol=TOL, n_samp=512, max_steps=6)
+    raised = sum(list(c) != list(p)[:len(c)] for c, p in zip(cert, pls))
+    worst = int(np.argmax(errs))
+    say(f"  certified rel err max {max(errs):.2e} (problem {worst}: n={len(xs[worst])} k={ks[worst]:.1f}) "
+        f"median {np.median(errs):.2e}; {raised} problems raised; {sum(e <= TOL for e in errs)}/256 within {TOL:g}")
+    qd = [torch.from_numpy(q).cuda() for q in qs]
+    od = [torch.empty(len(q), dtype=torch.complex128, device="cuda") for q in qs]
+    ms = B.benchmark(qd, od, warmup=1, reps=5)
+    say(f"  batch evaluate {ms:.1f} ms: {sum(len(x) for x in xs) / ms * 1e-3:.3e} evals/s (tables 4 degrees up)")
+    torch.cuda.synchronize()
+    for i in (0, worst, 255):
+        np.testing.assert_allclose(od[i].cpu().numpy(), phis[i], rtol=0, atol=1e-12 * np.abs(phis[i]).max())
+        st = B.plan(i)
+        one = torch.empty_like(od[i])
+        hf.evaluate_device(st, qd[i], one)
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(od[i].cpu().numpy(), one.cpu().numpy(), rtol=0, atol=1e-12 * np.abs(phis[i]).max())
+    assert max(errs) <= TOL
+    B.close()
