@@ -106,7 +106,8 @@ def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_le
     ``tree_device=True`` builds the octree and lists on the GPU (same arrays).
     ``m2l_fp64_levels`` (mixed precision only) lists the tree levels whose M2L stays in
     FP64; ``None`` applies the automatic rule (FP64 where k times the level's box diagonal
-    exceeds ``MIXED_FP32_MAX_KD``).
+    exceeds ``MIXED_FP32_MAX_KD``).  ``p2p_tile`` (mixed precision) sets the targets per
+    lane of the near-field kernel: 1, 2 or 4 (0: 2).
     """
     x, k = _check_problem(positions, k)
     o = make_opts(p, N_max, max_depth, precision=precision, p_level=p_level, use_graph=use_graph,

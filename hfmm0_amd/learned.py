@@ -211,14 +211,41 @@ def label_levels(state, q, budget=5e-7, p_lo=4, shares=None):
     return np.array(labels)
 
 
-def label_levels_joint(state, q, budget=6e-7, p_lo=4):
+DEFAULT_COSTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "m2l_cost.json")
+
+
+def m2l_pair_cost(path=None):
+    """Cost of one M2L pair as a function of the degree: the measured nanoseconds per V
+    pair of the matrix-core kernel (hfmm0_amd.train_degrees --calibrate writes the table)
+    where present; beyond the table, and without it, F_M2L(p) scaled to the table's last
+    entry (or the flop count itself)."""
+    import json
+    from .workloads import f_m2l
+    try:
+        with open(path or DEFAULT_COSTS) as f:
+            tab = {int(p): float(v) for p, v in json.load(f)["ns_per_pair"].items()}
+    except (OSError, ValueError, KeyError):
+        return f_m2l
+    top = max(tab)
+
+    def cost(p):
+        if p in tab:
+            return tab[p]
+        if p > top:
+            return tab[top] * f_m2l(p) / f_m2l(top)
+        return tab[min(tab)] * f_m2l(p) / f_m2l(min(tab))
+    return cost
+
+
+def label_levels_joint(state, q, budget=6e-7, p_lo=4, cost=None):
     """Cost-optimal degrees per level (2..depth) for a total error budget: starting from
     the table degrees, lower one level's degree at a time -- the level whose step saves
-    the most M2L flops (its V pairs times the drop of F_M2L) per unit of added error --
-    while the error of the whole evaluate (relative to the result at the table degrees)
-    stays within `budget`.  Every trial is one evaluate with evaluate-time degrees."""
+    the most M2L cost (its V pairs times the drop of the per-pair cost) per unit of added
+    error -- while the error of the whole evaluate (relative to the result at the table
+    degrees) stays within `budget`.  `cost(p)`: per-pair cost, by default the measured
+    time (m2l_pair_cost).  Every trial is one evaluate with evaluate-time degrees."""
     from .api import evaluate
-    from .workloads import f_m2l
+    f_m2l = cost or m2l_pair_cost()
     table = state.p_table
     depth = state.depth
     tree = state.tree()
@@ -246,7 +273,7 @@ def label_levels_joint(state, q, budget=6e-7, p_lo=4):
             e = err_of(trial)
             if e > budget:
                 continue
-            gain = pairs[lv] * (f_m2l(cur[lv]) - f_m2l(cur[lv] - 1)) / max(e - e_cur, 1e-15)
+            gain = pairs[lv] * max(f_m2l(cur[lv]) - f_m2l(cur[lv] - 1), 0.0) / max(e - e_cur, 1e-15)
             if best is None or gain > best[0]:
                 best = (gain, lv, e)
         if best is None:

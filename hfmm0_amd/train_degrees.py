@@ -1,11 +1,12 @@
 """Generates the learned-degree training data with the FMM itself and trains the MLP on
 the GPU.  Run on a GPU:  python -m hfmm0_amd.train_degrees --problems 200 --out DIR
 
-For every seeded problem (hfmm0_amd.workloads.mixed_problem, N log-uniform, leaf size
-drawn from 32/64/128) it builds one plan at the table degree, labels every level by the
-cost-optimal degree vector for the total budget (hfmm0_amd.learned.label_levels_joint:
-greedy descent from the table degrees, the level saving the most M2L flops per unit of
-added error first) and records the level features.  The network is then
+For every seeded problem (hfmm0_amd.workloads.mixed_problem, N log-uniform in [10k,
+200k], k uniform in [1, 80], leaf size drawn from 32/64/128) it builds one plan at the
+table degree, labels every level by the cost-optimal degree vector for the total budget
+(hfmm0_amd.learned.label_levels_joint: greedy descent from the table degrees, the level
+saving the most measured M2L time per unit of added error first; --calibrate measures the
+per-pair time of every degree first) and records the level features.  The network is then
 trained by the HIP kernels (hfmm_mlp_train) on 80% of the rows; the rest is held out.
 """
 import argparse
@@ -23,7 +24,33 @@ from .tree import build_tree
 from .workloads import mixed_problem
 
 
-def generate(n_problems, seed0=1000, n_range=(10_000, 200_000), k_range=(1.0, 60.0), leaves=(32, 64, 128),
+def calibrate(path, n=1_000_000, k=40.0, leaf=128, degrees=range(4, 24), reps=5):
+    """Measured M2L time per V pair against the degree: the uniform cube of n points,
+    mixed precision, every level at degree p, the finest level's matrix-core launch timed
+    with hipEvents (hfmm_benchmark_m2l_levels); nanoseconds per pair -> `path`."""
+    import torch
+    from .workloads import uniform_cube
+    x, q = uniform_cube(n)
+    q_dev = torch.from_numpy(q).cuda()
+    o_dev = torch.empty(n, dtype=torch.complex128, device="cuda")
+    table = {}
+    for p in degrees:
+        st = api.setup(x, k, p=p, N_max=leaf, precision="mixed")
+        t = st.tree()
+        lv = t["level"][t["V"][:, 0]]
+        pairs = int((lv == st.depth).sum())
+        ms = api.benchmark_m2l_levels(st, q_dev, o_dev, reps=reps)
+        table[p] = ms[st.depth] * 1e6 / pairs
+        print(f"calibrate: p={p} level {st.depth} {pairs} pairs {ms[st.depth]:.3f} ms -> {table[p]:.3f} ns/pair",
+              flush=True)
+        st.close()
+    with open(path, "w") as f:
+        json.dump({"workload": f"uniform cube N={n}, k={k:g}, leaf {leaf}, mixed precision, finest level",
+                   "ns_per_pair": table}, f, indent=1)
+    return table
+
+
+def generate(n_problems, seed0=1000, n_range=(10_000, 200_000), k_range=(1.0, 80.0), leaves=(32, 64, 128),
              p_tab=23, budget=6e-7, log=None):
     feats, labels, meta = [], [], []
     for i in range(n_problems):
@@ -53,8 +80,14 @@ def main():
     ap.add_argument("--epochs", type=int, default=400)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--budget", type=float, default=6e-7, help="total error budget of the labels")
+    ap.add_argument("--calibrate", action="store_true", help="measure the per-pair M2L cost table first")
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
+    if args.calibrate:
+        calibrate(os.path.join(args.out, "m2l_cost.json"))
+        import shutil
+        from .learned import DEFAULT_COSTS
+        shutil.copy(os.path.join(args.out, "m2l_cost.json"), DEFAULT_COSTS)
     X, p, meta = generate(args.problems, budget=args.budget, log=sys.stdout)
     np.savez_compressed(os.path.join(args.out, "degree_data.npz"), X=X, p=p,
                         meta=np.array([json.dumps(m) for m in meta]))
@@ -69,7 +102,8 @@ def main():
     net.save(os.path.join(args.out, "degree_mlp.npz"))
     with open(os.path.join(args.out, "degree_mlp.json"), "w") as f:
         json.dump({"problems": args.problems, "rows": int(len(X)), "train_rows": n_train, "epochs": args.epochs,
-                   "budget_total": args.budget, "labels": "cost-optimal joint degrees (greedy)", "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+                   "budget_total": args.budget, "k_range": [1.0, 80.0],
+                   "labels": "cost-optimal joint degrees (greedy), priced by measured M2L time per pair", "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
                    "heldout_mean_resid": float(resid.mean()), "heldout_under": float(np.mean(resid < 0)),
                    "heldout_within1": float(np.mean(np.abs(resid) <= 1))}, f, indent=1)
 

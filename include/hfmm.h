@@ -31,9 +31,11 @@ extern "C" {
 
 /* precision of the evaluate path */
 #define HFMM_PREC_FP64 0       /* FP64 everywhere (the plain FMM) */
-#define HFMM_PREC_MIXED 1      /* FP32 near-field weights (FP64 coordinate differences) and FP32 M2L on
-                                  matrix cores (at the levels hfmm_opts.m2l_fp64_levels leaves
-                                  in FP32); P2M/M2M/L2L/L2P and accumulation in FP64 */
+#define HFMM_PREC_MIXED 1      /* near field: coordinate differences and r^2 in FP64, weights
+                                  e^{ikr}/r in FP32 (the phase k r in FP64 for leaves whose pairs
+                                  reach k r > 8), 64-source sums in FP32 added in FP64; M2L in
+                                  FP32 on matrix cores (at the levels hfmm_opts.m2l_fp64_levels
+                                  leaves in FP32); P2M/M2M/L2L/L2P and accumulation in FP64 */
 
 /* pointer kinds for q / out */
 #define HFMM_HOST 0
@@ -90,9 +92,10 @@ typedef struct hfmm_plan_info {
     double pmin[3];
     int p_table[HFMM_MAX_LEVELS];   /* degree the level's tables were built for */
     int m2l_fp32_levels;            /* bit l set: the M2L of level l runs in FP32 */
-    int n_leaves_phase64;           /* HFMM_PREC_MIXED: leaves whose FP32 near field takes the
-                                       phase k r in FP64 (pairs reaching k r > 8, where an FP32
-                                       phase errs by ~k r 2^-23 rad) */
+    int n_leaves_phase64;           /* HFMM_PREC_MIXED: leaves whose FP32 near-field weights take
+                                       the phase k r in FP64 (pairs reaching k r > 8, where an FP32
+                                       phase errs by ~k r 2^-23 rad); every leaf takes coordinate
+                                       differences and r^2 in FP64 */
 } hfmm_plan_info;
 
 void hfmm_opts_default(hfmm_opts* o);

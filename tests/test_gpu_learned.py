@@ -1,6 +1,6 @@
 """GPU test of the learned truncation degree: on problems outside the training seeds, the
 shipped network's per-level degrees meet the 1e-6 target (checked against the FP64 direct
-sum) with less M2L work than the smallest fixed degree that meets it, and the certified
+sum) and evaluate faster than the smallest fixed degree that meets it, and the certified
 evaluate falls back to the plain plan when the tuned one misses the tolerance."""
 import numpy as np
 import pytest
@@ -22,40 +22,44 @@ def _v_per_level(st):
     return {int(l): int((lv == l).sum()) for l in range(2, st.depth + 1)}
 
 
-@pytest.mark.parametrize("n,k,leaf,seed,bound", [(200_000, 20.0, 32, 900_003, 0.9), (150_000, 35.0, 32, 900_002, 0.7),
-                                                (200_000, 20.0, 32, 900_001, 1.25)])
-def test_learned_degrees_beat_best_fixed_degree(n, k, leaf, seed, bound):
-    """Deep trees: the finest level holds most V pairs at a small k * box size, so
-    per-level degrees save much of the M2L work of the best single degree (measured: 24%
-    on the uniform problem, 45% on the sphere).  On clustered points (seed 900001) the network is
-    conservative at the deepest levels; its work stays within 25% of the best single
-    degree there."""
+@pytest.mark.parametrize("n,k,leaf,seed", [(200_000, 20.0, 32, 900_003), (150_000, 35.0, 32, 900_002),
+                                          (200_000, 20.0, 32, 900_001)])
+def test_learned_degrees_beat_best_fixed_degree(n, k, leaf, seed):
+    """On problems outside the training seeds (uniform, sphere, clusters), the network's
+    per-level degrees meet 1e-6 and evaluate faster (mixed precision, median of timed
+    graph replays) than the smallest single degree that meets 1e-6."""
+    import torch
     import hfmm0_amd as hf
     from hfmm0_amd.learned import DegreeMLP, m2l_flops, predict_p_level, sampled_error
     print(f"case: learned degrees n={n} k={k} leaf={leaf} seed={seed}", flush=True)
     x, q, kind = _problem(n, k, seed)
     net = DegreeMLP.load()
     pl = predict_p_level(net, x, q, k, n_max=leaf, margin=0.0)
-    # degree selection alone: both forms in FP64, against the best fixed degree
-    tuned = hf.setup(x, k, p=max(pl), N_max=leaf, p_level=pl)
+    net.close()
+    tuned = hf.setup(x, k, p=max(pl), N_max=leaf, p_level=pl, precision="mixed")
     err_t = sampled_error(x, q, k, hf.evaluate(tuned, q), n_samp=1024)
     vpl = _v_per_level(tuned)
-    f_t = m2l_flops(tuned.p_level, vpl)
     # the smallest fixed degree meeting the target, from one plan at degree 23
-    fixed = hf.setup(x, k, p=23, N_max=leaf)
+    probe = hf.setup(x, k, p=23, N_max=leaf, precision="mixed")
     best = None
     for p in range(8, 24):
-        fixed.set_degrees([p] * fixed.depth)
-        if sampled_error(x, q, k, hf.evaluate(fixed, q), n_samp=1024) <= TOL:
+        probe.set_degrees([p] * probe.depth)
+        if sampled_error(x, q, k, hf.evaluate(probe, q), n_samp=1024) <= TOL:
             best = p
             break
+    probe.close()
     assert best is not None
-    f_f = m2l_flops([best] * fixed.depth, vpl)
-    print(f"  {kind}: learned {tuned.p_level} err {err_t:.2e} flops {f_t:.3e}; "
-          f"fixed p={best} flops {f_f:.3e} (gain {f_f / f_t:.2f}x)", flush=True)
+    fixed = hf.setup(x, k, p=best, N_max=leaf, precision="mixed")
+    qd = torch.from_numpy(q).cuda()
+    od = torch.empty(len(q), dtype=torch.complex128, device="cuda")
+    ms_t = hf.benchmark(tuned, qd, od, warmup=2, reps=15)
+    ms_f = hf.benchmark(fixed, qd, od, warmup=2, reps=15)
+    f_t, f_f = m2l_flops(tuned.p_level, vpl), m2l_flops([best] * fixed.depth, vpl)
+    print(f"  {kind}: learned {tuned.p_level} err {err_t:.2e} {ms_t:.3f} ms ({f_t:.3e} M2L flops); "
+          f"fixed p={best} {ms_f:.3f} ms ({f_f:.3e}); gain {ms_f / ms_t:.2f}x in time", flush=True)
     assert err_t <= TOL
     assert tuned.depth >= 5
-    assert f_t < bound * f_f
+    assert ms_t < ms_f
     tuned.close()
     fixed.close()
 

@@ -63,3 +63,22 @@ def test_fixtures(path):
     if len(x) <= 640:
         again = kifmm.evaluate(x, q, k, p=int(f["p_ref"]), n_max=int(f["leaf"]))
         np.testing.assert_allclose(again, f["phi_ref"], rtol=1e-11, atol=1e-11 * np.abs(f["phi_ref"]).max())
+
+
+def test_c_evaluate_matches_numpy_restatement():
+    # the C/OpenMP evaluate phases (oracle/kifmm_eval.c, timed by bench.py's CPU baseline)
+    # against the NumPy restatement on a clustered input with W and X lists
+    from oracle import kifmm as K
+    from oracle import kifmm_c as KC
+    rng = np.random.default_rng(77)
+    n = 900
+    x = rng.uniform(size=(n, 3))
+    x[: n // 2] = rng.uniform(0.1, 0.3, (n // 2, 3))
+    q = rng.normal(size=n) + 1j * rng.normal(size=n)
+    pr = KC.Prepared(x, 3.0, p=4, n_max=12)
+    assert len(pr.arrays["w_src"]) > 0 and len(pr.arrays["x_src"]) > 0
+    got = pr.evaluate(q)
+    ref = K.evaluate(x, q, 3.0, p=4, n_max=12)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
+    # k = 0: one pass, exactly real
+    assert np.count_nonzero(KC.Prepared(x, 0.0, p=4, n_max=12).evaluate(q.real).imag) == 0
