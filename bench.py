@@ -71,6 +71,17 @@ def run_form(hf, x, q_dev, out_dev, k, leaf, p, p_level, precision, steps, warmu
                 n_v=int(info.n_v), m2l_level_ms=level_ms)
 
 
+def kernel_source_sha256():
+    """SHA-256 of the sources the matrix-core M2L kernel is compiled from (kernel file,
+    shared headers, build flags): a traffic figure measured on other sources is stale."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in ("m2l_mfma.hip", "kernels.h", "common.h", "Makefile"):
+        with open(os.path.join(ROOT, "hfmm0_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
 def m2l_roofline(form, precision):
     """Roofline of the dominant kernel, the M2L launch of the costliest level: algorithmic
     flops of that launch (F_M2L(p) per V pair x its V pairs) over its average duration,
@@ -85,19 +96,24 @@ def m2l_roofline(form, precision):
     achieved = flops / (lv_ms[lvl] * 1e-3)
     peak = PEAK["fp32"] if precision == "mixed" else PEAK["fp64"]
     traffic = traffic_alg = None
+    traffic_note = "no rocprofv3 FETCH_SIZE/WRITE_SIZE summary for this launch"
     tf = os.path.join(ROOT, "profiles", "m2l_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             rec = json.load(f)
         if rec.get("level") == lvl and rec.get("p") == p and rec.get("precision") == precision:
-            traffic = rec.get("bytes_per_launch")
-            traffic_alg = rec.get("algorithmic_bytes_per_launch")
+            if rec.get("source_sha256") == kernel_source_sha256():
+                traffic = rec.get("bytes_per_launch")
+                traffic_alg = rec.get("algorithmic_bytes_per_launch")
+                traffic_note = "profiles/m2l_traffic.json (same kernel sources)"
+            else:
+                traffic_note = "profiles/m2l_traffic.json was measured on other kernel sources: not used"
     total = form["m2l_flops"] / (sum(lv_ms.values()) * 1e-3)
     return {"bound": "mfma",
             "kernel": f"m2l_block_kernel, level {lvl} (p={p}, {form['v_per_level'][lvl]} V pairs), "
                       f"{'FP32' if precision == 'mixed' else 'FP64'} MFMA",
             "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": traffic,
+            "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_note,
             # compulsory bytes of the launch (multipoles read once, locals read and written);
             # the measured traffic above them is mostly register spill (profiles/m2l_traffic.json)
             "traffic_algorithmic": traffic_alg,

@@ -119,6 +119,11 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
     // (cos m phi, +-sin m phi), the sine negated for odd (imaginary) columns; orders
     // m >= PH hold zero
     constexpr int TL = 16 * KT;
+    // TB = 16: the real parts of a direction's 16 pairs fill one 16-column tile and the
+    // imaginary parts the next (SPL), so a value's partner (re <-> im) sits in the same lane
+    // of the other tile; otherwise (TB = 8) the columns of a pair are adjacent (re, im) and
+    // the partner is the neighbouring lane
+    constexpr bool SPL = TB == 16;
     constexpr int RSTEP = Mma<T>::row(0, 1) - Mma<T>::row(0, 0);   // rows between output registers
     // FP32 with one tile per block (p_tab <= 15, the finest levels): every unit of every
     // stage uses one fragment set, and the next unit's set is loaded while the current one
@@ -139,7 +144,7 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
     const int li = lane & 15, lk = lane >> 4;
     // this lane's phase entries of stage 3: add 16 mt for a unit's row tile mt and
     // g TL + r RSTEP for direction g and output register r
-    const C2* ps_lane = Ps + (li & 1) * G * TL + Mma<T>::row(lk, 0);
+    const C2* ps_lane = Ps + (SPL ? 0 : (li & 1)) * G * TL + Mma<T>::row(lk, 0);
     if (item.y >= item.z) return;   // padding item
     const int* tg = a.blk_targets + (size_t)item.x * TB;
     const C2* Mexp = (const C2*)a.M;
@@ -222,8 +227,9 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                 const T sg = (m & 1) ? (T)-1 : (T)1;
                 T* xu = X + cp * LDX + g * NCOLD;
                 T* xv = X + cm * LDX + g * NCOLD;
-                // sources in batches of JB (all loads of a batch in flight together)
-                constexpr int JB = sizeof(T) == 8 ? TB / 2 : TB;
+                // sources in batches of JB (all loads of a batch in flight together; half of
+                // the block when the split layout's separate re / im stores keep more live)
+                constexpr int JB = sizeof(T) == 8 || SPL ? TB / 2 : TB;
 #pragma unroll
                 for (int j0 = 0; j0 < TB; j0 += JB) {
                     C2 rp[JB], rm[JB];
@@ -245,11 +251,12 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                         const T ar = rp[j].x * pm.x - rp[j].y * pm.y, ai = rp[j].x * pm.y + rp[j].y * pm.x;
                         const T br = sg * (rm[j].x * pm.x + rm[j].y * pm.y);
                         const T bi = sg * (rm[j].y * pm.x - rm[j].x * pm.y);
-                        xu[2 * (j0 + j)] = ar + br;
-                        xu[2 * (j0 + j) + 1] = ai + bi;
+                        const int cre = SPL ? j0 + j : 2 * (j0 + j), cim = SPL ? TB + j0 + j : 2 * (j0 + j) + 1;
+                        xu[cre] = ar + br;
+                        xu[cim] = ai + bi;
                         if (m > 0) {
-                            xv[2 * (j0 + j)] = ar - br;
-                            xv[2 * (j0 + j) + 1] = ai - bi;
+                            xv[cre] = ar - br;
+                            xv[cim] = ai - bi;
                         }
                     }
                 }
@@ -404,6 +411,26 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                             V4 c4[NCT];
 #pragma unroll
                             for (int ct = 0; ct < NCT; ++ct) c4[ct] = V4{0, 0, 0, 0};
+                            if constexpr (SPL) {
+                                // re tile: Re T x - Im T y, im tile: Re T y + Im T x, with the
+                                // partner values in the same lane of the other tile
+                                T b0[4][NCT];
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) {
+                                    const int cin = sg ? rowm[j] : rowp[j];
+#pragma unroll
+                                    for (int ct = 0; ct < NCT; ++ct) b0[j][ct] = X[cin + ct * 16 + li];
+                                }
+                                const V4 nim = -aim;
+#pragma unroll
+                                for (int j = 0; j < 4; ++j)
+                                    if (j < nks)
+#pragma unroll
+                                        for (int ct = 0; ct < NCT; ++ct) {
+                                            c4[ct] = Mma<T>::mma(are[j], b0[j][ct], c4[ct]);
+                                            c4[ct] = Mma<T>::mma((ct & 1) ? aim[j] : nim[j], b0[j][ct ^ 1], c4[ct]);
+                                        }
+                            } else {
                             T b0[4][NCT], b1[4][NCT];
 #pragma unroll
                             for (int j = 0; j < 4; ++j) {
@@ -422,6 +449,7 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                                         c4[ct] = Mma<T>::mma(are[j], b0[j][ct], c4[ct]);
                                         c4[ct] = Mma<T>::mma(aim[j], b1[j][ct], c4[ct]);
                                     }
+                            }
 #pragma unroll
                             for (int r = 0; r < 4; ++r) {
                                 const int sr = r * RSTEP;
@@ -482,8 +510,16 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
     #pragma unroll
                                     for (int ct = 0; ct < NCT; ++ct) {
                                         b0[j][ct] = X[cin + ct * 16 + li];
-                                        b1[j][ct] = sgn1 * X[cin + ct * 16 + (li ^ 1)];
+                                        if constexpr (!SPL) b1[j][ct] = sgn1 * X[cin + ct * 16 + (li ^ 1)];
                                     }
+                                }
+                                if constexpr (SPL) {
+                                    // partner values in the same lane of the other tile; the
+                                    // re tiles take -Im T
+    #pragma unroll
+                                    for (int j = 0; j < 4; ++j)
+    #pragma unroll
+                                        for (int ct = 0; ct < NCT; ++ct) b1[j][ct] = b0[j][ct ^ 1];
                                 }
     #pragma unroll
                                 for (int j = 0; j < 4; ++j)
@@ -493,8 +529,9 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                                             if (t < nt)
     #pragma unroll
                                                 for (int ct = 0; ct < NCT; ++ct) {
+                                                    const T ai = SPL && !(ct & 1) ? -aim[t][j] : aim[t][j];
                                                     c4[sg][t][ct] = Mma<T>::mma(are[t][j], b0[j][ct], c4[sg][t][ct]);
-                                                    c4[sg][t][ct] = Mma<T>::mma(aim[t][j], b1[j][ct], c4[sg][t][ct]);
+                                                    c4[sg][t][ct] = Mma<T>::mma(ai, b1[j][ct], c4[sg][t][ct]);
                                                 }
                             }
                         }
@@ -578,11 +615,21 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
 #pragma unroll
                         for (int g = 0; g < G; ++g) {
                             const C2 pm = ps_lane[g * TL + r * RSTEP];
+                            if constexpr (SPL) {
+                                // tiles (re, im) of the direction: U += c u' - i s v', V likewise
+                                const T ur = cu[2 * g][r], ui = cu[2 * g + 1][r];
+                                const T vr = cv[2 * g][r], vi = cv[2 * g + 1][r];
+                                accu[uu][0][r] = fma(ur, pm.x, fma(vi, pm.y, accu[uu][0][r]));
+                                accu[uu][1][r] = fma(ui, pm.x, fma(-vr, pm.y, accu[uu][1][r]));
+                                accv[uu][0][r] = fma(vr, pm.x, fma(ui, pm.y, accv[uu][0][r]));
+                                accv[uu][1][r] = fma(vi, pm.x, fma(-ur, pm.y, accv[uu][1][r]));
+                            } else {
 #pragma unroll
                             for (int cd = 0; cd < NCTD; ++cd) {
                                 const T wu = cu[g * NCTD + cd][r], wv = cv[g * NCTD + cd][r];
                                 accu[uu][cd][r] = fma(wu, pm.x, fma(Mma<T>::swap_pair(wv), pm.y, accu[uu][cd][r]));
                                 accv[uu][cd][r] = fma(wv, pm.x, fma(Mma<T>::swap_pair(wu), pm.y, accv[uu][cd][r]));
+                            }
                             }
                         }
                     }
@@ -645,11 +692,20 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
 #pragma unroll
                         for (int g = 0; g < G; ++g) {
                             const C2 pm = pu[g * TL + r * RSTEP];
+                            if constexpr (SPL) {
+                                const T ur = cu[2 * g][r], ui = cu[2 * g + 1][r];
+                                const T vr = cv[2 * g][r], vi = cv[2 * g + 1][r];
+                                accu[uu][0][r] = fma(ur, pm.x, fma(vi, pm.y, accu[uu][0][r]));
+                                accu[uu][1][r] = fma(ui, pm.x, fma(-vr, pm.y, accu[uu][1][r]));
+                                accv[uu][0][r] = fma(vr, pm.x, fma(ui, pm.y, accv[uu][0][r]));
+                                accv[uu][1][r] = fma(vi, pm.x, fma(-ur, pm.y, accv[uu][1][r]));
+                            } else {
 #pragma unroll
                             for (int cd = 0; cd < NCTD; ++cd) {
                                 const T wu = cu[g * NCTD + cd][r], wv = cv[g * NCTD + cd][r];
                                 accu[uu][cd][r] = fma(wu, pm.x, fma(Mma<T>::swap_pair(wv), pm.y, accu[uu][cd][r]));
                                 accv[uu][cd][r] = fma(wv, pm.x, fma(Mma<T>::swap_pair(wu), pm.y, accv[uu][cd][r]));
+                            }
                             }
                         }
                     }
@@ -670,9 +726,9 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
 #pragma unroll
             for (int cd = 0; cd < NCTD; ++cd) {
                 const int col = cd * 16 + li;
-                const int t = tg[col >> 1];
+                const int t = tg[SPL ? li : col >> 1];
                 if (t >= 0) {
-                    double* dst = (double*)(a.L + a.exp_off[t]) + (col & 1);
+                    double* dst = (double*)(a.L + a.exp_off[t]) + (SPL ? cd : col & 1);
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int m = mt * 16 + Mma<T>::row(lk, r);

@@ -293,6 +293,32 @@ def sampled_error(positions, charges, k, phi, n_samp=256, seed=0):
     return float(np.linalg.norm(phi[idx] - ref) / max(np.linalg.norm(ref), 1e-300))
 
 
+def certified_degrees(net, positions, charges, k, n_max=16, tol=1e-6, margin=-0.25, n_samp=1024, headroom=2,
+                      max_steps=3, precision="mixed", max_depth=None):
+    """The network's per-level degrees (predict_p_level at `margin`), certified: one plan
+    with tables `headroom` degrees higher evaluates at the predicted degrees and is checked
+    against the FP64 direct sum on `n_samp` sampled targets; a miss raises every level by
+    one (evaluate-time degrees, no new setup), at most `max_steps` times -- the plain
+    fixed-degree climb as the network's judge.  Returns (degrees, error, climbs)."""
+    from .api import evaluate, setup
+    pl = predict_p_level(net, positions, charges, k, n_max=n_max, max_depth=max_depth, margin=margin)
+    tab = [min(P_CAP, v + headroom) for v in pl]
+    st = setup(positions, k, p=max(tab), N_max=n_max, p_level=tab, precision=precision, max_depth=max_depth)
+    try:
+        cur = list(pl)[:st.depth]
+        st.set_degrees(cur)
+        err = sampled_error(positions, charges, k, evaluate(st, charges), n_samp)
+        climbs = 0
+        while err > tol and climbs < max_steps:
+            cur = [min(c + 1, t) for c, t in zip(cur, st.p_table)]
+            st.set_degrees(cur)
+            err = sampled_error(positions, charges, k, evaluate(st, charges), n_samp)
+            climbs += 1
+        return cur, err, climbs
+    finally:
+        st.close()
+
+
 def evaluate_certified(tuned, plain, positions, charges, k, tol=1e-6, n_samp=256):
     """Evaluate with the tuned plan; if the sampled-target error misses `tol`, rerun
     with the plain fixed-degree plan.  Returns (phi, error, used_plain)."""
