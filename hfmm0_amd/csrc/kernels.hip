@@ -742,10 +742,19 @@ __device__ __forceinline__ void p2p_mixed_body(const P2PArgs& a, const int leaf,
     }
 }
 
+// Leaf of workgroup i of a launch over n leaves: workgroup i runs on XCD i % 8, and XCD x
+// takes the contiguous range of n / 8 (+1 for x < n % 8) leaves starting at
+// x (n / 8) + min(x, n % 8), so that its L2 holds one region's points (leaves are in Morton
+// order, and a leaf's sources are its neighbours).  A bijection of [0, n).
+__device__ inline int xcd_leaf(int i, int n) {
+    const int q = n / kXcds, r = n % kXcds, x = i % kXcds;
+    return x * q + min(x, r) + i / kXcds;
+}
+
 template <bool PH, bool LOWK, int R>
 __global__ __launch_bounds__(64) void p2p_mixed_kernel(P2PArgs a) {
     __shared__ P2PStage st;
-    p2p_mixed_body<PH, LOWK, R>(a, a.leaf0 + blockIdx.x, st);
+    p2p_mixed_body<PH, LOWK, R>(a, a.leaf0 + xcd_leaf(blockIdx.x, gridDim.x), st);
 }
 
 // FP64 near field (the plain form): one wave per target leaf; targets of a chunk of up to
@@ -820,7 +829,7 @@ __device__ __forceinline__ void p2p_fp64_body(const P2PArgs& a, const int leaf, 
 
 __global__ __launch_bounds__(64) void p2p_fp64_kernel(P2PArgs a) {
     __shared__ P2PStage st;
-    p2p_fp64_body(a, a.leaf0 + blockIdx.x, st);
+    p2p_fp64_body(a, a.leaf0 + xcd_leaf(blockIdx.x, gridDim.x), st);
 }
 
 // Persistent near field over the leaves of many problems (batches), one launch per leaf
