@@ -37,13 +37,29 @@ void launch_gather_charges(const void* q, int q_type, const int* perm, int64_t n
 // LDS bytes per staged point of the P2M kernel at degree p
 inline size_t p2m_bytes_per_point(int p) {
     const size_t P1 = p + 1;
-    return 2 * P1 * 16 + 2 * 16 + P1 * (P1 + 1) / 2 * 8;
+    return 24 * P1 + 32;
 }
-void launch_p2m(const int* leaves, int n_items, int chunk, DevBoxes bx, DevLevels lv, const double* X,
-                const double* Y, const double* Z, const double2* q, double k, int pmax, double2* M, hipStream_t s);
+// points staged per pass of the P2M kernel at the largest leaf degree pmax: the staged
+// points and the recurrence coefficients stay within 36 KB of LDS, so that four
+// workgroups fit on a CU
+int p2m_chunk(int pmax);
+// One leaf of the P2M and L2P launches, 64 bytes, read by 16 lanes with one vector load.
+struct LeafDesc {
+    int box, level, pstart, count;
+    int64_t exp_off;           // offset of the leaf's expansion
+    int64_t pad0;
+    double cx, cy, cz;         // box centre
+    double pad1;
+};
+static_assert(sizeof(LeafDesc) == 64, "LeafDesc is 16 words");
 
-void launch_l2p(const int* leaves, int n_items, DevBoxes bx, DevLevels lv, const double* X, const double* Y,
-                const double* Z, const double2* L, double k, int pmax, double2* far, hipStream_t s);
+// P2M and L2P run as persistent launches: a grid of resident workgroups, each taking the
+// leaves of a contiguous range on its XCD in turn (pmax: the largest leaf degree).
+void launch_p2m(const LeafDesc* leaves, int n_items, int chunk, DevLevels lv, const double* X, const double* Y,
+                const double* Z, const double2* q, double k, int pmax, double2* M, hipStream_t s);
+
+void launch_l2p(const LeafDesc* leaves, int n_items, DevLevels lv, const double* X, const double* Y, const double* Z,
+                const double2* L, double k, int pmax, double2* far, hipStream_t s);
 
 // Generic rotate / coaxial / rotate-back translation over a CSR pair list.
 // Target t = targets[blockIdx.x]; sources src[ptr[t_i] .. ptr[t_i+1]) with direction dir[].

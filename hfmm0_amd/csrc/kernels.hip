@@ -43,27 +43,36 @@ __device__ inline void nm_decode(int c, int& n, int& m) {
 //   P_m^m = -d_m sin(theta) P_{m-1}^{m-1},  d_m = sqrt((2m+1)/(2m))
 //   P_{m+1}^m = e_m cos(theta) P_m^m,       e_m = sqrt(2m+3)
 //   P_n^m = a_nm (cos(theta) P_{n-1}^m - b_nm P_{n-2}^m)
-// Also the downward-recurrence factors of jhat, miller[n] = 1 / ((2n+1)(2n+3)).
+// stored order-major, ab[m][n] = (a_nm, b_nm), so that the coefficients of one order are
+// contiguous; de[m] = (d_m, e_m).  Also the downward-recurrence factors of jhat,
+// miller[n] = 1 / ((2n+1)(2n+3)).  The P2M and L2P kernels copy the part their degree
+// needs into LDS once per workgroup: in the recurrence loops a constant-memory read at a
+// lane-varying index is a vector memory load, and even a uniform one waits on the scalar
+// cache, which this table (31 KB) does not fit.
 constexpr int kMillerN = 512;
+constexpr int kMillerLds = 128;   // factors staged in LDS (n < 128: k R below about 70 at p = 11)
 struct LegCoef {
-    double a[kMaxP + 1][kMaxP + 1];
-    double b[kMaxP + 1][kMaxP + 1];
-    double d[kMaxP + 1];
-    double e[kMaxP + 1];
+    double2 ab[kMaxP + 1][kMaxP + 1];
+    double2 de[kMaxP + 1];
+    double cm[kMaxP + 1];     // c_m = prod_{i <= m} (-d_i) / sqrt(4 pi): P_m^m = c_m sin^m(theta)
     double miller[kMillerN];
 };
 __constant__ LegCoef c_leg;
+
+__device__ __forceinline__ double miller_factor(const double* mil, int nmil, int n) {
+    return n < nmil ? mil[n] : 1.0 / ((2.0 * n + 1.0) * (2.0 * n + 3.0));
+}
 
 // jhat_n(x) for n = 0..p by Miller's downward recurrence
 //   jhat_{n-1} = jhat_n - x^2 jhat_{n+1} / ((2n+1)(2n+3)),
 // normalised with jhat_0 = sin(x)/x (or jhat_1 = 3(sin x - x cos x)/x^3 when that is
 // the larger one).  out[n * stride].  The recurrence starts at N = p + 1.5 xmax + 12 for
 // an upper bound xmax >= x common to the caller's workgroup (k R of the box), so every lane
-// walks the same n and the factor table is read at a uniform index.  Against SciPy's
-// spherical Bessel functions (p = 2..40, x = 1e-3..150) a start of p + 1.5 x + 8 already
-// reaches the rounding floor (7e-14 relative to the neighbouring orders' magnitude); + 6
-// gives 2e-13, + 4 gives 9e-11.
-__device__ void jhat_dev(double x, int p, double* out, int stride, double xmax) {
+// walks the same n and the factor table mil[0..nmil) is read at a uniform index.  Against
+// SciPy's spherical Bessel functions (p = 2..40, x = 1e-3..150) a start of p + 1.5 x + 8
+// already reaches the rounding floor (7e-14 relative to the neighbouring orders'
+// magnitude); + 6 gives 2e-13, + 4 gives 9e-11.
+__device__ void jhat_dev(double x, int p, double* out, int stride, double xmax, const double* mil, int nmil) {
     x = fabs(x);
     if (x == 0.0) {
         for (int n = 0; n <= p; ++n) out[n * stride] = 1.0;
@@ -73,8 +82,7 @@ __device__ void jhat_dev(double x, int p, double* out, int stride, double xmax) 
     double a_next = 0.0, a = 1.0;
     const double x2 = x * x;
     for (int n = N; n >= 1; --n) {
-        const double f = n < kMillerN ? c_leg.miller[n] : 1.0 / ((2.0 * n + 1.0) * (2.0 * n + 3.0));
-        const double am1 = a - x2 * a_next * f;
+        const double am1 = a - x2 * a_next * miller_factor(mil, nmil, n);
         a_next = a;
         a = am1;
         if (n - 1 <= p) out[(n - 1) * stride] = a;
@@ -98,6 +106,104 @@ __device__ void jhat_dev(double x, int p, double* out, int stride, double xmax) 
     for (int n = 0; n <= p; ++n) out[n * stride] *= scale;
 }
 
+// LDS copy of the recurrence coefficients of degree p: ab in order-major triangle layout
+// (the entries of order m, n = m..p, from leg_col(p + 1, m)), de and cm (p+1 each), and
+// the first kMillerLds Miller factors.
+__host__ __device__ constexpr int leg_col(int P1, int m) { return m * P1 - m * (m - 1) / 2; }
+__host__ __device__ constexpr int leg_lds_doubles(int p) { return (p + 1) * (p + 2) + 3 * (p + 1) + kMillerLds; }
+
+struct LegLds {
+    const double2* ab;    // ab[leg_col(p + 1, m) + n - m] = (a_nm, b_nm)
+    const double2* de;    // (d_m, e_m)
+    const double* cm;     // c_m
+    const double* mil;    // Miller factors, n < kMillerLds
+};
+
+__device__ LegLds stage_legendre(double* dst, int p, int tid, int nthreads) {
+    const int P1 = p + 1;
+    double2* ab = (double2*)dst;
+    double2* de = ab + P1 * (P1 + 1) / 2;
+    double* cm = (double*)(de + P1);
+    double* mil = cm + P1;
+    for (int t = tid; t < P1 * P1; t += nthreads) {
+        const int m = t / P1, n = t - m * P1;
+        if (n >= m) ab[leg_col(P1, m) + n - m] = c_leg.ab[m][n];
+    }
+    for (int t = tid; t < P1; t += nthreads) {
+        de[t] = c_leg.de[t];
+        cm[t] = c_leg.cm[t];
+    }
+    for (int t = tid; t < kMillerLds; t += nthreads) mil[t] = c_leg.miller[t];
+    return LegLds{ab, de, cm, mil};
+}
+
+// jhat_n(x), n = 0..p, up to one factor: out[n * stride] = jhat_n(x) / scale, with the
+// returned scale (x >= 0).  Miller's downward recurrence
+//   a_{n-1} = a_n - x^2 a_{n+1} / ((2n+1)(2n+3))
+// from a_N = 1, a_{N+1} = 0 with a start N common to the workgroup (uniform loop: the
+// caller gives N = p + 1.5 xmax + 13 for xmax >= x, as jhat_dev), normalised by
+// jhat_0 = sin(x)/x or jhat_1 = 3(sin x - x cos x)/x^3 (the larger).  The factors come from
+// mil[] (uniform LDS reads, four per wait) below kMillerLds.  x^2 f_n a_{n+1} depends on
+// the previous step only through a_{n+1}, so one add per step is on the dependency chain.
+// Values are brought back by 2^-830 when one passes 2^830 (where nothing is stored yet,
+// tested every 4 steps: a step grows them by a factor of at most 1 + x^2 / 15).
+__device__ __forceinline__ double miller_step_factor(const double* mil, int n) {
+    return n < kMillerLds ? mil[n] : 1.0 / ((2.0 * n + 1.0) * (2.0 * n + 3.0));
+}
+
+__device__ double jhat_unscaled(double x, int p, double* out, int stride, int N, const double* mil) {
+    if (x == 0.0) {
+        for (int n = 0; n <= p; ++n) out[n * stride] = 1.0;
+        return 1.0;
+    }
+    const double x2 = x * x;
+    double a_next = 0.0, a = 1.0;
+    int n = N;
+    // steps n = N .. p + 2: nothing is stored
+    for (; n - 3 >= p + 2; n -= 4) {
+        const double f0 = miller_step_factor(mil, n), f1 = miller_step_factor(mil, n - 1);
+        const double f2 = miller_step_factor(mil, n - 2), f3 = miller_step_factor(mil, n - 3);
+        double t = a - x2 * f0 * a_next;
+        a_next = a;
+        a = t;
+        t = a - x2 * f1 * a_next;
+        a_next = a;
+        a = t;
+        t = a - x2 * f2 * a_next;
+        a_next = a;
+        a = t;
+        t = a - x2 * f3 * a_next;
+        a_next = a;
+        a = t;
+        if (fabs(a) > 0x1p830) {
+            a *= 0x1p-830;
+            a_next *= 0x1p-830;
+        }
+    }
+    for (; n >= p + 2; --n) {
+        const double t = a - x2 * miller_step_factor(mil, n) * a_next;
+        a_next = a;
+        a = t;
+    }
+    // steps n = p + 1 .. 1 store a_{n-1}
+    for (; n >= 1; --n) {
+        const double t = a - x2 * miller_step_factor(mil, n) * a_next;
+        a_next = a;
+        a = t;
+        out[(n - 1) * stride] = a;
+        if (fabs(a) > 0x1p830) {
+            a *= 0x1p-830;
+            a_next *= 0x1p-830;
+            for (int j = n - 1; j <= p; ++j) out[j * stride] *= 0x1p-830;
+        }
+    }
+    // a = a_0, a_next = a_1
+    double sn, cs;
+    sincos(x, &sn, &cs);
+    if (x < 1.0) return sn / (x * a);
+    const double e0 = sn / x, e1 = 3.0 * (sn - x * cs) / (x * x * x);
+    return fabs(e0) >= fabs(e1) ? e0 / a : e1 / a_next;
+}
 
 // ---------------------------------------------------------------------------------
 // small utility kernels
@@ -129,178 +235,349 @@ __global__ void gather_charges_kernel(const void* q, int q_type, const int* perm
 }
 
 // ---------------------------------------------------------------------------------
-// P2M: one workgroup per leaf; points processed in chunks staged in LDS.
-//  1. one thread per point: offsets, angles and the radial factors
-//     q_j jhat_n(k rho_j) (rho_j / R)^n;
-//  2. one thread per (point j, order m): the normalised Legendre column
-//     P_n^m(cos beta_j), n = m..p, and e^{-i m alpha_j} (powers of e^{-i alpha_j});
-//  3. one thread per coefficient (n, m): accumulation over the chunk's points.
+// Persistent leaf loops (P2M, L2P).  The grid is a multiple of 8 resident workgroups;
+// workgroup i runs on XCD x = i % 8 and takes, in turn, the leaves l, l + g, l + 2g, ...
+// (l = i / 8, g = grid / 8) of XCD x's contiguous range of n / 8 (+1 for x < n % 8)
+// leaves, so that an XCD's L2 holds one region.  A leaf's descriptor is one 64-byte
+// vector load by 16 lanes whose words are read back with v_readlane; the next leaf's load
+// is issued before the current leaf is processed.
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void p2m_kernel(const int* leaves, int chunk, DevBoxes bx, DevLevels lv,
-                                                  const double* X, const double* Y, const double* Z,
-                                                  const double2* q, double k, double2* M) {
-    extern __shared__ double smem[];
-    const int b = leaves[blockIdx.x];
-    const int L = bx.level[b];
-    const int p = lv.p[L];
-    const double R = lv.R[L];
-    const int P1 = p + 1, tri = P1 * (P1 + 1) / 2, nc = P1 * P1;
-    double2* s_qrad = (double2*)smem;             // chunk x P1
-    double2* s_ph = s_qrad + chunk * P1;          // chunk x P1
-    double2* s_geo = s_ph + chunk * P1;           // chunk x 2: (cos b, sin b), (cos a, sin a)
-    double* s_leg = (double*)(s_geo + 2 * chunk); // chunk x tri
-    const double cx = bx.centre[3 * b], cy = bx.centre[3 * b + 1], cz = bx.centre[3 * b + 2];
-    const int ps = bx.pstart[b], cnt = bx.count[b];
-    double2 acc[kAccPerThread];
-#pragma unroll
-    for (int t = 0; t < kAccPerThread; ++t) acc[t] = make_double2(0.0, 0.0);
+struct LeafInfo {
+    int box, level, pstart, count;
+    int64_t exp_off;
+    double cx, cy, cz;
+};
 
-    for (int j0 = 0; j0 < cnt; j0 += chunk) {
-        const int nch = min(chunk, cnt - j0);
-        for (int j = threadIdx.x; j < nch; j += blockDim.x) {
-            const int i = ps + j0 + j;
-            const double dx = X[i] - cx, dy = Y[i] - cy, dz = Z[i] - cz;
-            const double rxy2 = dx * dx + dy * dy;
-            const double rho = sqrt(rxy2 + dz * dz), rxy = sqrt(rxy2);
-            s_geo[2 * j] = rho > 0.0 ? make_double2(dz / rho, rxy / rho) : make_double2(1.0, 0.0);
-            s_geo[2 * j + 1] = rxy > 0.0 ? make_double2(dx / rxy, dy / rxy) : make_double2(1.0, 0.0);
-            double* rad = &s_qrad[j * P1].x;
-            jhat_dev(k * rho, p, rad, 2, k * R);
-            const double2 qq = q[i];
-            const double rr = rho / R;
-            double pw = 1.0;
-            for (int n = 0; n <= p; ++n) {
-                const double g = rad[2 * n] * pw;
-                s_qrad[j * P1 + n] = make_double2(qq.x * g, qq.y * g);
-                pw *= rr;
+__device__ __forceinline__ int leaf_desc_word(const LeafDesc* d, int it) {
+    return ((const int*)(d + it))[threadIdx.x & 15];
+}
+
+__device__ __forceinline__ uint64_t readlane64(int w, int lo) {
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(w, lo + 1) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane(w, lo);
+}
+
+__device__ __forceinline__ LeafInfo leaf_desc_unpack(int w) {
+    LeafInfo f;
+    f.box = __builtin_amdgcn_readlane(w, 0);
+    f.level = __builtin_amdgcn_readlane(w, 1);
+    f.pstart = __builtin_amdgcn_readlane(w, 2);
+    f.count = __builtin_amdgcn_readlane(w, 3);
+    f.exp_off = (int64_t)readlane64(w, 4);
+    f.cx = __builtin_bit_cast(double, readlane64(w, 8));
+    f.cy = __builtin_bit_cast(double, readlane64(w, 10));
+    f.cz = __builtin_bit_cast(double, readlane64(w, 12));
+    return f;
+}
+
+struct LeafRange {
+    int it, end, step;
+};
+
+__device__ __forceinline__ LeafRange leaf_range(int n) {
+    const int g = gridDim.x / kXcds, x = blockIdx.x % kXcds, l = blockIdx.x / kXcds;
+    const int q = n / kXcds, r = n % kXcds;
+    const int beg = x * q + min(x, r);
+    return LeafRange{beg + l, beg + q + (x < r ? 1 : 0), g};
+}
+
+// ---------------------------------------------------------------------------------
+// P2M and L2P in solid-harmonic form.  With (x, y, z) a point relative to the box centre,
+// rho its distance and R the expansion radius,
+//   (rho/R)^n P_n^m(cos theta) e^{i m alpha} = Z_m T_n^m,
+//   Z_m = c_m ((x + i y)/R)^m,   T_m^m = 1,  T_{m+1}^m = e_m z/R,
+//   T_n^m = a_nm (z/R T_{n-1}^m - b_nm (rho/R)^2 T_{n-2}^m),
+// the Legendre recurrence multiplied by (rho/R)^n: no angles and no divisions per point.
+// jhat_n(k rho) comes from jhat_unscaled, whose factor joins the charge (P2M) or the sum
+// (L2P).
+// ---------------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------------
+// P2M: a workgroup per leaf at a time; points processed in chunks of up to 64 in LDS.
+//  0. the recurrence coefficients of the largest leaf degree are copied into LDS once;
+//  1. one thread per point: jhat_n(k rho) up to a factor, which joins the charge
+//     (q' = q scale), z/R, (rho/R)^2, and Z_m for m = 0..p;
+//  2. one thread per (order m, block of up to 4 degrees n) and point group g (points
+//     j = g mod G): per point T_n^m from T_m^m = 1, and for each (n, m) of the block, with
+//     w_j = q'_j jhat_n T_n^m, the sums U = sum_j w_j Re Z_m and V = sum_j w_j Im Z_m,
+//     which give both orders: Mh_n^m = U - iV and Mh_n^-m = (-1)^m (U + iV), since
+//     conj(Y_n^-m) = (-1)^m Y_n^m.  The G groups are added at the end of the leaf.
+// ---------------------------------------------------------------------------------
+constexpr int kP2MThreads = 256;
+constexpr int kP2MBlockN = 4;     // degrees per thread block of an order
+
+__host__ __device__ constexpr int p2m_blocks(int p) {
+    int nb = 0;
+    for (int m = 0; m <= p; ++m) nb += (p + 1 - m + kP2MBlockN - 1) / kP2MBlockN;
+    return nb;
+}
+// point groups: blocks x groups fill the workgroup, at most 16
+__host__ __device__ constexpr int p2m_groups(int p) {
+    return kP2MThreads / p2m_blocks(p) < 16 ? (kP2MThreads / p2m_blocks(p) > 0 ? kP2MThreads / p2m_blocks(p) : 1) : 16;
+}
+
+// Miller start for a leaf (uniform): p + 1.5 k R + 13
+__device__ __forceinline__ int miller_start(int p, double kR) { return p + (int)(1.5 * fabs(kR)) + 13; }
+
+// held to 4 waves per SIMD (128 VGPRs), as many as LDS allows
+__global__ __launch_bounds__(kP2MThreads) __attribute__((amdgpu_waves_per_eu(4))) void p2m_kernel(
+    const LeafDesc* __restrict__ leaves, int n_items, int chunk, int pmax, DevLevels lv, const double* X,
+    const double* Y, const double* Z, const double2* q, double k, double2* M) {
+    extern __shared__ double smem[];
+    const int tid = threadIdx.x;
+    const int PM1 = pmax + 1;
+    const LegLds lg = stage_legendre(smem, pmax, tid, kP2MThreads);
+    double* region = smem + leg_lds_doubles(pmax);
+    LeafRange lr = leaf_range(n_items);
+    int w = lr.it < lr.end ? leaf_desc_word(leaves, lr.it) : 0;
+    __syncthreads();
+    for (; lr.it < lr.end; lr.it += lr.step) {
+        const LeafInfo f = leaf_desc_unpack(w);
+        if (lr.it + lr.step < lr.end) w = leaf_desc_word(leaves, lr.it + lr.step);
+        const int p = lv.p[f.level];
+        const double R = lv.R[f.level];
+        const double invR = 1.0 / R;
+        const int N = miller_start(p, k * R);
+        const int P1 = p + 1;
+        double2* s_z = (double2*)region;              // chunk x P1: Z_m
+        double2* s_q = s_z + chunk * P1;              // chunk: q' = q scale
+        double2* s_g = s_q + chunk;                   // chunk: (z/R, (rho/R)^2)
+        double* s_j = (double*)(s_g + chunk);         // chunk x P1: jhat_n / scale
+        // this thread's block: order m, degrees n0..n1, point group g (g >= G: idle)
+        const int NB = p2m_blocks(p), G = p2m_groups(p);
+        const int bi = tid % NB, g = tid / NB;
+        int m = 0, n0 = 0;
+        for (int r = bi;; ++m) {
+            const int nbm = (p + 1 - m + kP2MBlockN - 1) / kP2MBlockN;
+            if (r < nbm) {
+                n0 = m + r * kP2MBlockN;
+                break;
             }
+            r -= nbm;
         }
-        __syncthreads();
-        for (int w = threadIdx.x; w < nch * P1; w += blockDim.x) {
-            const int j = w / P1, m = w - j * P1;
-            const double2 cb = s_geo[2 * j], ea = s_geo[2 * j + 1];
-            const double ct = cb.x, st = cb.y;
-            double pmm = 0.28209479177387814;   // 1/sqrt(4 pi)
-            double2 z = make_double2(1.0, 0.0);
-            for (int i2 = 1; i2 <= m; ++i2) {
-                pmm *= -c_leg.d[i2] * st;
-                z = cmul(z, ea);
-            }
-            s_ph[j * P1 + m] = make_double2(z.x, -z.y);
-            double* Pl = s_leg + j * tri;
-            Pl[m * (m + 1) / 2 + m] = pmm;
-            if (m < p) {
-                double p2 = pmm, p1 = c_leg.e[m] * ct * pmm;
-                Pl[(m + 1) * (m + 2) / 2 + m] = p1;
-                for (int n = m + 2; n <= p; ++n) {
-                    const double pn = c_leg.a[n][m] * (ct * p1 - c_leg.b[n][m] * p2);
-                    Pl[n * (n + 1) / 2 + m] = pn;
-                    p2 = p1;
-                    p1 = pn;
+        const int n1 = min(p, n0 + kP2MBlockN - 1);
+        const bool active = g < G;
+        const double2* abm = lg.ab + leg_col(PM1, m) - m;   // (a_nm, b_nm) at abm[n], n >= m + 2
+        double2 U[kP2MBlockN], V[kP2MBlockN];
+#pragma unroll
+        for (int s = 0; s < kP2MBlockN; ++s) U[s] = V[s] = make_double2(0.0, 0.0);
+
+        for (int j0 = 0; j0 < f.count; j0 += chunk) {
+            const int nch = min(chunk, f.count - j0);
+            for (int j = tid; j < nch; j += kP2MThreads) {
+                const int i = f.pstart + j0 + j;
+                const double dx = X[i] - f.cx, dy = Y[i] - f.cy, dz = Z[i] - f.cz;
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double scale = jhat_unscaled(k * sqrt(r2), p, s_j + j * P1, 1, N, lg.mil);
+                const double2 qq = q[i];
+                s_q[j] = make_double2(qq.x * scale, qq.y * scale);
+                s_g[j] = make_double2(dz * invR, r2 * invR * invR);
+                const double2 u = make_double2(dx * invR, dy * invR);
+                double2 um = make_double2(1.0, 0.0);   // ((x + i y)/R)^m
+                s_z[j * P1] = make_double2(lg.cm[0], 0.0);
+                for (int mm = 1; mm <= p; ++mm) {
+                    um = cmul(um, u);
+                    const double c = lg.cm[mm];
+                    s_z[j * P1 + mm] = make_double2(c * um.x, c * um.y);
                 }
             }
-        }
-        __syncthreads();
+            __syncthreads();
+            if (active) {
+                const double e_m = lg.de[m].y;
+                // two points per pass: two independent recurrences and their LDS reads
+                // interleave (a missing second point repeats the first with weight 0)
+                for (int j = g; j < nch; j += 2 * G) {
+                    const bool two = j + G < nch;
+                    const int j2 = two ? j + G : j;
+                    const double2 zA = s_z[j * P1 + m], zB = s_z[j2 * P1 + m];
+                    const double2 qA = s_q[j], qB = two ? s_q[j2] : make_double2(0.0, 0.0);
+                    const double2 gA = s_g[j], gB = s_g[j2];
+                    const double* jA = s_j + j * P1;
+                    const double* jB = s_j + j2 * P1;
+                    // T_n^m at n = n0 (t1) and n0 - 1 (t2)
+                    double a1 = 1.0, a2 = 0.0, b1 = 1.0, b2 = 0.0;
+                    for (int n = m + 1; n <= n0; ++n) {
+                        const double2 cf = n == m + 1 ? make_double2(e_m, 0.0) : abm[n];
+                        const double ta = cf.x * (gA.x * a1 - cf.y * gA.y * a2);
+                        const double tb = cf.x * (gB.x * b1 - cf.y * gB.y * b2);
+                        a2 = a1;
+                        a1 = ta;
+                        b2 = b1;
+                        b1 = tb;
+                    }
 #pragma unroll
-        for (int t = 0; t < kAccPerThread; ++t) {
-            const int c = threadIdx.x + t * blockDim.x;
-            if (c < nc) {
-                int n, m;
-                nm_decode(c, n, m);
-                const int am = m < 0 ? -m : m;
-                const double sg = (m < 0 && (am & 1)) ? -1.0 : 1.0;
-                const double sy = m < 0 ? -1.0 : 1.0;
-                double2 a = acc[t];
-                for (int j = 0; j < nch; ++j) {
-                    const double2 qr = s_qrad[j * P1 + n];
-                    const double pl = sg * s_leg[j * tri + n * (n + 1) / 2 + am];
-                    const double2 ph = s_ph[j * P1 + am];
-                    a = cmadd(a, qr, make_double2(pl * ph.x, pl * sy * ph.y));
+                    for (int s = 0; s < kP2MBlockN; ++s) {
+                        const int n = n0 + s;
+                        if (n > n1) break;
+                        if (s > 0) {
+                            const double2 cf = n == m + 1 ? make_double2(e_m, 0.0) : abm[n];
+                            const double ta = cf.x * (gA.x * a1 - cf.y * gA.y * a2);
+                            const double tb = cf.x * (gB.x * b1 - cf.y * gB.y * b2);
+                            a2 = a1;
+                            a1 = ta;
+                            b2 = b1;
+                            b1 = tb;
+                        }
+                        const double hA = jA[n] * a1, hB = jB[n] * b1;
+                        const double wrA = qA.x * hA, wiA = qA.y * hA, wrB = qB.x * hB, wiB = qB.y * hB;
+                        U[s].x = fma(wrA, zA.x, fma(wrB, zB.x, U[s].x));
+                        U[s].y = fma(wiA, zA.x, fma(wiB, zB.x, U[s].y));
+                        V[s].x = fma(wrA, zA.y, fma(wrB, zB.y, V[s].x));
+                        V[s].y = fma(wiA, zA.y, fma(wiB, zB.y, V[s].y));
+                    }
                 }
-                acc[t] = a;
+            }
+            __syncthreads();
+        }
+        // the groups' partial sums meet in LDS (the chunk region is free now); group 0 writes
+        if (G > 1) {
+            double2* red = (double2*)region;   // (G - 1) x NB x kP2MBlockN x (U, V)
+            if (g >= 1 && active)
+#pragma unroll
+                for (int s = 0; s < kP2MBlockN; ++s) {
+                    const int o = 2 * (((g - 1) * NB + bi) * kP2MBlockN + s);
+                    red[o] = U[s];
+                    red[o + 1] = V[s];
+                }
+            __syncthreads();
+            if (g == 0)
+                for (int h = 1; h < G; ++h)
+#pragma unroll
+                    for (int s = 0; s < kP2MBlockN; ++s) {
+                        const int o = 2 * (((h - 1) * NB + bi) * kP2MBlockN + s);
+                        const double2 u = red[o], v = red[o + 1];
+                        U[s].x += u.x;
+                        U[s].y += u.y;
+                        V[s].x += v.x;
+                        V[s].y += v.y;
+                    }
+            __syncthreads();   // the next leaf's chunk overwrites red
+        }
+        if (g == 0) {
+            double2* out = M + f.exp_off;
+            const double sg = (m & 1) ? -1.0 : 1.0;
+#pragma unroll
+            for (int s = 0; s < kP2MBlockN; ++s) {
+                const int n = n0 + s;
+                if (n > n1) break;
+                const double2 u = U[s], v = V[s];
+                out[n * n + n + m] = make_double2(u.x + v.y, u.y - v.x);
+                if (m > 0) out[n * n + n - m] = make_double2(sg * (u.x - v.y), sg * (u.y + v.x));
             }
         }
-        __syncthreads();
-    }
-    double2* out = M + bx.exp_off[b];
-#pragma unroll
-    for (int t = 0; t < kAccPerThread; ++t) {
-        const int c = threadIdx.x + t * blockDim.x;
-        if (c < nc) out[c] = acc[t];
     }
 }
 
 // ---------------------------------------------------------------------------------
-// L2P: one workgroup (one wave) per leaf, one thread per target point.
+// L2P: a workgroup (one wave) per leaf at a time.  The recurrence coefficients (once) and
+// the leaf's local expansion, recombined per (n, m >= 0) as
+//   A_nm = Lh_n^m + (-1)^m Lh_n^-m,  B_nm = Lh_n^m - (-1)^m Lh_n^-m  (B_n0 = 0),
+// are staged in LDS (Y_n^-m = (-1)^m conj(Y_n^m)).  A leaf of at most 16 (32) points
+// gives each point 4 (2) lanes, which split the orders m between them (zig-zag, so that
+// the parts get about equal numbers of (n, m) terms) and add their results by shuffles.
+// For each of its orders m a lane sums S = sum_n j_n T_n^m A_nm and T = sum_n j_n T_n^m B_nm
+// (j_n = jhat_n(k rho) / scale, kept in LDS as [n][lane]) and adds
+// S Re Z_m + i T Im Z_m; the sum is multiplied by the scale at the end.
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void l2p_kernel(const int* leaves, DevBoxes bx, DevLevels lv, const double* X,
-                                                 const double* Y, const double* Z, const double2* Lexp, double k,
-                                                 double2* far) {
+__host__ __device__ constexpr int l2p_lds_doubles(int p) {
+    return leg_lds_doubles(p) + 2 * (p + 1) * (p + 2) + 64 * (p + 1);
+}
+
+__global__ __launch_bounds__(64) void l2p_kernel(const LeafDesc* __restrict__ leaves, int n_items, int pmax,
+                                                 DevLevels lv, const double* X, const double* Y, const double* Z,
+                                                 const double2* Lexp, double k, double2* far) {
     extern __shared__ double smem[];
-    const int b = leaves[blockIdx.x];
-    const int L = bx.level[b];
-    const int p = lv.p[L];
-    const double R = lv.R[L];
-    const int P1 = p + 1, nc = P1 * P1;
-    double2* s_L = (double2*)smem;
-    double* s_rad = (double*)(s_L + nc);
-    const double2* src = Lexp + bx.exp_off[b];
-    for (int c = threadIdx.x; c < nc; c += blockDim.x) s_L[c] = src[c];
-    __syncthreads();
-    const double cx = bx.centre[3 * b], cy = bx.centre[3 * b + 1], cz = bx.centre[3 * b + 2];
-    const int ps = bx.pstart[b], cnt = bx.count[b];
-    double* rad = s_rad + threadIdx.x * P1;
-    for (int i0 = 0; i0 < cnt; i0 += blockDim.x) {
-        const int il = i0 + threadIdx.x;
-        if (il >= cnt) break;
-        const int i = ps + il;
-        const double dx = X[i] - cx, dy = Y[i] - cy, dz = Z[i] - cz;
-        const double r = sqrt(dx * dx + dy * dy + dz * dz);
-        const double rxy = sqrt(dx * dx + dy * dy);
-        const double ct = r > 0.0 ? dz / r : 1.0;
-        const double st = r > 0.0 ? rxy / r : 0.0;
-        const double2 eia = rxy > 0.0 ? make_double2(dx / rxy, dy / rxy) : make_double2(1.0, 0.0);
-        jhat_dev(k * r, p, rad, 1, k * R);
-        const double rr = r / R;
-        double pw = 1.0;
-        for (int n = 0; n <= p; ++n) {
-            rad[n] *= pw;
-            pw *= rr;
-        }
-        double2 phi = make_double2(0.0, 0.0);
-        double pmm = 0.28209479177387814;
-        double2 eim = make_double2(1.0, 0.0);
-        for (int m = 0; m <= p; ++m) {
+    const int lane = threadIdx.x;
+    const int PM1 = pmax + 1;
+    const LegLds lg = stage_legendre(smem, pmax, lane, 64);
+    double2* s_AB = (double2*)(smem + leg_lds_doubles(pmax));    // (A, B) per (m, n), order-major triangle
+    double* s_rad = (double*)(s_AB + PM1 * (PM1 + 1));           // [n][lane]
+    double* rad = s_rad + lane;
+    LeafRange lr = leaf_range(n_items);
+    int w = lr.it < lr.end ? leaf_desc_word(leaves, lr.it) : 0;
+    for (; lr.it < lr.end; lr.it += lr.step) {
+        const LeafInfo f = leaf_desc_unpack(w);
+        if (lr.it + lr.step < lr.end) w = leaf_desc_word(leaves, lr.it + lr.step);
+        const int p = lv.p[f.level];
+        const double R = lv.R[f.level];
+        const double invR = 1.0 / R;
+        const int N = miller_start(p, k * R);
+        const int P1 = p + 1;
+        const double2* src = Lexp + f.exp_off;
+        __syncthreads();   // the previous leaf's reads of s_AB are done (and the staging at the start)
+        for (int t = lane; t < P1 * P1; t += 64) {
+            const int m = t / P1, n = t - m * P1;
+            if (n < m) continue;
+            const double2 lp = src[n * n + n + m];
+            double2 A = lp, B = make_double2(0.0, 0.0);
             if (m > 0) {
-                pmm *= -c_leg.d[m] * st;
-                eim = cmul(eim, eia);
+                const double2 lm = src[n * n + n - m];
+                const double sg = (m & 1) ? -1.0 : 1.0;
+                A = make_double2(lp.x + sg * lm.x, lp.y + sg * lm.y);
+                B = make_double2(lp.x - sg * lm.x, lp.y - sg * lm.y);
             }
-            const double sgn = (m & 1) ? -1.0 : 1.0;
-            double p2 = 0.0, p1 = 0.0;
-            for (int n = m; n <= p; ++n) {
-                double pn;
-                if (n == m)
-                    pn = pmm;
-                else if (n == m + 1)
-                    pn = c_leg.e[m] * ct * pmm;
-                else
-                    pn = c_leg.a[n][m] * (ct * p1 - c_leg.b[n][m] * p2);
-                p2 = p1;
-                p1 = pn;
-                const double g = rad[n] * pn;
-                const double2 lp = s_L[n * n + n + m];
-                double2 term = cmul(lp, eim);
-                if (m > 0) {
-                    const double2 lm = s_L[n * n + n - m];
-                    const double2 t2 = cmul(lm, conjd(eim));
-                    term.x += sgn * t2.x;
-                    term.y += sgn * t2.y;
-                }
-                phi.x = fma(g, term.x, phi.x);
-                phi.y = fma(g, term.y, phi.y);
-            }
+            const int c = leg_col(P1, m) + n - m;
+            s_AB[2 * c] = A;
+            s_AB[2 * c + 1] = B;
         }
-        far[i] = phi;
+        __syncthreads();
+        const int cnt = f.count;
+        const int S = cnt <= 16 ? 4 : (cnt <= 32 ? 2 : 1);   // lanes per point
+        const int W = 64 / S;                                // points per pass
+        const int part = lane / W;
+        for (int i0 = 0; i0 < cnt; i0 += W) {
+            const int il = i0 + lane % W;
+            const bool valid = il < cnt;
+            const int i = f.pstart + (valid ? il : cnt - 1);
+            const double dx = X[i] - f.cx, dy = Y[i] - f.cy, dz = Z[i] - f.cz;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double scale = jhat_unscaled(k * sqrt(r2), p, rad, 64, N, lg.mil);
+            const double zr = dz * invR, rr2 = r2 * invR * invR;
+            const double2 u = make_double2(dx * invR, dy * invR);
+            double2 phi = make_double2(0.0, 0.0);
+            double2 um = make_double2(1.0, 0.0);   // ((x + i y)/R)^m
+            for (int m = 0; m <= p; ++m) {
+                if (m > 0) um = cmul(um, u);
+                const int mr = m % S;
+                if (((m / S) & 1 ? S - 1 - mr : mr) != part) continue;
+                const double2* abm = lg.ab + leg_col(PM1, m) - m;     // (a_nm, b_nm) at abm[n]
+                const double2* ABm = s_AB + 2 * (leg_col(P1, m) - m);  // (A_nm, B_nm) at ABm[2n], ABm[2n + 1]
+                double h = rad[64 * m];
+                double2 Sv = make_double2(h * ABm[2 * m].x, h * ABm[2 * m].y);
+                double2 Tv = make_double2(h * ABm[2 * m + 1].x, h * ABm[2 * m + 1].y);
+                if (m < p) {
+                    double t2 = 1.0, t1 = lg.de[m].y * zr;
+                    h = rad[64 * (m + 1)] * t1;
+                    Sv.x = fma(h, ABm[2 * m + 2].x, Sv.x);
+                    Sv.y = fma(h, ABm[2 * m + 2].y, Sv.y);
+                    Tv.x = fma(h, ABm[2 * m + 3].x, Tv.x);
+                    Tv.y = fma(h, ABm[2 * m + 3].y, Tv.y);
+#pragma unroll 2
+                    for (int n = m + 2; n <= p; ++n) {
+                        const double2 cf = abm[n];
+                        const double tn = cf.x * (zr * t1 - cf.y * rr2 * t2);
+                        t2 = t1;
+                        t1 = tn;
+                        h = rad[64 * n] * tn;
+                        const double2 A = ABm[2 * n], B = ABm[2 * n + 1];
+                        Sv.x = fma(h, A.x, Sv.x);
+                        Sv.y = fma(h, A.y, Sv.y);
+                        Tv.x = fma(h, B.x, Tv.x);
+                        Tv.y = fma(h, B.y, Tv.y);
+                    }
+                }
+                const double c = lg.cm[m];
+                const double zx = c * um.x, zy = c * um.y;
+                phi.x += Sv.x * zx - Tv.y * zy;
+                phi.y += Sv.y * zx + Tv.x * zy;
+            }
+            // the parts of a point are W lanes apart
+            for (int off = W; off < 64; off <<= 1) {
+                phi.x += __shfl_xor(phi.x, off);
+                phi.y += __shfl_xor(phi.y, off);
+            }
+            if (valid && part == 0) far[f.pstart + il] = make_double2(phi.x * scale, phi.y * scale);
+        }
     }
 }
 
@@ -382,7 +659,7 @@ __global__ __launch_bounds__(64) void l2p_field_kernel(const int* leaves, DevBox
         const double ct = r > 0.0 ? dz / r : 1.0;
         const double st = r > 0.0 ? rxy / r : 0.0;
         const double2 eia = rxy > 0.0 ? make_double2(dx / rxy, dy / rxy) : make_double2(1.0, 0.0);
-        jhat_dev(k * r, p + 1, rad, 1, k * R);
+        jhat_dev(k * r, p + 1, rad, 1, k * R, c_leg.miller, kMillerN);
         const double rr = r / R;
         double pw = 1.0;
         for (int n = 0; n <= p + 1; ++n) {
@@ -395,7 +672,7 @@ __global__ __launch_bounds__(64) void l2p_field_kernel(const int* leaves, DevBox
         double2 eim = make_double2(1.0, 0.0);
         for (int m = 0; m <= p + 1; ++m) {
             if (m > 0) {
-                pmm *= -c_leg.d[m] * st;
+                pmm *= -c_leg.de[m].x * st;
                 eim = cmul(eim, eia);
             }
             const double sgn = (m & 1) ? -1.0 : 1.0;
@@ -405,9 +682,9 @@ __global__ __launch_bounds__(64) void l2p_field_kernel(const int* leaves, DevBox
                 if (n == m)
                     pn = pmm;
                 else if (n == m + 1)
-                    pn = c_leg.e[m] * ct * pmm;
+                    pn = c_leg.de[m].y * ct * pmm;
                 else
-                    pn = c_leg.a[n][m] * (ct * p1 - c_leg.b[n][m] * p2);
+                    pn = c_leg.ab[m][n].x * (ct * p1 - c_leg.ab[m][n].y * p2);
                 p2 = p1;
                 p1 = pn;
                 const double g = rad[n] * pn;
@@ -1033,13 +1310,15 @@ void upload_legendre_coefficients() {
     LegCoef h{};
     for (int n = 0; n < kMillerN; ++n) h.miller[n] = 1.0 / ((2.0 * n + 1.0) * (2.0 * n + 3.0));
     for (int m = 0; m <= kMaxP; ++m) {
-        h.d[m] = m > 0 ? std::sqrt((2.0 * m + 1.0) / (2.0 * m)) : 0.0;
-        h.e[m] = std::sqrt(2.0 * m + 3.0);
+        h.de[m] = make_double2(m > 0 ? std::sqrt((2.0 * m + 1.0) / (2.0 * m)) : 0.0, std::sqrt(2.0 * m + 3.0));
+        h.cm[m] = m > 0 ? -h.de[m].x * h.cm[m - 1] : 0.28209479177387814;   // 1/sqrt(4 pi)
         for (int n = 0; n <= kMaxP; ++n) {
-            if (n >= m + 2) {
-                h.a[n][m] = std::sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m));
-                h.b[n][m] = std::sqrt(((n - 1.0) * (n - 1.0) - (double)m * m) / (4.0 * (n - 1.0) * (n - 1.0) - 1.0));
-            }
+            if (n >= m + 2)
+                h.ab[m][n] = make_double2(
+                    std::sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m)),
+                    std::sqrt(((n - 1.0) * (n - 1.0) - (double)m * m) / (4.0 * (n - 1.0) * (n - 1.0) - 1.0)));
+            else
+                h.ab[m][n] = make_double2(0.0, 0.0);
         }
     }
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_leg), &h, sizeof(h)) != hipSuccess)
@@ -1066,22 +1345,48 @@ void launch_gather_charges(const void* q, int q_type, const int* perm, int64_t n
     check_launch("gather_charges");
 }
 
-void launch_p2m(const int* leaves, int n_items, int chunk, DevBoxes bx, DevLevels lv, const double* X,
-                const double* Y, const double* Z, const double2* q, double k, int pmax, double2* M, hipStream_t s) {
+int p2m_chunk(int pmax) {
+    const long budget = 36 * 1024 - (long)leg_lds_doubles(pmax) * (long)sizeof(double);
+    return (int)std::max<long>(1, std::min<long>(64, budget / (long)p2m_bytes_per_point(pmax)));
+}
+
+size_t p2m_lds_bytes(int pmax, int chunk) {
+    // the chunk region also holds the point groups' partial sums at the end of a leaf:
+    // (G - 1) x blocks x 4 x 32 bytes, with G x blocks <= 256
+    return (size_t)leg_lds_doubles(pmax) * sizeof(double) +
+           std::max<size_t>((size_t)chunk * p2m_bytes_per_point(pmax),
+                            (size_t)kP2MThreads * kP2MBlockN * 2 * sizeof(double2));
+}
+
+// resident workgroups of a persistent leaf loop: the occupancy of (fn, threads, shm) on
+// every CU, a multiple of kXcds, at most the leaves rounded up to kXcds
+static int leaf_loop_grid(const void* fn, int threads, size_t shm, int n_items) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, shm))
+        fail(-3, "occupancy query (leaf loop) failed");
+    const int full = std::max(kXcds, std::max(1, per_cu) * cus / kXcds * kXcds);
+    return std::min(full, (n_items + kXcds - 1) / kXcds * kXcds);
+}
+
+void launch_p2m(const LeafDesc* leaves, int n_items, int chunk, DevLevels lv, const double* X, const double* Y,
+                const double* Z, const double2* q, double k, int pmax, double2* M, hipStream_t s) {
     if (n_items <= 0) return;
     upload_legendre_coefficients();
-    const size_t shm = (size_t)chunk * p2m_bytes_per_point(pmax);
-    hipLaunchKernelGGL(p2m_kernel, dim3(n_items), dim3(256), shm, s, leaves, chunk, bx, lv, X, Y, Z, q, k, M);
+    const size_t shm = p2m_lds_bytes(pmax, chunk);
+    const int grid = leaf_loop_grid((const void*)p2m_kernel, kP2MThreads, shm, n_items);
+    hipLaunchKernelGGL(p2m_kernel, dim3(grid), dim3(kP2MThreads), shm, s, leaves, n_items, chunk, pmax, lv, X, Y, Z,
+                       q, k, M);
     check_launch("p2m");
 }
 
-void launch_l2p(const int* leaves, int n_items, DevBoxes bx, DevLevels lv, const double* X, const double* Y,
-                const double* Z, const double2* L, double k, int pmax, double2* far, hipStream_t s) {
+void launch_l2p(const LeafDesc* leaves, int n_items, DevLevels lv, const double* X, const double* Y, const double* Z,
+                const double2* L, double k, int pmax, double2* far, hipStream_t s) {
     if (n_items <= 0) return;
     upload_legendre_coefficients();
-    const int P1 = pmax + 1;
-    const size_t shm = (size_t)P1 * P1 * sizeof(double2) + 64 * P1 * sizeof(double);
-    hipLaunchKernelGGL(l2p_kernel, dim3(n_items), dim3(64), shm, s, leaves, bx, lv, X, Y, Z, L, k, far);
+    const size_t shm = (size_t)l2p_lds_doubles(pmax) * sizeof(double);
+    const int grid = leaf_loop_grid((const void*)l2p_kernel, 64, shm, n_items);
+    hipLaunchKernelGGL(l2p_kernel, dim3(grid), dim3(64), shm, s, leaves, n_items, pmax, lv, X, Y, Z, L, k, far);
     check_launch("l2p");
 }
 

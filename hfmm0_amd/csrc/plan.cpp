@@ -117,6 +117,7 @@ struct hfmm_plan {
     int* d_plevel = nullptr;
     double* d_R = nullptr;
     int* d_far_leaves = nullptr;
+    LeafDesc* d_leafdesc = nullptr;   // per far leaf (d_far_leaves order): P2M / L2P descriptors
     int *d_leaf_box = nullptr, *d_rng_ptr = nullptr, *d_rng_start = nullptr, *d_rng_pref = nullptr,
         *d_leaf_src_total = nullptr;
     double* d_rot = nullptr;
@@ -924,12 +925,31 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
     P.d_plevel = P.upload(pl);
     P.d_R = P.upload(rl);
     build_p2p(P);
+    {
+        // P2M / L2P descriptors of the far leaves, in the order of d_far_leaves
+        std::vector<LeafDesc> ld;
+        for (int b = 0; b < nb; ++b)
+            if (t.is_leaf[b] && t.level[b] >= 2) {
+                LeafDesc d{};
+                d.box = b;
+                d.level = t.level[b];
+                d.pstart = t.pstart[b];
+                d.count = t.count[b];
+                d.exp_off = P.exp_off[b];
+                d.cx = centre[3 * b];
+                d.cy = centre[3 * b + 1];
+                d.cz = centre[3 * b + 2];
+                ld.push_back(d);
+            }
+        if ((int)ld.size() != P.n_far_leaves) fail(HFMM_ERR_INTERNAL, "far leaf descriptors out of step");
+        P.d_leafdesc = P.upload(ld);
+    }
     build_tables(P);
-    // P2M: points staged per pass so that LDS stays within 48 KB at the largest leaf degree
+    // P2M: points staged per pass, sized for the largest leaf degree
     P.p_leafmax = 2;
     for (int b = 0; b < t.n_boxes(); ++b)
         if (t.is_leaf[b]) P.p_leafmax = std::max(P.p_leafmax, P.p_level[t.level[b]]);
-    P.p2m_chunk = (int)std::max<size_t>(1, std::min<size_t>(64, 48 * 1024 / p2m_bytes_per_point(P.p_leafmax)));
+    P.p2m_chunk = p2m_chunk(P.p_leafmax);
     P.dM = P.alloc<double2>(P.n_coef);
     bool any_fp32 = false;
     for (int L = 0; L < kMaxLevels; ++L) any_fp32 = any_fp32 || (P.m2lb[L].use && P.m2lb[L].fp32);
@@ -1053,8 +1073,7 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
         launch_fill_zero(P.dL, P.n_coef, s);
         launch_fill_zero(P.d_far, P.n, s);
         mark(1);
-        launch_p2m(P.d_far_leaves, P.n_far_leaves, P.p2m_chunk, bx, lv, P.dX, P.dY, P.dZ, P.d_q, P.k, P.p_leafmax,
-                   P.dM, s);
+        launch_p2m(P.d_leafdesc, P.n_far_leaves, P.p2m_chunk, lv, P.dX, P.dY, P.dZ, P.d_q, P.k, P.p_leafmax, P.dM, s);
         mark(2);
         for (int L = P.depth - 1; L >= 2; --L) {
             if (P.m2mb[L].use) {
@@ -1108,7 +1127,7 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
             launch_l2p_field(P.d_far_leaves, P.n_far_leaves, bx, lv, P.dX, P.dY, P.dZ, P.dL, P.k, P.p_leafmax,
                              P.d_far, P.d_far_e, s);
         } else {
-            launch_l2p(P.d_far_leaves, P.n_far_leaves, bx, lv, P.dX, P.dY, P.dZ, P.dL, P.k, P.pmax, P.d_far, s);
+            launch_l2p(P.d_leafdesc, P.n_far_leaves, lv, P.dX, P.dY, P.dZ, P.dL, P.k, P.p_leafmax, P.d_far, s);
         }
         mark(6);
     }
