@@ -95,11 +95,11 @@ __device__ inline void decode_q(int q, int& n, int& m) {
 // number of 16-row tiles of the largest (u or v) block, m2l_tiles(p_tab + 1) (1 up to
 // p = 15, 2 up to 23); it bounds the A fragments a unit keeps in registers.
 // waves per SIMD the compiler fits the registers to: FP32 with 16 x 2 pairs per group and
-// one tile per block (the finest levels) runs faster at 4 (128 VGPRs with 100 bytes per
-// lane spilled to scratch, mostly around the stage-3 accumulation; 5.0 against 5.3 ms at
-// p = 11, N = 1M, k = 40) than at the 3 its 156 VGPRs allow.  The spill stores reach HBM
-// (about 2.5 GB written per such launch), which the traffic counters show.  The other
-// instances keep the compiler's choice (FP64 at 4 spills heavily and runs 35% slower).
+// one tile per block (the finest levels) is held to 4 (128 VGPRs; 4.31 ms at p = 11,
+// N = 1M, k = 40, against 4.69 at the 3 waves its registers would otherwise take).  It fits
+// without spilling because the lane-derived offsets are formed again in every group and the
+// gather's source offsets are scalar.  The other instances keep the compiler's choice
+// (FP64 at 4 spills heavily and runs 35% slower).
 template <typename T, int TB, int G, int KT>
 constexpr int m2l_block_min_waves() {
     return sizeof(T) == 4 && TB == 16 && G == 2 && KT == 1 ? 4 : 1;
@@ -187,6 +187,14 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
     __syncthreads();
 
     for (int e = item.y; e < item.z; ++e) {
+        // lane-derived offsets are formed again in every group from an opaque copy of the
+        // lane index: kept live across the loop they would be spilled (4 waves per SIMD
+        // leave 128 registers) and reloaded from scratch in every group
+        int zero_o = 0;
+        asm volatile("" : "+v"(zero_o));
+        const int lane_o = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, (unsigned)zero_o));
+        const int li = lane_o & 15, lk = lane_o >> 4;
+        const C2* ps_lane = Ps + (SPL ? 0 : (li & 1)) * G * TL + Mma<T>::row(lk, 0);
         const unsigned short* gd = a.grp_dir + (size_t)e * G;
         const DispInfo di = a.dinfo[gd[0]];   // shared by the directions of the group
         const T* R1 = (const T*)a.rot1 + (size_t)di.theta * a.rot_stride1;
@@ -216,7 +224,7 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
             const int g = wave / WPD;
             const unsigned short d = gd[g];
             const int* so = soff + g * TB;
-            for (int q = (wave % WPD) * 64 + lane; q < nq; q += 64 * WPD) {
+            for (int q = (wave % WPD) * 64 + lane_o; q < nq; q += 64 * WPD) {
                 int n, m;
                 decode_q(q, n, m);
                 const int cp = n * n + n + m, cm = n * n + n - m;
@@ -235,7 +243,8 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                     C2 rp[JB], rm[JB];
 #pragma unroll
                     for (int j = 0; j < JB; ++j) {
-                        const int o = so[j0 + j];
+                        // wave-uniform: a scalar base for the lanes' coefficient offsets
+                        const int o = __builtin_amdgcn_readfirstlane(so[j0 + j]);
                         rp[j].x = 0;
                         rp[j].y = 0;
                         rm[j].x = 0;
@@ -263,7 +272,7 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
             }
             // target degrees above the source degree start from zero (orders |m| > p
             // receive nothing in stage 2)
-            for (int c = nc + (wave % WPD) * 64 + lane; c < nct; c += 64 * WPD) {
+            for (int c = nc + (wave % WPD) * 64 + lane_o; c < nct; c += 64 * WPD) {
                 T* xr = X + c * LDX + g * NCOLD;
 #pragma unroll
                 for (int j = 0; j < NCOLD; ++j) xr[j] = (T)0;
@@ -282,7 +291,7 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
             };
             int i1 = nx(0);
             V4 an = V4{0, 0, 0, 0};
-            if (i1 < nu1) an = Mma<T>::load4(R1 + (__builtin_amdgcn_readlane(vt1, i1) >> 9) + lane * 4);
+            if (i1 < nu1) an = Mma<T>::load4(R1 + (__builtin_amdgcn_readlane(vt1, i1) >> 9) + lane_o * 4);
             __syncthreads();
             while (i1 < nu1) {
                 const int code = __builtin_amdgcn_readlane(vt1, i1) & 511;
@@ -290,7 +299,7 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                 const int len = h ? n : n + 1, base = h ? n * n : n * n + n, nks = (len + 3) >> 2;
                 const V4 av = an;
                 i1 = nx(i1 + 1);
-                if (i1 < nu1) an = Mma<T>::load4(R1 + (__builtin_amdgcn_readlane(vt1, i1) >> 9) + lane * 4);
+                if (i1 < nu1) an = Mma<T>::load4(R1 + (__builtin_amdgcn_readlane(vt1, i1) >> 9) + lane_o * 4);
                 V4 c4[NCT];
                 T bv[4][NCT];
 #pragma unroll
@@ -332,7 +341,7 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                 V4 av[kMaxTiles];
 #pragma unroll
                 for (int mt = 0; mt < kMaxTiles; ++mt)
-                    if (mt < nt) av[mt] = Mma<T>::load4(A + ((size_t)(mt * nt + kg) * 64 + lane) * 4);
+                    if (mt < nt) av[mt] = Mma<T>::load4(A + ((size_t)(mt * nt + kg) * 64 + lane_o) * 4);
                 T bv[4][NCT];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
@@ -377,7 +386,7 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                 };
                 V4 anr = V4{0, 0, 0, 0}, ani = V4{0, 0, 0, 0};
                 auto pre = [&](int i) {   // fragment (row tile 0, k group 0) of the order's block
-                    const T* At = Tz + (__builtin_amdgcn_readlane(vt2, i) >> 9) + lane * 8;
+                    const T* At = Tz + (__builtin_amdgcn_readlane(vt2, i) >> 9) + lane_o * 8;
                     anr = Mma<T>::load4(At);
                     ani = Mma<T>::load4(At + 4);
                 };
@@ -484,7 +493,7 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
     #pragma unroll
                         for (int t = 0; t < kMaxTiles; ++t)
                             if (t < nt) {
-                                const T* At = A + ((size_t)(t * ntab + kg) * 64 + lane) * 8;
+                                const T* At = A + ((size_t)(t * ntab + kg) * 64 + lane_o) * 8;
                                 are[t] = Mma<T>::load4(At);
                                 aim[t] = Mma<T>::load4(At + 4);
                             }
@@ -573,7 +582,7 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
             V4 anu = V4{0, 0, 0, 0}, anv = V4{0, 0, 0, 0};
             auto pre = [&](int uu) {   // the u block's fragment, then the v block's (one tile each)
                 const int t = __builtin_amdgcn_readlane(vt3, uu);
-                const T* Au = R3 + (t >> 9) + lane * 4;
+                const T* Au = R3 + (t >> 9) + lane_o * 4;
                 anu = Mma<T>::load4(Au);
                 if ((t & 255) > 0) anv = Mma<T>::load4(Au + 256);
             };
@@ -654,7 +663,7 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                         cv[ct] = V4{0, 0, 0, 0};
                     }
                     for (int kg = 0; kg < ntu; ++kg) {
-                        const V4 av = Mma<T>::load4(Au + ((size_t)kg * 64 + lane) * 4);
+                        const V4 av = Mma<T>::load4(Au + ((size_t)kg * 64 + lane_o) * 4);
                         T bv[4][NCT];
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
@@ -669,7 +678,7 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                                 for (int ct = 0; ct < NCT; ++ct) cu[ct] = Mma<T>::mma(av[j], bv[j][ct], cu[ct]);
                     }
                     for (int kg = 0; kg < ntv; ++kg) {
-                        const V4 av = Mma<T>::load4(Av + ((size_t)kg * 64 + lane) * 4);
+                        const V4 av = Mma<T>::load4(Av + ((size_t)kg * 64 + lane_o) * 4);
                         T bv[4][NCT];
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
