@@ -144,6 +144,11 @@ void launch_direct(const double* X, const double* Y, const double* Z, const doub
 // x_{-m}: the u block of degree n is rows n^2+n .. n^2+2n (m ascending), the v block rows
 // n^2 .. n^2+n-1 (m = n down to 1).
 constexpr int kM2LBlockMaxP = 23;  // largest degree handled by the block kernel
+// FP32 one-tile instances: blocks of at most this many rows (forward rotation) or of at
+// most this size squared (coaxial) run on the 4x4x1 MFMA in 4-row tiles.  The fragments
+// beyond a block's first are loaded when used; with 8 (two tiles) those loads cost more
+// than the MFMA passes saved (level-5 M2L at p = 11: 4.38 against 3.90 ms).
+constexpr int kM2LQuadMaxRows = 4;
 constexpr int kXcds = 8;            // accelerator dies; workgroups are dealt to them round-robin
 
 // Work queues of a persistent launch (batches of problems): work items [start[x],
@@ -218,6 +223,7 @@ struct M2LBlockArgs {
     const void* rot1;                // per rotation: fragments of the u and v blocks of E_n^T (T)
     const void* rot3;                // per rotation: fragments of the u and v blocks of E_n (T)
     const int* rot_off1;             // offsets of (n, block) [2 n + block] within rot1's rotation
+    const int* rot_off1q;            // blocks of at most 4 rows: their 4x4x1 fragments (-1: larger)
     const int* rot_off3;             // the same within rot3's
     int rot_stride1, rot_stride3;
     const void* phase;               // e^{i m phi} per direction (T complex)
@@ -225,6 +231,7 @@ struct M2LBlockArgs {
     const DispInfo* dinfo;
     const void* coax;                // per distance: fragments of T^m, re then im per lane (T)
     const int* coax_off;             // offset of order m's fragments within one distance
+    const int* coax_offq;            // blocks of at most 4 x 4: their 4x4x1 fragments (-1: larger)
     int coax_stride;
     const void* M;                   // multipoles (T complex)
     double2* L;                      // local expansions (FP64), accumulated into

@@ -52,6 +52,12 @@ struct Mma<float> {
     __device__ static inline v4 mma(float a, float b, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    // 4 rows x 64 columns, K = 1: the 4x4x1 MFMA over 16 blocks with the A operand of block 0
+    // (lanes 0..3, row = lane) broadcast to all blocks; b: lane l holds column l, and output
+    // element r of lane l is row r of column l.  Same FP32 rate as 16x16x4, 8 passes.
+    __device__ static inline v4 mma4(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, 0, 0);
+    }
     // output element r of lane group lk (= lane / 16) is row 4 lk + r of the 16x16 tile
     __host__ __device__ static constexpr int row(int lk, int r) { return 4 * lk + r; }
     __device__ static inline v4 load4(const float* p) { return *(const v4*)p; }
@@ -129,6 +135,13 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
     // stage uses one fragment set, and the next unit's set is loaded while the current one
     // is multiplied (the L2 latency of the operator tables is otherwise exposed per unit)
     constexpr bool PF = sizeof(T) == 4 && KT == 1;
+    // with PF, blocks of at most kM2LQuadMaxRows rows run on the 4x4x1 MFMA in 4-row tiles
+    // (lane = column; a 32-column block reads its columns twice and writes from the first
+    // 32 lanes); the code takes up to two tiles
+    static_assert(kM2LQuadMaxRows <= 8, "4x4x1 path: at most two 4-row tiles");
+    constexpr int QT = (kM2LQuadMaxRows + 3) / 4, QR = 4 * QT;   // tiles and rows of the path
+    constexpr int NCOLS = G * NCOLD;
+    constexpr bool Q4 = PF && (NCOLS == 64 || NCOLS == 32);
     extern __shared__ char smem_raw[];
     // source degree p (multipoles or parent locals) and target degree pt; equal for M2L
     const int p = a.p, P1 = p + 1, nc = P1 * P1, nq = P1 * (P1 + 1) / 2;
@@ -170,13 +183,22 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
     const int nu1 = a.n_units1 >> 2, nu2 = a.n_units2 >> 2;   // the lists are padded to rows of 4
     int vt1 = 255, vt2 = 255, vt3 = 255;
     if constexpr (PF) {
+        // blocks of at most kM2LQuadMaxRows rows (forward rotation; square, so the size is
+        // fixed by the degree) and coaxial blocks of at most that size at the table degrees
+        // take the 4x4x1 fragments (bit 8 of a coaxial unit's code marks them)
         if (lane < nu1) {
             const int c = a.units1[wave + 4 * lane];
-            if ((c & 255) <= p) vt1 = (a.rot_off1[2 * (c & 255) + (c >> 8)] << 9) | c;
+            if ((c & 255) <= p) {
+                const int b = 2 * (c & 255) + (c >> 8), oq = Q4 ? a.rot_off1q[b] : -1;
+                vt1 = ((oq >= 0 ? oq : a.rot_off1[b]) << 9) | c;
+            }
         }
         if (lane < nu2) {
             const int c = a.units2[wave + 4 * lane];
-            if (c <= p && c <= pt) vt2 = (a.coax_off[c] << 9) | c;
+            if (c <= p && c <= pt) {
+                const int oq = Q4 ? a.coax_offq[c] : -1;
+                vt2 = oq >= 0 ? (oq << 9) | 256 | c : (a.coax_off[c] << 9) | c;
+            }
         }
         if (lane < MAXU && wave + 4 * lane < a.n_units3) {
             const int c = a.units3[wave + 4 * lane];
@@ -294,12 +316,40 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
             if (i1 < nu1) an = Mma<T>::load4(R1 + (__builtin_amdgcn_readlane(vt1, i1) >> 9) + lane_o * 4);
             __syncthreads();
             while (i1 < nu1) {
-                const int code = __builtin_amdgcn_readlane(vt1, i1) & 511;
+                const int code1 = __builtin_amdgcn_readlane(vt1, i1);
+                const int code = code1 & 511;
                 const int n = code & 255, h = code >> 8;
                 const int len = h ? n : n + 1, base = h ? n * n : n * n + n, nks = (len + 3) >> 2;
                 const V4 av = an;
                 i1 = nx(i1 + 1);
                 if (i1 < nu1) an = Mma<T>::load4(R1 + (__builtin_amdgcn_readlane(vt1, i1) >> 9) + lane_o * 4);
+                if (Q4 && len <= kM2LQuadMaxRows) {
+                    // 4-row tiles: lane = column, k = 0..len-1 over the block's rows; the
+                    // fragment of tile t and k group g at (t nt4 + g) x 256 (the first is av)
+                    const int cl = lane_o & (NCOLS - 1), nt4 = (len + 3) >> 2;
+                    const T* A4 = R1 + (code1 >> 9) + lane_o * 4;
+                    T b4[QR];
+#pragma unroll
+                    for (int k = 0; k < QR; ++k) b4[k] = X[(base + k) * LDX + cl];   // past len: A is 0
+#pragma unroll
+                    for (int t4 = 0; t4 < QT; ++t4) {
+                        if (t4 >= nt4) break;
+                        V4 d4 = V4{0, 0, 0, 0};
+#pragma unroll
+                        for (int g4 = 0; g4 < QT; ++g4) {
+                            if (g4 >= nt4) break;
+                            const V4 a4 = t4 + g4 == 0 ? av : Mma<T>::load4(A4 + (t4 * nt4 + g4) * 256);
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                if (4 * g4 + j < len) d4 = Mma<T>::mma4(a4[j], b4[4 * g4 + j], d4);
+                        }
+                        if (lane_o < NCOLS)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                if (4 * t4 + r < len) X[(base + 4 * t4 + r) * LDX + cl] = d4[r];
+                    }
+                    continue;
+                }
                 V4 c4[NCT];
                 T bv[4][NCT];
 #pragma unroll
@@ -394,12 +444,66 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                 if (i2 < nu2) pre(i2);
                 __syncthreads();
                 while (i2 < nu2) {
-                    const int am = __builtin_amdgcn_readlane(vt2, i2) & 511;
+                    const int code2 = __builtin_amdgcn_readlane(vt2, i2);
+                    const int am = code2 & 255;
                     const int ns = am > 0 ? 2 : 1;
                     const int len = P1 - am, lent = PT1 - am, nks = (len + 3) >> 2;
                     const V4 are = anr, aim = ani;
                     i2 = nx(i2 + 1);
                     if (i2 < nu2) pre(i2);
+                    if (Q4 && (code2 & 256)) {
+                        // block of at most kM2LQuadMaxRows squared (table degrees) in 4-row tiles: lane = column;
+                        // Re T on the column, Im T on its partner (re <-> im), signed for the
+                        // real columns.  Fragment of tile t and k group g at (t nk4 + g) x 512
+                        // (the first is are, aim).
+                        constexpr int PART = SPL ? 16 : 1;
+                        const int cl = lane_o & (NCOLS - 1);
+                        const T sgi = (cl & PART) ? (T)1 : (T)-1;
+                        const int nt4 = (a.pt_tab + 4 - am) >> 2, nk4 = (a.p_tab + 4 - am) >> 2;
+                        const T* A4 = Tz + (code2 >> 9) + lane_o * 8;
+#pragma unroll
+                        for (int sg = 0; sg < 2; ++sg) {
+                            if (sg < ns) {
+                                T b1[QR], b2[QR];
+#pragma unroll
+                                for (int k = 0; k < QR; ++k) {
+                                    const int nn = am + k;
+                                    const int row = k < len ? nn * (nn + 1) + (sg ? -am : am) : zero_row;
+                                    b1[k] = X[row * LDX + cl];
+                                    b2[k] = X[row * LDX + (cl ^ PART)];
+                                }
+#pragma unroll
+                                for (int t4 = 0; t4 < QT; ++t4) {
+                                    if (t4 >= nt4 || 4 * t4 >= lent) break;
+                                    V4 d1 = V4{0, 0, 0, 0}, d2 = V4{0, 0, 0, 0};
+#pragma unroll
+                                    for (int g4 = 0; g4 < QT; ++g4) {
+                                        if (g4 >= nk4) break;
+                                        V4 ar = are, ai = aim;
+                                        if (t4 + g4 > 0) {
+                                            ar = Mma<T>::load4(A4 + (t4 * nk4 + g4) * 512);
+                                            ai = Mma<T>::load4(A4 + (t4 * nk4 + g4) * 512 + 4);
+                                        }
+#pragma unroll
+                                        for (int j = 0; j < 4; ++j)
+                                            if (4 * g4 + j < len) {
+                                                d1 = Mma<T>::mma4(ar[j], b1[4 * g4 + j], d1);
+                                                d2 = Mma<T>::mma4(ai[j], b2[4 * g4 + j], d2);
+                                            }
+                                    }
+                                    if (lane_o < NCOLS)
+#pragma unroll
+                                        for (int r = 0; r < 4; ++r)
+                                            if (4 * t4 + r < lent) {
+                                                const int nn = am + 4 * t4 + r;
+                                                X[(nn * (nn + 1) + (sg ? -am : am)) * LDX + cl] =
+                                                    fma(sgi, d2[r], d1[r]);
+                                            }
+                                }
+                            }
+                        }
+                        continue;
+                    }
                     const int q0 = __umul24(am + lk, am + lk + 1);
                     int rowp[4], rowm[4];
 #pragma unroll

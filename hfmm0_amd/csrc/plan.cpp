@@ -63,6 +63,7 @@ struct M2LBlockLevel {
     int n_units3 = 0;
     void* coax = nullptr;          // A fragments of the coaxial tables, kernel precision
     int* coax_off = nullptr;       // per order m
+    int* coax_offq = nullptr;      // per order m: its 4 x 4 fragment (-1: block larger than 4 x 4)
     int coax_stride = 0;
     bool split = false;            // M2M split over items: the targets are cleared first
     int n_zero = 0;                // targets to clear
@@ -134,6 +135,7 @@ struct hfmm_plan {
     double *d_rf1 = nullptr, *d_rf3 = nullptr;
     float *d_rf1_32 = nullptr, *d_rf3_32 = nullptr;
     int *d_rf_off1 = nullptr, *d_rf_off3 = nullptr;
+    int* d_rf_off1q = nullptr;   // forward rotation blocks of at most 4 rows: their 4 x 4 fragments
     int rf_stride1 = 0, rf_stride3 = 0;
     float2* d_phase32 = nullptr;
     float2* dM32 = nullptr;
@@ -236,13 +238,23 @@ void make_units(int ps, int pt, std::vector<int>& units1, std::vector<int>& unit
 // Coaxial tables as A fragments: per distance, per order m = 0..min(ps, pt) the packed
 // (pt+1-m) x (ps+1-m) block T[i][k] (row i: target degree m+i, column k: source degree
 // m+k), padded to 16x16 tiles; per (row tile, k group, lane) eight values: Re then Im
-// of j = 0..3.  coff[m] is the offset of order m within one distance.
+// of j = 0..3.  coff[m] is the offset of order m within one distance.  Blocks of at most
+// kM2LQuadMaxRows squared also get fragments for the 4x4x1 MFMA with its A broadcast from lanes 0..3: per
+// 4-row tile t and k group g (t-major), lane l holds Re then Im of T[4t + l % 4][4g + j],
+// j = 0..3; they follow the 16x16 fragments of the distance, at coffq[m] (-1 for larger
+// blocks).
 int coax_fragments(const std::vector<double>& tabs, int n_dist, int ps, int pt, std::vector<double>& cf,
-                   std::vector<int>& coff) {
+                   std::vector<int>& coff, std::vector<int>& coffq) {
     const int mm = std::min(ps, pt);
     coff.assign(mm + 2, 0);
     for (int m = 0; m <= mm; ++m) coff[m + 1] = coff[m] + m2l_tiles(pt + 1 - m) * m2l_tiles(ps + 1 - m) * 64 * 8;
-    const int stride = coff[mm + 1];
+    int stride = coff[mm + 1];
+    coffq.assign(mm + 1, -1);
+    for (int m = 0; m <= mm; ++m)
+        if (pt + 1 - m <= kM2LQuadMaxRows && ps + 1 - m <= kM2LQuadMaxRows) {
+            coffq[m] = stride;
+            stride += ((pt + 4 - m) / 4) * ((ps + 4 - m) / 4) * 64 * 8;
+        }
     cf.assign((size_t)n_dist * stride, 0.0);
     const int csz = coax_size(pt, ps);
     for (int di = 0; di < n_dist; ++di) {
@@ -262,6 +274,19 @@ int coax_fragments(const std::vector<double>& tabs, int n_dist, int ps, int pt, 
                             f[j] = tab[2 * (off + i * lens + k)];
                             f[4 + j] = tab[2 * (off + i * lens + k) + 1];
                         }
+            if (coffq[m] >= 0) {
+                const int nt4 = (lent + 3) / 4, nk4 = (lens + 3) / 4;
+                for (int t4 = 0; t4 < nt4; ++t4)
+                    for (int g4 = 0; g4 < nk4; ++g4)
+                        for (int lane = 0; lane < 64; ++lane)
+                            for (int j = 0; j < 4; ++j) {
+                                const int i = 4 * t4 + (lane & 3), k = 4 * g4 + j;
+                                if (i >= lent || k >= lens) continue;
+                                double* f = dst + coffq[m] + ((size_t)(t4 * nk4 + g4) * 64 + lane) * 8;
+                                f[j] = tab[2 * (off + i * lens + k)];
+                                f[4 + j] = tab[2 * (off + i * lens + k) + 1];
+                            }
+            }
             off += lent * lens;
         }
     }
@@ -399,8 +424,8 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     B.units3 = P.upload(units3);
     B.n_units3 = (int)units3.size();
     std::vector<double> cf;
-    std::vector<int> coff;
-    B.coax_stride = coax_fragments(coax_d, n_dist, p, p, cf, coff);
+    std::vector<int> coff, coffq;
+    B.coax_stride = coax_fragments(coax_d, n_dist, p, p, cf, coff, coffq);
     if (fp32) {
         std::vector<float> c32(cf.begin(), cf.end());
         B.coax = P.upload(c32);
@@ -408,6 +433,7 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
         B.coax = P.upload(cf);
     }
     B.coax_off = P.upload(coff);
+    B.coax_offq = P.upload(coffq);
     prepare_m2l_block(p, tb, grp, fp32);
 }
 
@@ -508,10 +534,11 @@ void build_tree_blocks(hfmm_plan& P, int L, int kind, const std::vector<double>&
         B.zero = P.upload(z);
     }
     std::vector<double> cf;
-    std::vector<int> coff;
-    B.coax_stride = coax_fragments(tab, 1, ps, pt, cf, coff);
+    std::vector<int> coff, coffq;
+    B.coax_stride = coax_fragments(tab, 1, ps, pt, cf, coff, coffq);
     B.coax = P.upload(cf);
     B.coax_off = P.upload(coff);
+    B.coax_offq = P.upload(coffq);
     prepare_m2l_block(pm, tb, grp, 0);
 }
 
@@ -576,6 +603,18 @@ void build_tables(hfmm_plan& P) {
             off3[2 * n + 1] = s3;
             s3 += m2l_tiles(n + 1) * m2l_tiles(n) * 256;
         }
+        // blocks of at most kM2LQuadMaxRows rows also get fragments for the 4x4x1 MFMA with its A broadcast
+        // from lanes 0..3 (per 4-row tile t and k group g, t-major: lane l holds row
+        // 4t + l % 4, k = 4g..4g+3), after the 16x16 fragments
+        std::vector<int> off1q(2 * (pb + 1), -1);
+        for (int n = 0; n <= pb; ++n)
+            for (int h = 0; h < 2; ++h) {
+                const int len = h ? n : n + 1;
+                if (len >= 1 && len <= kM2LQuadMaxRows) {
+                    off1q[2 * n + h] = s1;
+                    s1 += ((len + 3) / 4) * ((len + 3) / 4) * 256;
+                }
+            }
         std::vector<double> f1((size_t)thetas.size() * s1, 0.0), f3((size_t)thetas.size() * s3, 0.0);
         // A fragments of an R x K matrix a(i, k) at dst
         auto put = [](double* dst, int R, int K, const std::function<double(int, int)>& a) {
@@ -586,6 +625,17 @@ void build_tables(hfmm_plan& P) {
                         for (int j = 0; j < 4; ++j) {
                             const int i = 16 * mt + (lane & 15), k = 16 * kg + 4 * j + (lane >> 4);
                             if (i < R && k < K) dst[((size_t)(mt * nk + kg) * 64 + lane) * 4 + j] = a(i, k);
+                        }
+        };
+        // 4x4x1 fragments of an R x K matrix (R, K <= 8), 4-row tiles
+        auto put4 = [](double* dst, int R, int K, const std::function<double(int, int)>& a) {
+            const int nt4 = (R + 3) / 4, nk4 = (K + 3) / 4;
+            for (int t4 = 0; t4 < nt4; ++t4)
+                for (int g4 = 0; g4 < nk4; ++g4)
+                    for (int lane = 0; lane < 64; ++lane)
+                        for (int j = 0; j < 4; ++j) {
+                            const int i = 4 * t4 + (lane & 3), k = 4 * g4 + j;
+                            if (i < R && k < K) dst[((t4 * nk4 + g4) * 64 + lane) * 4 + j] = a(i, k);
                         }
         };
         std::vector<double> Wu, Wv;
@@ -601,6 +651,9 @@ void build_tables(hfmm_plan& P) {
                         double* b = &f1[th * s1];
                         put(b + off1[2 * n], n + 1, n + 1, wu);
                         if (n > 0) put(b + off1[2 * n + 1], n, n, [&](int i, int k) { return wv(n - i, n - k); });
+                        if (off1q[2 * n] >= 0) put4(b + off1q[2 * n], n + 1, n + 1, wu);
+                        if (off1q[2 * n + 1] >= 0)
+                            put4(b + off1q[2 * n + 1], n, n, [&](int i, int k) { return wv(n - i, n - k); });
                     } else {
                         double* b = &f3[th * s3];
                         put(b + off3[2 * n], n + 1, n + 1, wu);
@@ -613,6 +666,7 @@ void build_tables(hfmm_plan& P) {
         P.rf_stride1 = s1;
         P.rf_stride3 = s3;
         P.d_rf_off1 = P.upload(off1);
+        P.d_rf_off1q = P.upload(off1q);
         P.d_rf_off3 = P.upload(off3);
         if (P.precision == HFMM_PREC_MIXED) {
             P.d_rf1_32 = P.upload(std::vector<float>(f1.begin(), f1.end()));
@@ -1008,6 +1062,7 @@ M2LBlockArgs block_args(const hfmm_plan& P, const M2LBlockLevel& B, bool fp32, c
     a.rot1 = fp32 ? (const void*)P.d_rf1_32 : (const void*)P.d_rf1;
     a.rot3 = fp32 ? (const void*)P.d_rf3_32 : (const void*)P.d_rf3;
     a.rot_off1 = P.d_rf_off1;
+    a.rot_off1q = P.d_rf_off1q;
     a.rot_off3 = P.d_rf_off3;
     a.rot_stride1 = P.rf_stride1;
     a.rot_stride3 = P.rf_stride3;
@@ -1016,6 +1071,7 @@ M2LBlockArgs block_args(const hfmm_plan& P, const M2LBlockLevel& B, bool fp32, c
     a.dinfo = P.d_dinfo;
     a.coax = B.coax;
     a.coax_off = B.coax_off;
+    a.coax_offq = B.coax_offq;
     a.coax_stride = B.coax_stride;
     a.M = src;
     a.L = dst;
