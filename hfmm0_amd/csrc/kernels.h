@@ -40,8 +40,7 @@ inline size_t p2m_bytes_per_point(int p) {
     return 24 * P1 + 32;
 }
 // points staged per pass of the P2M kernel at the largest leaf degree pmax: the staged
-// points and the recurrence coefficients stay within 36 KB of LDS, so that four
-// workgroups fit on a CU
+// points and the recurrence coefficients stay within the LDS budget of its workgroup size
 int p2m_chunk(int pmax);
 // One leaf of the P2M and L2P launches, 64 bytes, read by 16 lanes with one vector load.
 struct LeafDesc {

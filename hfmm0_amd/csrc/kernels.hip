@@ -303,7 +303,6 @@ __device__ __forceinline__ LeafRange leaf_range(int n) {
 //     which give both orders: Mh_n^m = U - iV and Mh_n^-m = (-1)^m (U + iV), since
 //     conj(Y_n^-m) = (-1)^m Y_n^m.  The G groups are added at the end of the leaf.
 // ---------------------------------------------------------------------------------
-constexpr int kP2MThreads = 256;
 constexpr int kP2MBlockN = 4;     // degrees per thread block of an order
 
 __host__ __device__ constexpr int p2m_blocks(int p) {
@@ -311,22 +310,28 @@ __host__ __device__ constexpr int p2m_blocks(int p) {
     for (int m = 0; m <= p; ++m) nb += (p + 1 - m + kP2MBlockN - 1) / kP2MBlockN;
     return nb;
 }
-// point groups: blocks x groups fill the workgroup, at most 16
-__host__ __device__ constexpr int p2m_groups(int p) {
-    return kP2MThreads / p2m_blocks(p) < 16 ? (kP2MThreads / p2m_blocks(p) > 0 ? kP2MThreads / p2m_blocks(p) : 1) : 16;
+// threads of a workgroup: 128 while every block of the largest leaf degree gets a thread,
+// else 256 (p2m_blocks(40) = 231).  128 keeps 7 leaves in flight per CU against 4 for 256,
+// which the latency of the per-point recurrences needs (P2M at N = 1M, k = 40: 0.32 against
+// 0.47 ms); 64 leaves each thread three times the points and measured 0.45 ms
+__host__ __device__ constexpr int p2m_threads(int pmax) { return p2m_blocks(pmax) <= 128 ? 128 : 256; }
+// point groups: blocks x groups fill the workgroup of NT threads, at most 16
+__host__ __device__ constexpr int p2m_groups(int p, int NT) {
+    return NT / p2m_blocks(p) < 16 ? (NT / p2m_blocks(p) > 0 ? NT / p2m_blocks(p) : 1) : 16;
 }
 
 // Miller start for a leaf (uniform): p + 1.5 k R + 13
 __device__ __forceinline__ int miller_start(int p, double kR) { return p + (int)(1.5 * fabs(kR)) + 13; }
 
-// held to 4 waves per SIMD (128 VGPRs), as many as LDS allows
-__global__ __launch_bounds__(kP2MThreads) __attribute__((amdgpu_waves_per_eu(4))) void p2m_kernel(
+// held to 4 waves per SIMD (128 VGPRs)
+template <int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void p2m_kernel(
     const LeafDesc* __restrict__ leaves, int n_items, int chunk, int pmax, DevLevels lv, const double* X,
     const double* Y, const double* Z, const double2* q, double k, double2* M) {
     extern __shared__ double smem[];
     const int tid = threadIdx.x;
     const int PM1 = pmax + 1;
-    const LegLds lg = stage_legendre(smem, pmax, tid, kP2MThreads);
+    const LegLds lg = stage_legendre(smem, pmax, tid, NT);
     double* region = smem + leg_lds_doubles(pmax);
     LeafRange lr = leaf_range(n_items);
     int w = lr.it < lr.end ? leaf_desc_word(leaves, lr.it) : 0;
@@ -344,7 +349,7 @@ __global__ __launch_bounds__(kP2MThreads) __attribute__((amdgpu_waves_per_eu(4))
         double2* s_g = s_q + chunk;                   // chunk: (z/R, (rho/R)^2)
         double* s_j = (double*)(s_g + chunk);         // chunk x P1: jhat_n / scale
         // this thread's block: order m, degrees n0..n1, point group g (g >= G: idle)
-        const int NB = p2m_blocks(p), G = p2m_groups(p);
+        const int NB = p2m_blocks(p), G = p2m_groups(p, NT);
         const int bi = tid % NB, g = tid / NB;
         int m = 0, n0 = 0;
         for (int r = bi;; ++m) {
@@ -364,7 +369,7 @@ __global__ __launch_bounds__(kP2MThreads) __attribute__((amdgpu_waves_per_eu(4))
 
         for (int j0 = 0; j0 < f.count; j0 += chunk) {
             const int nch = min(chunk, f.count - j0);
-            for (int j = tid; j < nch; j += kP2MThreads) {
+            for (int j = tid; j < nch; j += NT) {
                 const int i = f.pstart + j0 + j;
                 const double dx = X[i] - f.cx, dy = Y[i] - f.cy, dz = Z[i] - f.cz;
                 const double r2 = dx * dx + dy * dy + dz * dz;
@@ -1346,16 +1351,19 @@ void launch_gather_charges(const void* q, int q_type, const int* perm, int64_t n
 }
 
 int p2m_chunk(int pmax) {
-    const long budget = 36 * 1024 - (long)leg_lds_doubles(pmax) * (long)sizeof(double);
+    // LDS with the coefficients within 22 KB for 128 threads (7 workgroups per CU) and 36 KB
+    // for 256 (4 per CU)
+    const long budget =
+        (p2m_threads(pmax) == 128 ? 22 : 36) * 1024L - (long)leg_lds_doubles(pmax) * (long)sizeof(double);
     return (int)std::max<long>(1, std::min<long>(64, budget / (long)p2m_bytes_per_point(pmax)));
 }
 
 size_t p2m_lds_bytes(int pmax, int chunk) {
     // the chunk region also holds the point groups' partial sums at the end of a leaf:
-    // (G - 1) x blocks x 4 x 32 bytes, with G x blocks <= 256
+    // (G - 1) x blocks x 4 x 32 bytes, with G x blocks <= threads
     return (size_t)leg_lds_doubles(pmax) * sizeof(double) +
            std::max<size_t>((size_t)chunk * p2m_bytes_per_point(pmax),
-                            (size_t)kP2MThreads * kP2MBlockN * 2 * sizeof(double2));
+                            (size_t)p2m_threads(pmax) * kP2MBlockN * 2 * sizeof(double2));
 }
 
 // resident workgroups of a persistent leaf loop: the occupancy of (fn, threads, shm) on
@@ -1374,9 +1382,15 @@ void launch_p2m(const LeafDesc* leaves, int n_items, int chunk, DevLevels lv, co
     if (n_items <= 0) return;
     upload_legendre_coefficients();
     const size_t shm = p2m_lds_bytes(pmax, chunk);
-    const int grid = leaf_loop_grid((const void*)p2m_kernel, kP2MThreads, shm, n_items);
-    hipLaunchKernelGGL(p2m_kernel, dim3(grid), dim3(kP2MThreads), shm, s, leaves, n_items, chunk, pmax, lv, X, Y, Z,
-                       q, k, M);
+    if (p2m_threads(pmax) == 128) {
+        const int grid = leaf_loop_grid((const void*)p2m_kernel<128>, 128, shm, n_items);
+        hipLaunchKernelGGL(p2m_kernel<128>, dim3(grid), dim3(128), shm, s, leaves, n_items, chunk, pmax, lv, X, Y,
+                           Z, q, k, M);
+    } else {
+        const int grid = leaf_loop_grid((const void*)p2m_kernel<256>, 256, shm, n_items);
+        hipLaunchKernelGGL(p2m_kernel<256>, dim3(grid), dim3(256), shm, s, leaves, n_items, chunk, pmax, lv, X, Y,
+                           Z, q, k, M);
+    }
     check_launch("p2m");
 }
 
