@@ -108,9 +108,13 @@ __device__ void jhat_dev(double x, int p, double* out, int stride, double xmax, 
 
 // LDS copy of the recurrence coefficients of degree p: ab in order-major triangle layout
 // (the entries of order m, n = m..p, from leg_col(p + 1, m)), de and cm (p+1 each), and
-// the first kMillerLds Miller factors.
+// the first kMillerLds Miller factors; rounded up to 32 bytes, so that the double2 arrays
+// the kernels place after it stay 16-byte aligned (3 (p+1) is odd for even p: misaligned
+// 16-byte LDS accesses made L2P 3.8x slower at p = 12 than at p = 11)
 __host__ __device__ constexpr int leg_col(int P1, int m) { return m * P1 - m * (m - 1) / 2; }
-__host__ __device__ constexpr int leg_lds_doubles(int p) { return (p + 1) * (p + 2) + 3 * (p + 1) + kMillerLds; }
+__host__ __device__ constexpr int leg_lds_doubles(int p) {
+    return ((p + 1) * (p + 2) + 3 * (p + 1) + kMillerLds + 3) / 4 * 4;
+}
 
 struct LegLds {
     const double2* ab;    // ab[leg_col(p + 1, m) + n - m] = (a_nm, b_nm)
