@@ -182,6 +182,12 @@ __host__ __device__ constexpr int m2l_tiles(int len) { return (len + 15) / 16; }
 __host__ __device__ constexpr int m2l_block_maxu(int p) {
     return p <= 7 ? 2 : p <= 11 ? 3 : p <= 15 ? 4 : p <= 17 ? 5 : p <= 19 ? 6 : 8;
 }
+// FP32 one-tile instances (p <= 15) pack the stage-3 units of the degrees n1 < n2 with
+// n1 + n2 = min(p_tab, kM2LPackMax) into one 16-row tile: rows 0..7 take the orders of n1,
+// rows 8..15 those of n2 (unit code 256 | n1 << 4 | n2; an odd count leaves the middle
+// degree alone), which lowers the units per wave to these
+constexpr int kM2LPackMax = 7;
+__host__ __device__ constexpr int m2l_block_maxu_packed(int p) { return p <= 7 ? 1 : p <= 11 ? 2 : 3; }
 // accumulator VGPRs of a (MAXU units per wave, TB targets, element size) configuration:
 // two (u and v) 16x16 tiles per unit and direction column tile
 __host__ __device__ constexpr int m2l_acc_regs(int maxu, int tb, int elem_bytes) {
@@ -224,6 +230,7 @@ struct M2LBlockArgs {
     const int* rot_off1;             // offsets of (n, block) [2 n + block] within rot1's rotation
     const int* rot_off1q;            // blocks of at most 4 rows: their 4x4x1 fragments (-1: larger)
     const int* rot_off3;             // the same within rot3's
+    const int* rot_off3p;            // packed stage-3 units (n1, n2): [8 n1 + n2] within rot3's
     int rot_stride1, rot_stride3;
     const void* phase;               // e^{i m phi} per direction (T complex)
     int phase_stride;

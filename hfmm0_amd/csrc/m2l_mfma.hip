@@ -202,7 +202,11 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
         }
         if (lane < MAXU && wave + 4 * lane < a.n_units3) {
             const int c = a.units3[wave + 4 * lane];
-            if (c <= pt) vt3 = (a.rot_off3[2 * c] << 9) | c;   // KT = 1: row tile 0, code = n
+            if (c & 256) {   // packed degrees (n1, n2): active while n1 is
+                if (((c >> 4) & 15) <= pt) vt3 = (a.rot_off3p[8 * ((c >> 4) & 15) + (c & 15)] << 9) | c;
+            } else if (c <= pt) {
+                vt3 = (a.rot_off3[2 * c] << 9) | c;   // KT = 1: row tile 0, code = n
+            }
         }
     }
     auto skip = [](int t) { return (t & 511) == 255; };
@@ -695,8 +699,15 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
 #pragma unroll
             for (int uu = 0; uu < MAXU; ++uu) {
                 if (uu == n3) {
-                    const int n = __builtin_amdgcn_readlane(vt3, uu) & 255;
-                    const int nku = (n + 4) >> 2, nkv = (n + 3) >> 2;
+                    // a packed unit (bit 8) holds degree n1 in rows 0..7 and n2 in rows 8..15;
+                    // its u inputs are n1's n1+1 then n2's n2+1, its v inputs n1's n1 then
+                    // n2's n2, and its rows take the phases of the orders row mod 8
+                    const int code3 = __builtin_amdgcn_readlane(vt3, uu) & 511;
+                    const bool pk = code3 & 256;
+                    const int n = code3 & 255, n1 = (code3 >> 4) & 15, n2 = code3 & 15;
+                    const int nku = pk ? (n1 + n2 + 5) >> 2 : (n + 4) >> 2;
+                    const int nkv = pk ? (n1 + n2 + 3) >> 2 : (n + 3) >> 2;
+                    const C2* psu = ps_lane - (pk ? 8 * (lk >> 1) : 0);
                     const V4 avu = anu, avv = anv;
                     int nx = uu + 1;
                     while (nx < MAXU && skip(__builtin_amdgcn_readlane(vt3, nx))) ++nx;
@@ -710,11 +721,14 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                     }
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
+                        const int k = 4 * j + lk;   // past the inputs: A is 0, the rows finite
+                        const int ru = pk ? (k <= n1 ? n1 * n1 + n1 + k : n2 * n2 + n2 + k - n1 - 1) : n * n + n + k;
+                        const int rv = pk ? (k < n1 ? n1 * n1 + k : n2 * n2 + k - n1) : n * n + k;
                         T bu[NCT], bw[NCT];
 #pragma unroll
                         for (int ct = 0; ct < NCT; ++ct) {
-                            bu[ct] = X[(n * n + n + 4 * j + lk) * LDX + ct * 16 + li];
-                            bw[ct] = X[(n * n + 4 * j + lk) * LDX + ct * 16 + li];
+                            bu[ct] = X[ru * LDX + ct * 16 + li];
+                            bw[ct] = X[rv * LDX + ct * 16 + li];
                         }
                         if (j < nku)
 #pragma unroll
@@ -727,7 +741,7 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                     for (int r = 0; r < 4; ++r) {
 #pragma unroll
                         for (int g = 0; g < G; ++g) {
-                            const C2 pm = ps_lane[g * TL + r * RSTEP];
+                            const C2 pm = psu[g * TL + r * RSTEP];
                             if constexpr (SPL) {
                                 // tiles (re, im) of the direction: U += c u' - i s v', V likewise
                                 const T ur = cu[2 * g][r], ui = cu[2 * g + 1][r];
@@ -833,9 +847,10 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
 #pragma unroll
     for (int uu = 0; uu < MAXU; ++uu) {
         const int u = wave + 4 * uu;
-        if (u < a.n_units3 && (a.units3[u] & 255) <= pt) {
-            const int code = a.units3[u];
-            const int n = code & 255, mt = code >> 8;
+        const int code = u < a.n_units3 ? a.units3[u] : 255;
+        // a packed unit (FP32 one-tile instances): degree n1 in rows 0..7, n2 in rows 8..15
+        const bool pk = PF && (code & 256);
+        if (pk ? ((code >> 4) & 15) <= pt : (code & 255) <= pt) {
 #pragma unroll
             for (int cd = 0; cd < NCTD; ++cd) {
                 const int col = cd * 16 + li;
@@ -844,8 +859,10 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
                     double* dst = (double*)(a.L + a.exp_off[t]) + (SPL ? cd : col & 1);
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const int m = mt * 16 + Mma<T>::row(lk, r);
-                        if (m <= n) {
+                        const int row = Mma<T>::row(lk, r);
+                        const int n = pk ? (row < 8 ? (code >> 4) & 15 : code & 15) : code & 255;
+                        const int m = pk ? row & 7 : (code >> 8) * 16 + row;
+                        if (m <= n && n <= pt) {
                             const double U = (double)accu[uu][cd][r], V = (double)accv[uu][cd][r];
                             const double xp = m == 0 ? U : 0.5 * (U + V);
                             const double xm = (m & 1) ? -0.5 * (U - V) : 0.5 * (U - V);
@@ -959,15 +976,17 @@ void launch_one(const M2LBlockArgs* a, size_t shm, hipStream_t s, const Persiste
     hipLaunchKernelGGL((m2l_block_kernel<T, TB, G, KT, MAXU>), dim3(a->n_items), dim3(256), shm, s, *a);
 }
 
-// (KT, MAXU) follow from the table degree: KT = m2l_tiles(p + 1), MAXU = m2l_block_maxu(p)
+// (KT, MAXU) follow from the table degree: KT = m2l_tiles(p + 1), MAXU = m2l_block_maxu(p),
+// or m2l_block_maxu_packed(p) for the FP32 one-tile instances (the host packs their units)
 template <typename T, int TB, int G>
 void launch_p(const M2LBlockArgs* a, size_t shm, int p, hipStream_t s, const PersistentLaunch* pl) {
+    constexpr bool PK = sizeof(T) == 4;
     if (p <= 7)
-        launch_one<T, TB, G, 1, m2l_block_maxu(7)>(a, shm, s, pl);
+        launch_one<T, TB, G, 1, PK ? m2l_block_maxu_packed(7) : m2l_block_maxu(7)>(a, shm, s, pl);
     else if (p <= 11)
-        launch_one<T, TB, G, 1, m2l_block_maxu(11)>(a, shm, s, pl);
+        launch_one<T, TB, G, 1, PK ? m2l_block_maxu_packed(11) : m2l_block_maxu(11)>(a, shm, s, pl);
     else if (p <= 15)
-        launch_one<T, TB, G, 1, m2l_block_maxu(15)>(a, shm, s, pl);
+        launch_one<T, TB, G, 1, PK ? m2l_block_maxu_packed(15) : m2l_block_maxu(15)>(a, shm, s, pl);
     else if (p <= 17)
         launch_one<T, TB, G, 2, m2l_block_maxu(17)>(a, shm, s, pl);
     else if (p <= 19)

@@ -136,6 +136,7 @@ struct hfmm_plan {
     float *d_rf1_32 = nullptr, *d_rf3_32 = nullptr;
     int *d_rf_off1 = nullptr, *d_rf_off3 = nullptr;
     int* d_rf_off1q = nullptr;   // forward rotation blocks of at most 4 rows: their 4 x 4 fragments
+    int* d_rf_off3p = nullptr;   // packed stage-3 units: [8 n1 + n2]
     int rf_stride1 = 0, rf_stride3 = 0;
     float2* d_phase32 = nullptr;
     float2* dM32 = nullptr;
@@ -201,13 +202,26 @@ int octant(uint64_t code) { return (int)(code & 7); }
 // wave gets about the same number of MFMAs: units sorted by cost, each row's units given,
 // largest first, to the waves with the least work so far.  Rows are padded with code 255,
 // which every stage skips (a degree or order above any table).
-void make_units(int ps, int pt, std::vector<int>& units1, std::vector<int>& units2, std::vector<int>& units3) {
+void make_units(int ps, int pt, std::vector<int>& units1, std::vector<int>& units2, std::vector<int>& units3,
+                bool pack3 = false) {
     std::vector<std::pair<int, int>> u1, u2, u3;   // (cost, code)
     for (int n = 0; n <= ps; ++n) {
         u1.push_back({m2l_tiles(n + 1) * ((n + 4) / 4), n});                   // u block
         if (n > 0) u1.push_back({m2l_tiles(n) * ((n + 3) / 4), n | (1 << 8)});  // v block
     }
-    for (int n = 0; n <= pt; ++n)
+    int n3 = 0;
+    if (pack3) {   // degrees n1 + n2 = q share a unit (rows 0..7: n1, rows 8..15: n2)
+        const int q = std::min(pt, kM2LPackMax);
+        for (int n1 = 0; n1 <= q - n1; ++n1) {
+            const int n2 = q - n1;
+            if (n1 < n2)
+                u3.push_back({(n1 + n2 + 5) / 4 + (n1 + n2 + 3) / 4 + 2, 256 | (n1 << 4) | n2});
+            else
+                u3.push_back({(n1 + 4) / 4 + (n1 + 3) / 4 + 2, n1});
+        }
+        n3 = q + 1;
+    }
+    for (int n = n3; n <= pt; ++n)
         for (int mt = 0; mt < m2l_tiles(n + 1); ++mt)
             u3.push_back({(n + 4) / 4 + (n + 3) / 4 + 2, n | (mt << 8)});   // + phase epilogue
     for (int m = 0; m <= std::min(ps, pt); ++m) {
@@ -336,14 +350,14 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     if (p > kM2LBlockMaxP) return;   // higher degrees use the generic kernel (coarse levels only)
     const int fp32 = P.m2l_fp32[L];
     const int eb = fp32 ? 4 : 8;
-    const int maxu = m2l_block_maxu(p);   // the instantiation's stage-3 units per wave
+    // the instantiation's stage-3 units per wave (FP32 one-tile instances pack small degrees)
+    const int maxu = fp32 && m2l_tiles(p + 1) == 1 ? m2l_block_maxu_packed(p) : m2l_block_maxu(p);
     // shape of one operator application: TB target boxes x G directions (of equal theta
     // and |d|).  A requested pair count TB*G (m2l_kernel = 8, 16, 32, 64) takes the first
     // fitting shape with that count in the order below (larger TB first: partly filled
     // direction groups waste columns, target blocks rarely do).  Otherwise, from
     // measurements at N = 1M, k = 40 (finest level, 5.4M pairs): FP32 takes 16 x 2 up to
-    // p = 11 (5.27 ms at p = 11 against 5.61 for 8 x 2; 3 waves per SIMD) and 8 x 2 above
-    // (6.42 ms at p = 12 against 7.02 for 16 x 2, which falls to 2 waves per SIMD); FP64
+    // p = 11 (5.27 ms at p = 11 against 5.61 for 8 x 2) and 8 x 2 above (see below); FP64
     // takes 8 x 1 (38.9 ms at p = 17 against 48.4 for 8 x 2); then the next smaller shape
     // that fits.
     auto fits = [&](int c, int g) {
@@ -356,15 +370,19 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
     if (P.m2l_kernel > 1)
         for (auto& sh : shapes)
             if (!tb && sh[0] * sh[1] == P.m2l_kernel && fits(sh[0], sh[1])) tb = sh[0], grp = sh[1];
+    // with the packed stage-3 units the one-tile FP32 instances fit 16 x 2 up to p = 15:
+    // at p = 12 on the finest level (32768 targets) 5.20 ms against 6.0 for 8 x 2, but on
+    // the 4096 targets of level 4 0.96 against 0.80 ms (fewer, longer work items)
     const int pref[3][2][2] = {{{16, 2}, {8, 2}}, {{8, 2}, {8, 1}}, {{8, 1}, {8, 2}}};
-    const int* pr = &pref[!fp32 ? 2 : p <= 11 ? 0 : 1][0][0];
+    const bool wide = p <= 11 || (p <= 15 && targets.size() >= 16384);
+    const int* pr = &pref[!fp32 ? 2 : wide ? 0 : 1][0][0];
     for (int i = 0; i < 2 && !tb; ++i)
         if (fits(pr[2 * i], pr[2 * i + 1])) tb = pr[2 * i], grp = pr[2 * i + 1];
     for (auto& sh : shapes)
         if (!tb && fits(sh[0], sh[1])) tb = sh[0], grp = sh[1];
     if (!tb) return;
     std::vector<int> units1, units2, units3;
-    make_units(p, p, units1, units2, units3);
+    make_units(p, p, units1, units2, units3, fp32 && m2l_tiles(p + 1) == 1);
 
     const Tree& t = P.tree;
     std::vector<int> tpos(t.n_boxes(), -1);     // target ordinal in `targets`
@@ -615,6 +633,16 @@ void build_tables(hfmm_plan& P) {
                     s1 += ((len + 3) / 4) * ((len + 3) / 4) * 256;
                 }
             }
+        // packed stage-3 units (n1 < n2, n1 + n2 <= kM2LPackMax): a 16 x 16 block-diagonal
+        // fragment each for u (rows 0..n1 and 8..8+n2; inputs n1's n1+1, then n2's n2+1) and
+        // for v (same rows; inputs n1's n1, then n2's n2)
+        std::vector<int> off3p(64, -1);
+        for (int n1 = 0; n1 <= kM2LPackMax; ++n1)
+            for (int n2 = n1 + 1; n1 + n2 <= kM2LPackMax; ++n2)
+                if (n2 <= pb) {
+                    off3p[8 * n1 + n2] = s3;
+                    s3 += 512;
+                }
         std::vector<double> f1((size_t)thetas.size() * s1, 0.0), f3((size_t)thetas.size() * s3, 0.0);
         // A fragments of an R x K matrix a(i, k) at dst
         auto put = [](double* dst, int R, int K, const std::function<double(int, int)>& a) {
@@ -639,7 +667,8 @@ void build_tables(hfmm_plan& P) {
                         }
         };
         std::vector<double> Wu, Wv;
-        for (size_t th = 0; th < thetas.size(); ++th)
+        std::vector<std::vector<double>> Wu3(pb + 1), Wv3(pb + 1);   // stage-3 blocks of one rotation
+        for (size_t th = 0; th < thetas.size(); ++th) {
             for (int n = 0; n <= pb; ++n) {
                 const double* E = &rot[th * rsz + rot_offset(n)];
                 for (int st = 0; st < 2; ++st) {
@@ -660,14 +689,39 @@ void build_tables(hfmm_plan& P) {
                         if (n > 0)
                             put(b + off3[2 * n + 1], n + 1, n,
                                 [&](int i, int k) { return i == 0 ? 0.0 : wv(i, n - k); });
+                        Wu3[n] = Wu;
+                        Wv3[n] = Wv;
                     }
                 }
             }
+            double* b3 = &f3[th * s3];
+            for (int n1 = 0; n1 <= kM2LPackMax; ++n1)
+                for (int n2 = n1 + 1; n1 + n2 <= kM2LPackMax; ++n2) {
+                    if (off3p[8 * n1 + n2] < 0) continue;
+                    const std::vector<double>&U1 = Wu3[n1], &U2 = Wu3[n2], &V1 = Wv3[n1], &V2 = Wv3[n2];
+                    put(b3 + off3p[8 * n1 + n2], 16, n1 + n2 + 2, [&](int i, int k) {
+                        if (i < 8) return i <= n1 && k <= n1 ? U1[(size_t)i * (n1 + 1) + k] : 0.0;
+                        const int ii = i - 8, kk = k - n1 - 1;
+                        return ii <= n2 && kk >= 0 ? U2[(size_t)ii * (n2 + 1) + kk] : 0.0;
+                    });
+                    // v: row m (0: zero), input k in LDS order (v_m at k = n - m)
+                    put(b3 + off3p[8 * n1 + n2] + 256, 16, n1 + n2, [&](int i, int k) {
+                        if (i < 8) {
+                            if (i == 0 || i > n1 || k >= n1) return 0.0;
+                            return V1[(size_t)(i - 1) * n1 + (n1 - k - 1)];
+                        }
+                        const int ii = i - 8, kk = k - n1;
+                        if (ii == 0 || ii > n2 || kk < 0) return 0.0;
+                        return V2[(size_t)(ii - 1) * n2 + (n2 - kk - 1)];
+                    });
+                }
+        }
         P.rf_stride1 = s1;
         P.rf_stride3 = s3;
         P.d_rf_off1 = P.upload(off1);
         P.d_rf_off1q = P.upload(off1q);
         P.d_rf_off3 = P.upload(off3);
+        P.d_rf_off3p = P.upload(off3p);
         if (P.precision == HFMM_PREC_MIXED) {
             P.d_rf1_32 = P.upload(std::vector<float>(f1.begin(), f1.end()));
             P.d_rf3_32 = P.upload(std::vector<float>(f3.begin(), f3.end()));
@@ -1064,6 +1118,7 @@ M2LBlockArgs block_args(const hfmm_plan& P, const M2LBlockLevel& B, bool fp32, c
     a.rot_off1 = P.d_rf_off1;
     a.rot_off1q = P.d_rf_off1q;
     a.rot_off3 = P.d_rf_off3;
+    a.rot_off3p = P.d_rf_off3p;
     a.rot_stride1 = P.rf_stride1;
     a.rot_stride3 = P.rf_stride3;
     a.phase = fp32 ? (const void*)P.d_phase32 : (const void*)P.d_phase;
