@@ -213,12 +213,14 @@ __device__ __forceinline__ void m2l_block_body(const M2LBlockArgs& a, const int4
     __syncthreads();
 
     for (int e = item.y; e < item.z; ++e) {
-        // lane-derived offsets are formed again in every group from an opaque copy of the
-        // lane index: kept live across the loop they would be spilled (4 waves per SIMD
-        // leave 128 registers) and reloaded from scratch in every group
+        // FP32 one-tile instances (4 waves per SIMD, 128 registers) form the lane-derived
+        // offsets again in every group from an opaque copy of the lane index: kept live across
+        // the loop they would be spilled and reloaded from scratch in every group.  The other
+        // instances keep them (the FP64 ones ran 14% slower with the recomputation).
         int zero_o = 0;
-        asm volatile("" : "+v"(zero_o));
-        const int lane_o = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, (unsigned)zero_o));
+        if constexpr (PF) asm volatile("" : "+v"(zero_o));
+        const int lane_o =
+            PF ? __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, (unsigned)zero_o)) : lane;
         const int li = lane_o & 15, lk = lane_o >> 4;
         const C2* ps_lane = Ps + (SPL ? 0 : (li & 1)) * G * TL + Mma<T>::row(lk, 0);
         const unsigned short* gd = a.grp_dir + (size_t)e * G;
