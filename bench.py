@@ -128,8 +128,10 @@ def dtype_label(st):
         return "f64"
     lv = st.m2l_fp32_levels
     m2l = (f"f32-MFMA M2L at levels {lv[0]}-{lv[-1]}" if lv else "f64-MFMA M2L")
-    return (f"mixed: {m2l}, f32 P2P weights and phase from f64 coordinate differences"
-            f" ({info.n_leaves_phase64} leaves with f64 phase), f64 P2M/M2M/L2L/L2P and accumulation")
+    tl = st.tree_fp32_levels
+    tree = (f"f32-MFMA M2M into and L2L out of levels {tl[0]}-{tl[-1]}" if tl else "f64-MFMA M2M/L2L")
+    return (f"mixed: {m2l}, {tree}, f32 P2P weights and phase from f64 coordinate differences"
+            f" ({info.n_leaves_phase64} leaves with f64 phase), f64 P2M/L2P and accumulation")
 
 
 def host_cores():

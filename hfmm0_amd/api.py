@@ -57,6 +57,12 @@ class HFMMState:
         return [i for i in range(1, self.info.depth + 1) if (m >> i) & 1]
 
     @property
+    def tree_fp32_levels(self):
+        """Levels l whose M2M (into l) and L2L (out of l) run FP32 arithmetic (mixed precision)."""
+        m = self.info.tree_fp32_levels
+        return [i for i in range(1, self.info.depth + 1) if (m >> i) & 1]
+
+    @property
     def p_table(self):
         """Degree the tables of levels 1..depth were built for."""
         return [self.info.p_table[i] for i in range(1, self.info.depth + 1)]

@@ -127,7 +127,7 @@ struct hfmm_plan {
     XlateLaunch m2l[kMaxLevels], m2m[kMaxLevels], l2l[kMaxLevels];
     M2LBlockLevel m2lb[kMaxLevels];
     bool m2l_fp32[kMaxLevels] = {};  // M2L of the level in FP32 (mixed precision)
-    M2LBlockLevel m2mb[kMaxLevels], l2lb[kMaxLevels];   // M2M / L2L on matrix cores (FP64)
+    M2LBlockLevel m2mb[kMaxLevels], l2lb[kMaxLevels];   // M2M / L2L on matrix cores
     int m2l_kernel = 0;
     int p2p_tile = 2;               // targets per lane of the mixed near field
     size_t max_lds = 65536;
@@ -140,6 +140,7 @@ struct hfmm_plan {
     int rf_stride1 = 0, rf_stride3 = 0;
     float2* d_phase32 = nullptr;
     float2* dM32 = nullptr;
+    float2* dL32 = nullptr;         // FP32 copy of the locals that FP32 L2L translations read
     double2 *dM = nullptr, *dL = nullptr, *d_q = nullptr, *d_far = nullptr;
     void* d_qin = nullptr;          // staging for host input
     double2* d_out = nullptr;       // staging for host output
@@ -456,8 +457,8 @@ void build_m2l_blocks(hfmm_plan& P, int L, const std::vector<int>& targets, cons
 }
 
 // M2M (kind 1: the parents of level L from their children) and L2L (kind 2: the children
-// at level L+1 from their parents) on the matrix-core block kernel, in FP64 at every
-// precision (these expansions feed every coarser or finer level).  Blocks of 8 targets,
+// at level L+1 from their parents) on the matrix-core block kernel, in FP32 or FP64 (see
+// below; the results are FP64 either way).  Blocks of 8 targets,
 // one direction per group: an M2M block runs one group per child octant present among
 // its parents, every target receiving all of its children within the one work item, so
 // the result is stored (no clear of the parent multipoles is needed); an L2L block holds
@@ -470,8 +471,13 @@ void build_tree_blocks(hfmm_plan& P, int L, int kind, const std::vector<double>&
     const int pt = kind == 1 ? P.p_level[L] : P.p_level[L + 1];
     const int pm = std::max(ps, pt);
     if (pm > kM2LBlockMaxP) return;
+    // FP32 arithmetic (FP64 results) where the M2L of every level from 2 to L+1 runs in FP32:
+    // the translated expansions then only feed FP32 translations, whose own rounding is of
+    // the same size; a chain through an FP64 M2L level keeps FP64
+    bool fp32 = P.d_rf1_32 != nullptr;
+    for (int l = 2; l <= L + 1; ++l) fp32 = fp32 && P.m2l_fp32[l] && P.p_level[l] <= kM2LBlockMaxP;
     const int tb = 8, grp = 1;
-    if (m2l_block_lds_bytes(pm, tb, grp, 8) > P.max_lds) return;
+    if (m2l_block_lds_bytes(pm, tb, grp, fp32 ? 4 : 8) > P.max_lds) return;
     std::vector<int> blk_targets, blk_ptr{0}, grp_off;
     std::vector<unsigned short> grp_dir;
     if (kind == 1) {
@@ -523,9 +529,10 @@ void build_tree_blocks(hfmm_plan& P, int L, int kind, const std::vector<double>&
     const bool split = kind == 1 && nblk < 512;
     const std::vector<int4> items = make_items(t, blk_targets, blk_ptr, tb, split);
     std::vector<int> units1, units2, units3;
-    make_units(ps, pt, units1, units2, units3);
+    make_units(ps, pt, units1, units2, units3, fp32 && m2l_tiles(pm + 1) == 1);
     M2LBlockLevel& B = kind == 1 ? P.m2mb[L] : P.l2lb[L];
     B.use = true;
+    B.fp32 = fp32;
     B.tb = tb;
     B.grp = grp;
     B.ps = ps;
@@ -554,10 +561,15 @@ void build_tree_blocks(hfmm_plan& P, int L, int kind, const std::vector<double>&
     std::vector<double> cf;
     std::vector<int> coff, coffq;
     B.coax_stride = coax_fragments(tab, 1, ps, pt, cf, coff, coffq);
-    B.coax = P.upload(cf);
+    if (fp32) {
+        std::vector<float> c32(cf.begin(), cf.end());
+        B.coax = P.upload(c32);
+    } else {
+        B.coax = P.upload(cf);
+    }
     B.coax_off = P.upload(coff);
     B.coax_offq = P.upload(coffq);
-    prepare_m2l_block(pm, tb, grp, 0);
+    prepare_m2l_block(pm, tb, grp, fp32);
 }
 
 void build_tables(hfmm_plan& P) {
@@ -726,7 +738,7 @@ void build_tables(hfmm_plan& P) {
             P.d_rf1_32 = P.upload(std::vector<float>(f1.begin(), f1.end()));
             P.d_rf3_32 = P.upload(std::vector<float>(f3.begin(), f3.end()));
         }
-        P.d_rf1 = P.upload(f1);   // FP64: M2L at FP64, M2M and L2L at every precision
+        P.d_rf1 = P.upload(f1);   // FP64: M2L, M2M and L2L of FP64 levels
         P.d_rf3 = P.upload(f3);
     }
     std::vector<double2> phase((size_t)kNumDirs * (pmax + 1));
@@ -1059,10 +1071,14 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
         if (t.is_leaf[b]) P.p_leafmax = std::max(P.p_leafmax, P.p_level[t.level[b]]);
     P.p2m_chunk = p2m_chunk(P.p_leafmax);
     P.dM = P.alloc<double2>(P.n_coef);
-    bool any_fp32 = false;
-    for (int L = 0; L < kMaxLevels; ++L) any_fp32 = any_fp32 || (P.m2lb[L].use && P.m2lb[L].fp32);
+    bool any_fp32 = false, any_l2l32 = false;
+    for (int L = 0; L < kMaxLevels; ++L) {
+        any_fp32 = any_fp32 || (P.m2lb[L].use && P.m2lb[L].fp32) || (P.m2mb[L].use && P.m2mb[L].fp32);
+        any_l2l32 = any_l2l32 || (P.l2lb[L].use && P.l2lb[L].fp32);
+    }
     if (any_fp32) P.dM32 = P.alloc<float2>(P.n_coef);
     P.dL = P.alloc<double2>(P.n_coef);
+    if (any_l2l32) P.dL32 = P.alloc<float2>(P.n_coef);
     P.d_q = P.alloc<double2>(n);
     P.d_far = P.alloc<double2>(n);
     P.d_qin = P.alloc<double2>(n);
@@ -1143,6 +1159,15 @@ constexpr int kSeqDown = 8;        // L2L, L2P
 constexpr int kSeqP2P = 16;        // near field and output
 constexpr int kSeqAll = 31;
 
+// first coefficient of the expansions of tree level l (boxes are in level order); n_coef
+// past the deepest level
+int64_t level_coef(const hfmm_plan& P, int l) {
+    const Tree& t = P.tree;
+    if (l > t.depth) return P.n_coef;
+    const int b = t.level_start[l];
+    return b < t.n_boxes() ? P.exp_off[b] : P.n_coef;
+}
+
 int out_mode_of(const hfmm_plan& P, int q_type) {
     return (P.conj ? 1 : 0) | ((P.k == 0.0 && real_charges(q_type)) ? 2 : 0);
 }
@@ -1186,20 +1211,28 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
         mark(1);
         launch_p2m(P.d_leafdesc, P.n_far_leaves, P.p2m_chunk, lv, P.dX, P.dY, P.dZ, P.d_q, P.k, P.p_leafmax, P.dM, s);
         mark(2);
+        int conv = P.depth + 1;   // levels >= conv have their multipoles in FP32 already
         for (int L = P.depth - 1; L >= 2; --L) {
             if (P.m2mb[L].use) {
                 const M2LBlockLevel& B = P.m2mb[L];
+                if (B.fp32) {   // the children's multipoles (complete now) in FP32
+                    const int64_t c0 = level_coef(P, L + 1), c1 = level_coef(P, L + 2);
+                    launch_to_float(P.dM + c0, P.dM32 + c0, c1 - c0, s);
+                    conv = L + 1;
+                }
                 if (B.split) launch_zero_boxes(B.zero, B.n_zero, P.d_expoff, ncoef(P.p_eff[L]), P.dM, s);
-                launch_m2l_block(block_args(P, B, false, P.dM, P.dM, B.split ? 0 : 1), B.tb, B.grp, 0, s);
+                launch_m2l_block(block_args(P, B, B.fp32, B.fp32 ? (const void*)P.dM32 : (const void*)P.dM, P.dM,
+                                            B.split ? 0 : 1),
+                                 B.tb, B.grp, B.fp32, s);
             }
             else if (P.m2m[L].n_targets)
                 launch_xlate(xargs(P, P.m2m[L], P.dM, P.dM, 0), P.m2m[L].n_targets, s);
         }
         mark(3);
-        // multipoles in FP32 for the FP32 block levels
+        // multipoles in FP32 for the FP32 block levels (those not converted for FP32 M2M)
         bool any_fp32 = false;
         for (int L = 2; L <= P.depth; ++L) any_fp32 |= P.m2l[L].n_targets && P.m2lb[L].use && P.m2lb[L].fp32;
-        if (any_fp32) launch_to_float(P.dM, P.dM32, P.n_coef, s);
+        if (any_fp32) launch_to_float(P.dM, P.dM32, level_coef(P, conv), s);
     }
     for (int L = 2; L <= P.depth; ++L) {
         if (!P.m2l[L].n_targets) continue;
@@ -1227,8 +1260,13 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
     const int out_mode = out_mode_of(P, q_type);
     if (parts & kSeqDown) {
         for (int L = 2; L < P.depth; ++L) {
-            if (P.l2lb[L].use)
-                launch_m2l_block(block_args(P, P.l2lb[L], false, P.dL, P.dL, 0), P.l2lb[L].tb, P.l2lb[L].grp, 0, s);
+            const M2LBlockLevel& B = P.l2lb[L];
+            if (B.use && B.fp32) {   // the parents' locals (complete now) in FP32
+                const int64_t c0 = level_coef(P, L), c1 = level_coef(P, L + 1);
+                launch_to_float(P.dL + c0, P.dL32 + c0, c1 - c0, s);
+                launch_m2l_block(block_args(P, B, true, P.dL32, P.dL, 0), B.tb, B.grp, 1, s);
+            } else if (B.use)
+                launch_m2l_block(block_args(P, B, false, P.dL, P.dL, 0), B.tb, B.grp, 0, s);
             else if (P.l2l[L].n_targets)
                 launch_xlate(xargs(P, P.l2l[L], P.dL, P.dL, 1), P.l2l[L].n_targets, s);
         }
@@ -1376,6 +1414,8 @@ int hfmm_plan_info_get(const hfmm_plan* P, hfmm_plan_info* info) {
             info->p_level[L] = (L >= 1 && L <= P->depth) ? P->p_eff[L] : 0;
             info->p_table[L] = (L >= 1 && L <= P->depth) ? P->p_level[L] : 0;
             if (P->m2lb[L].use && P->m2lb[L].fp32) info->m2l_fp32_levels |= 1 << L;
+            if ((P->m2mb[L].use && P->m2mb[L].fp32) || (P->l2lb[L].use && P->l2lb[L].fp32))
+                info->tree_fp32_levels |= 1 << L;
         }
         info->extent = P->tree.extent;
         for (int a = 0; a < 3; ++a) info->pmin[a] = P->tree.pmin[a];

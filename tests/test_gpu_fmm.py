@@ -200,7 +200,11 @@ def test_matrix_core_m2l_fp32(hf):
     x, q = D.uniform_problem(30000)
     idx = D.sample_targets(30000, 500)
     ref = D.direct_c(x, q, 20.0, idx)
-    a = hf.evaluate(hf.setup(x, 20.0, p=14, N_max=32, precision="mixed"), q)
+    st = hf.setup(x, 20.0, p=14, N_max=32, precision="mixed")
+    # k D <= 32 at every level: M2L, and with it M2M and L2L, in FP32
+    assert st.m2l_fp32_levels == list(range(2, st.depth + 1))
+    assert st.tree_fp32_levels == list(range(2, st.depth))
+    a = hf.evaluate(st, q)
     b = hf.evaluate(hf.setup(x, 20.0, p=14, N_max=32, m2l_kernel="generic"), q)
     ea, eb = D.relative_error(a[idx], ref), D.relative_error(b[idx], ref)
     say(f"  mixed {ea:.3e} fp64 {eb:.3e} diff {D.relative_error(a, b):.3e}")
@@ -224,6 +228,8 @@ def test_mixed_precision_per_level(hf):
     want = [lv for lv in levels if k * 2 * np.sqrt(3) * ext * 0.5 ** (lv + 1) <= api.MIXED_FP32_MAX_KD]
     say(f"  depth {auto.depth} fp32 levels {auto.m2l_fp32_levels} want {want}")
     assert auto.m2l_fp32_levels == want and 2 not in want
+    # M2M and L2L take FP32 only where every M2L level from 2 on is FP32: none here
+    assert auto.tree_fp32_levels == []
     ref = hf.evaluate(hf.setup(x, k, precision="fp64", **base), q)
     all64 = hf.setup(x, k, precision="mixed", m2l_fp64_levels=levels, **base)
     assert all64.m2l_fp32_levels == []
@@ -232,10 +238,10 @@ def test_mixed_precision_per_level(hf):
     assert all64.info.n_leaves_phase64 == all64.info.n_leaves > 0
     d_all64 = D.relative_error(hf.evaluate(all64, q), ref)
     one = hf.setup(x, k, precision="mixed", m2l_fp64_levels=[lv for lv in levels if lv != 3], **base)
-    assert one.m2l_fp32_levels == [3]
+    assert one.m2l_fp32_levels == [3] and one.tree_fp32_levels == []
     d_one = D.relative_error(hf.evaluate(one, q), ref)
     none64 = hf.setup(x, k, precision="mixed", m2l_fp64_levels=[], **base)
-    assert none64.m2l_fp32_levels == levels
+    assert none64.m2l_fp32_levels == levels and none64.tree_fp32_levels == levels[:-1]
     d_none = D.relative_error(hf.evaluate(none64, q), ref)
     say(f"  vs fp64 plan: all fp64 {d_all64:.2e} level 3 fp32 {d_one:.2e} all fp32 {d_none:.2e}")
     assert d_all64 < 3e-7
