@@ -40,7 +40,8 @@ class PlanInfo(ctypes.Structure):
                 ("n_p2p_pairs", ctypes.c_int64), ("p_level", ctypes.c_int * MAX_LEVELS),
                 ("extent", ctypes.c_double), ("pmin", ctypes.c_double * 3),
                 ("p_table", ctypes.c_int * MAX_LEVELS), ("m2l_fp32_levels", ctypes.c_int),
-                ("n_leaves_phase64", ctypes.c_int), ("tree_fp32_levels", ctypes.c_int)]
+                ("n_leaves_phase64", ctypes.c_int), ("m2m_fp32_levels", ctypes.c_int),
+                ("l2l_fp32_levels", ctypes.c_int)]
 
 
 class HFMMError(RuntimeError):

@@ -57,9 +57,15 @@ class HFMMState:
         return [i for i in range(1, self.info.depth + 1) if (m >> i) & 1]
 
     @property
-    def tree_fp32_levels(self):
-        """Levels l whose M2M (into l) and L2L (out of l) run FP32 arithmetic (mixed precision)."""
-        m = self.info.tree_fp32_levels
+    def m2m_fp32_levels(self):
+        """Levels l whose M2M (into l, from l+1) runs FP32 arithmetic (mixed precision)."""
+        m = self.info.m2m_fp32_levels
+        return [i for i in range(1, self.info.depth + 1) if (m >> i) & 1]
+
+    @property
+    def l2l_fp32_levels(self):
+        """Levels l whose L2L (out of l, into l+1) runs FP32 arithmetic (mixed precision)."""
+        m = self.info.l2l_fp32_levels
         return [i for i in range(1, self.info.depth + 1) if (m >> i) & 1]
 
     @property

@@ -473,11 +473,13 @@ void build_tree_blocks(hfmm_plan& P, int L, int kind, const std::vector<double>&
     if (pm > kM2LBlockMaxP) return;
     // FP32 arithmetic (FP64 results) where the M2L of every level from 2 to L+1 runs in FP32:
     // the translated expansions then only feed FP32 translations, whose own rounding is of
-    // the same size; a chain through an FP64 M2L level keeps FP64
+    // the same size; a chain through an FP64 M2L level keeps FP64.  Split M2M levels (below)
+    // keep FP64 as well: there the FP32 form was slower (N=1M, k=40: 55.7 against 40.4 us
+    // for level 4 -> 3, where the level 5 -> 4 M2M took 81.9 against 113.5 us).
     bool fp32 = P.d_rf1_32 != nullptr;
     for (int l = 2; l <= L + 1; ++l) fp32 = fp32 && P.m2l_fp32[l] && P.p_level[l] <= kM2LBlockMaxP;
     const int tb = 8, grp = 1;
-    if (m2l_block_lds_bytes(pm, tb, grp, fp32 ? 4 : 8) > P.max_lds) return;
+    if (m2l_block_lds_bytes(pm, tb, grp, 8) > P.max_lds) return;
     std::vector<int> blk_targets, blk_ptr{0}, grp_off;
     std::vector<unsigned short> grp_dir;
     if (kind == 1) {
@@ -527,6 +529,7 @@ void build_tree_blocks(hfmm_plan& P, int L, int kind, const std::vector<double>&
     // that add atomically, after the parents are cleared; L2L blocks hold one group
     const int nblk = (int)blk_ptr.size() - 1;
     const bool split = kind == 1 && nblk < 512;
+    if (split) fp32 = false;
     const std::vector<int4> items = make_items(t, blk_targets, blk_ptr, tb, split);
     std::vector<int> units1, units2, units3;
     make_units(ps, pt, units1, units2, units3, fp32 && m2l_tiles(pm + 1) == 1);
@@ -1414,8 +1417,8 @@ int hfmm_plan_info_get(const hfmm_plan* P, hfmm_plan_info* info) {
             info->p_level[L] = (L >= 1 && L <= P->depth) ? P->p_eff[L] : 0;
             info->p_table[L] = (L >= 1 && L <= P->depth) ? P->p_level[L] : 0;
             if (P->m2lb[L].use && P->m2lb[L].fp32) info->m2l_fp32_levels |= 1 << L;
-            if ((P->m2mb[L].use && P->m2mb[L].fp32) || (P->l2lb[L].use && P->l2lb[L].fp32))
-                info->tree_fp32_levels |= 1 << L;
+            if (P->m2mb[L].use && P->m2mb[L].fp32) info->m2m_fp32_levels |= 1 << L;
+            if (P->l2lb[L].use && P->l2lb[L].fp32) info->l2l_fp32_levels |= 1 << L;
         }
         info->extent = P->tree.extent;
         for (int a = 0; a < 3; ++a) info->pmin[a] = P->tree.pmin[a];

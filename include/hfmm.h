@@ -96,9 +96,13 @@ typedef struct hfmm_plan_info {
                                        the phase k r in FP64 (pairs reaching k r > 8, where an FP32
                                        phase errs by ~k r 2^-23 rad); every leaf takes coordinate
                                        differences and r^2 in FP64 */
-    int tree_fp32_levels;           /* bit l set: the M2M into level l and the L2L out of level l
-                                       run FP32 MFMA arithmetic (FP64 results); mixed precision,
-                                       where the M2L of every level from 2 to l+1 is FP32 */
+    int m2m_fp32_levels;            /* bit l set: the M2M into level l runs FP32 MFMA arithmetic
+                                       (FP64 results); mixed precision, where the M2L of every
+                                       level from 2 to l+1 is FP32 and the level is not split
+                                       over atomic work items */
+    int l2l_fp32_levels;            /* bit l set: the L2L out of level l runs FP32 MFMA arithmetic
+                                       (FP64 results), where the M2L of every level from 2 to l+1
+                                       is FP32 */
 } hfmm_plan_info;
 
 void hfmm_opts_default(hfmm_opts* o);

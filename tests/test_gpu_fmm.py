@@ -201,9 +201,10 @@ def test_matrix_core_m2l_fp32(hf):
     idx = D.sample_targets(30000, 500)
     ref = D.direct_c(x, q, 20.0, idx)
     st = hf.setup(x, 20.0, p=14, N_max=32, precision="mixed")
-    # k D <= 32 at every level: M2L, and with it M2M and L2L, in FP32
+    # k D <= 32 at every level: M2L, and with it L2L and the M2M of levels with at least
+    # 512 blocks of 8 parents (none at this size; those with fewer split and stay FP64), in FP32
     assert st.m2l_fp32_levels == list(range(2, st.depth + 1))
-    assert st.tree_fp32_levels == list(range(2, st.depth))
+    assert st.l2l_fp32_levels == list(range(2, st.depth)) and st.m2m_fp32_levels == []
     a = hf.evaluate(st, q)
     b = hf.evaluate(hf.setup(x, 20.0, p=14, N_max=32, m2l_kernel="generic"), q)
     ea, eb = D.relative_error(a[idx], ref), D.relative_error(b[idx], ref)
@@ -229,7 +230,7 @@ def test_mixed_precision_per_level(hf):
     say(f"  depth {auto.depth} fp32 levels {auto.m2l_fp32_levels} want {want}")
     assert auto.m2l_fp32_levels == want and 2 not in want
     # M2M and L2L take FP32 only where every M2L level from 2 on is FP32: none here
-    assert auto.tree_fp32_levels == []
+    assert auto.l2l_fp32_levels == [] and auto.m2m_fp32_levels == []
     ref = hf.evaluate(hf.setup(x, k, precision="fp64", **base), q)
     all64 = hf.setup(x, k, precision="mixed", m2l_fp64_levels=levels, **base)
     assert all64.m2l_fp32_levels == []
@@ -238,10 +239,10 @@ def test_mixed_precision_per_level(hf):
     assert all64.info.n_leaves_phase64 == all64.info.n_leaves > 0
     d_all64 = D.relative_error(hf.evaluate(all64, q), ref)
     one = hf.setup(x, k, precision="mixed", m2l_fp64_levels=[lv for lv in levels if lv != 3], **base)
-    assert one.m2l_fp32_levels == [3] and one.tree_fp32_levels == []
+    assert one.m2l_fp32_levels == [3] and one.l2l_fp32_levels == []
     d_one = D.relative_error(hf.evaluate(one, q), ref)
     none64 = hf.setup(x, k, precision="mixed", m2l_fp64_levels=[], **base)
-    assert none64.m2l_fp32_levels == levels and none64.tree_fp32_levels == levels[:-1]
+    assert none64.m2l_fp32_levels == levels and none64.l2l_fp32_levels == levels[:-1]
     d_none = D.relative_error(hf.evaluate(none64, q), ref)
     say(f"  vs fp64 plan: all fp64 {d_all64:.2e} level 3 fp32 {d_one:.2e} all fp32 {d_none:.2e}")
     assert d_all64 < 3e-7
