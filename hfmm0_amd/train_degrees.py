@@ -1,5 +1,7 @@
 """Generates the learned-degree training data with the FMM itself and trains the MLP on
 the GPU.  Run on a GPU:  python -m hfmm0_amd.train_degrees --problems 200 --out DIR
+(--seed0 S labels the problems S, S+1, ...; --label-only stops after writing the data;
+--data A.npz,B.npz trains on those labelled sets, in order, after the ones generated).
 
 For every seeded problem (hfmm0_amd.workloads.mixed_problem, N log-uniform in [10k,
 200k], k uniform in [1, 80], leaf size drawn from 32/64/128) it builds one plan at the
@@ -81,6 +83,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--budget", type=float, default=6e-7, help="total error budget of the labels")
     ap.add_argument("--calibrate", action="store_true", help="measure the per-pair M2L cost table first")
+    ap.add_argument("--seed0", type=int, default=1000, help="seed of the first generated problem")
+    ap.add_argument("--label-only", action="store_true", help="write the labelled data, do not train")
+    ap.add_argument("--data", default="", help="comma-separated degree_data.npz files to train on as well")
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
     if args.calibrate:
@@ -88,9 +93,23 @@ def main():
         import shutil
         from .learned import DEFAULT_COSTS
         shutil.copy(os.path.join(args.out, "m2l_cost.json"), DEFAULT_COSTS)
-    X, p, meta = generate(args.problems, budget=args.budget, log=sys.stdout)
-    np.savez_compressed(os.path.join(args.out, "degree_data.npz"), X=X, p=p,
-                        meta=np.array([json.dumps(m) for m in meta]))
+    Xs, ps = [], []
+    n_problems = 0
+    if args.problems > 0:
+        X, p, meta = generate(args.problems, seed0=args.seed0, budget=args.budget, log=sys.stdout)
+        np.savez_compressed(os.path.join(args.out, "degree_data.npz"), X=X, p=p,
+                            meta=np.array([json.dumps(m) for m in meta]))
+        Xs.append(X)
+        ps.append(p)
+        n_problems += args.problems
+    if args.label_only:
+        return
+    for path in [d for d in args.data.split(",") if d]:
+        with np.load(path, allow_pickle=False) as z:
+            Xs.append(z["X"])
+            ps.append(z["p"])
+            n_problems += len({json.loads(m)[0] for m in z["meta"]})
+    X, p = np.vstack(Xs), np.concatenate(ps)
     y = ((p - P_LO) / P_SPAN).astype(np.float32)
     n_train = int(0.8 * len(X))
     net = DegreeMLP(seed=args.seed)
@@ -101,7 +120,7 @@ def main():
           f"under-predicted {np.mean(resid < 0):.1%}, |err|<=1 {np.mean(np.abs(resid) <= 1):.1%}", flush=True)
     net.save(os.path.join(args.out, "degree_mlp.npz"))
     with open(os.path.join(args.out, "degree_mlp.json"), "w") as f:
-        json.dump({"problems": args.problems, "rows": int(len(X)), "train_rows": n_train, "epochs": args.epochs,
+        json.dump({"problems": n_problems, "rows": int(len(X)), "train_rows": n_train, "epochs": args.epochs,
                    "budget_total": args.budget, "k_range": [1.0, 80.0],
                    "labels": "cost-optimal joint degrees (greedy), priced by measured M2L time per pair", "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
                    "heldout_mean_resid": float(resid.mean()), "heldout_under": float(np.mean(resid < 0)),
