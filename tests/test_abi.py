@@ -54,3 +54,31 @@ def test_noalloc_direct_rejects_sizes_beyond_its_scratch():
     assert lib.hfmm_direct_noalloc(dummy, 10, dummy, 1.0, dummy, 65537, dummy, None) == L.HFMM_ERR_ARG
     assert b"HFMM_DIRECT_NOALLOC_MAX_TARGETS" in lib.hfmm_last_error()
     assert lib.hfmm_direct_noalloc(None, 10, dummy, 1.0, dummy, 5, dummy, None) == L.HFMM_ERR_ARG
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of hfmm_opts and hfmm_plan_info have the header's size and field
+    offsets (a shorter mirror would let the library write past it)."""
+    import shutil
+    import subprocess
+    from hfmm0_amd import _lib as L
+    if shutil.which("gcc") is None:
+        import pytest
+        pytest.skip("gcc not available")
+    structs = {"hfmm_opts": L.Opts, "hfmm_plan_info": L.PlanInfo}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "hfmm.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
