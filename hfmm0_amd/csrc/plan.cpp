@@ -1214,14 +1214,14 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
         mark(1);
         launch_p2m(P.d_leafdesc, P.n_far_leaves, P.p2m_chunk, lv, P.dX, P.dY, P.dZ, P.d_q, P.k, P.p_leafmax, P.dM, s);
         mark(2);
-        int conv = P.depth + 1;   // levels >= conv have their multipoles in FP32 already
+        bool conv[kMaxLevels + 1] = {};   // levels whose multipoles are in FP32 already
         for (int L = P.depth - 1; L >= 2; --L) {
             if (P.m2mb[L].use) {
                 const M2LBlockLevel& B = P.m2mb[L];
                 if (B.fp32) {   // the children's multipoles (complete now) in FP32
                     const int64_t c0 = level_coef(P, L + 1), c1 = level_coef(P, L + 2);
                     launch_to_float(P.dM + c0, P.dM32 + c0, c1 - c0, s);
-                    conv = L + 1;
+                    conv[L + 1] = true;
                 }
                 if (B.split) launch_zero_boxes(B.zero, B.n_zero, P.d_expoff, ncoef(P.p_eff[L]), P.dM, s);
                 launch_m2l_block(block_args(P, B, B.fp32, B.fp32 ? (const void*)P.dM32 : (const void*)P.dM, P.dM,
@@ -1232,10 +1232,22 @@ void run_sequence(hfmm_plan& P, const void* q, int q_type, double2* out, hipStre
                 launch_xlate(xargs(P, P.m2m[L], P.dM, P.dM, 0), P.m2m[L].n_targets, s);
         }
         mark(3);
-        // multipoles in FP32 for the FP32 block levels (those not converted for FP32 M2M)
+        // multipoles in FP32 for the FP32 block levels: every level not converted for an FP32
+        // M2M (the converted ones need not be contiguous), one launch per run of levels
         bool any_fp32 = false;
         for (int L = 2; L <= P.depth; ++L) any_fp32 |= P.m2l[L].n_targets && P.m2lb[L].use && P.m2lb[L].fp32;
-        if (any_fp32) launch_to_float(P.dM, P.dM32, level_coef(P, conv), s);
+        if (any_fp32)
+            for (int l0 = 0; l0 <= P.depth;) {
+                if (conv[l0]) {
+                    ++l0;
+                    continue;
+                }
+                int l1 = l0;
+                while (l1 + 1 <= P.depth && !conv[l1 + 1]) ++l1;
+                const int64_t c0 = level_coef(P, l0), c1 = level_coef(P, l1 + 1);
+                launch_to_float(P.dM + c0, P.dM32 + c0, c1 - c0, s);
+                l0 = l1 + 1;
+            }
     }
     for (int L = 2; L <= P.depth; ++L) {
         if (!P.m2l[L].n_targets) continue;
