@@ -129,7 +129,7 @@ def dtype_label(st):
     lv = st.m2l_fp32_levels
     m2l = (f"f32-MFMA M2L at levels {lv[0]}-{lv[-1]}" if lv else "f64-MFMA M2L")
     def span(name, levels):
-        return f"{name} levels {','.join(map(str, levels))}" if levels else None
+        return f"{name} level{'s' if len(levels) > 1 else ''} {','.join(map(str, levels))}" if levels else None
     f32 = [s for s in (span("M2M into", st.m2m_fp32_levels), span("L2L out of", st.l2l_fp32_levels)) if s]
     tree = "f32-MFMA " + " and ".join(f32) + ", the other M2M/L2L f64-MFMA" if f32 else "f64-MFMA M2M/L2L"
     return (f"mixed: {m2l}, {tree}, f32 P2P weights and phase from f64 coordinate differences"
