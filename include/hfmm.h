@@ -35,7 +35,9 @@ extern "C" {
                                   e^{ikr}/r in FP32 (the phase k r in FP64 for leaves whose pairs
                                   reach k r > 8), 64-source sums in FP32 added in FP64; M2L in
                                   FP32 on matrix cores (at the levels hfmm_opts.m2l_fp64_levels
-                                  leaves in FP32); P2M/M2M/L2L/L2P and accumulation in FP64 */
+                                  leaves in FP32); M2M and L2L in FP32 on matrix cores where the
+                                  M2L of every level of the chain is FP32 (hfmm_plan_info says
+                                  which), else FP64; P2M/L2P and accumulation in FP64 */
 
 /* pointer kinds for q / out */
 #define HFMM_HOST 0
