@@ -132,8 +132,9 @@ def dtype_label(st):
         return f"{name} level{'s' if len(levels) > 1 else ''} {','.join(map(str, levels))}" if levels else None
     f32 = [s for s in (span("M2M into", st.m2m_fp32_levels), span("L2L out of", st.l2l_fp32_levels)) if s]
     tree = "f32-MFMA " + " and ".join(f32) + ", the other M2M/L2L f64-MFMA" if f32 else "f64-MFMA M2M/L2L"
-    return (f"mixed: {m2l}, {tree}, f32 P2P weights and phase from f64 coordinate differences"
-            f" ({info.n_leaves_phase64} leaves with f64 phase), f64 P2M/L2P and accumulation")
+    return (f"mixed: {m2l}, {tree}, f32 P2P weights, coordinate differences in f32 at"
+            f" {info.n_leaves_coord32} of {info.n_leaves} leaves (f64 at the others;"
+            f" {info.n_leaves_phase64} leaves with f64 phase), f64 P2M/L2P and accumulation")
 
 
 def host_cores():

@@ -30,7 +30,8 @@ class Opts(ctypes.Structure):
     _fields_ = [("p", ctypes.c_int), ("p_level", ctypes.c_int * MAX_LEVELS), ("n_max", ctypes.c_int),
                 ("max_depth", ctypes.c_int), ("precision", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("m2l_kernel", ctypes.c_int), ("tree_device", ctypes.c_int),
-                ("m2l_fp64_levels", ctypes.c_int), ("p2p_tile", ctypes.c_int)]
+                ("m2l_fp64_levels", ctypes.c_int), ("p2p_tile", ctypes.c_int),
+                ("p2p_coord32_tol", ctypes.c_double)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -41,7 +42,7 @@ class PlanInfo(ctypes.Structure):
                 ("extent", ctypes.c_double), ("pmin", ctypes.c_double * 3),
                 ("p_table", ctypes.c_int * MAX_LEVELS), ("m2l_fp32_levels", ctypes.c_int),
                 ("n_leaves_phase64", ctypes.c_int), ("m2m_fp32_levels", ctypes.c_int),
-                ("l2l_fp32_levels", ctypes.c_int)]
+                ("l2l_fp32_levels", ctypes.c_int), ("n_leaves_coord32", ctypes.c_int)]
 
 
 class HFMMError(RuntimeError):

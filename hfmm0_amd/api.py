@@ -57,6 +57,11 @@ class HFMMState:
         return [i for i in range(1, self.info.depth + 1) if (m >> i) & 1]
 
     @property
+    def n_leaves_coord32(self):
+        """Leaves whose near field takes FP32 coordinate differences (mixed precision)."""
+        return int(self.info.n_leaves_coord32)
+
+    @property
     def m2m_fp32_levels(self):
         """Levels l whose M2M (into l, from l+1) runs FP32 arithmetic (mixed precision)."""
         m = self.info.m2m_fp32_levels
@@ -105,7 +110,7 @@ class HFMMState:
 
 
 def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, use_graph=True,
-          m2l_kernel="auto", tree_device=False, m2l_fp64_levels=None, p2p_tile=0):
+          m2l_kernel="auto", tree_device=False, m2l_fp64_levels=None, p2p_tile=0, p2p_coord32_tol=0.0):
     """Build the octree, lists and operator tables; upload everything to the GPU.
 
     Validation order and messages follow ``HFMM.py:57-64`` and ``HFMM.py:14-16``.
@@ -119,12 +124,14 @@ def setup(positions, k, p=4, N_max=16, max_depth=None, *, precision="fp64", p_le
     ``m2l_fp64_levels`` (mixed precision only) lists the tree levels whose M2L stays in
     FP64; ``None`` applies the automatic rule (FP64 where k times the level's box diagonal
     exceeds ``MIXED_FP32_MAX_KD``).  ``p2p_tile`` (mixed precision) sets the targets per
-    lane of the near-field kernel: 1, 2 or 4 (0: 2).
+    lane of the near-field kernel: 1, 2 or 4 (0: 2).  ``p2p_coord32_tol`` (mixed precision)
+    is the estimated relative near-field error up to which a leaf takes FP32 coordinate
+    differences (0: ``HFMM_P2P_COORD32_TOL``, 2^-15; negative: FP64 coordinates everywhere).
     """
     x, k = _check_problem(positions, k)
     o = make_opts(p, N_max, max_depth, precision=precision, p_level=p_level, use_graph=use_graph,
                   m2l_kernel=m2l_kernel, tree_device=tree_device, m2l_fp64_levels=m2l_fp64_levels,
-                  p2p_tile=p2p_tile, _k=k)
+                  p2p_tile=p2p_tile, p2p_coord32_tol=p2p_coord32_tol, _k=k)
     h = ctypes.c_void_p()
     L.check(L.lib().hfmm_plan_create(L.ptr(x), len(x), k, ctypes.byref(o), ctypes.byref(h)))
     return HFMMState(h, len(x), k, o)
@@ -138,7 +145,8 @@ def _check_problem(positions, k):
 
 
 def make_opts(p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, use_graph=True,
-              m2l_kernel="auto", tree_device=False, m2l_fp64_levels=None, p2p_tile=0, _k=0.0):
+              m2l_kernel="auto", tree_device=False, m2l_fp64_levels=None, p2p_tile=0, p2p_coord32_tol=0.0,
+              _k=0.0):
     """Validated plan options (the C ``hfmm_opts``), in the reference's validation order
     after the positions check (``HFMM.py:57-64``, ``HFMM.py:14-16``)."""
     if p < 2 or int(p) != p or N_max < 1 or int(N_max) != N_max:
@@ -162,6 +170,9 @@ def make_opts(p=4, N_max=16, max_depth=None, *, precision="fp64", p_level=None, 
     if p2p_tile not in (0, 1, 2, 4):
         raise ValueError("p2p_tile must be 0, 1, 2 or 4")
     o.p2p_tile = int(p2p_tile)
+    if not np.isfinite(float(p2p_coord32_tol)):
+        raise ValueError("p2p_coord32_tol must be a finite number")
+    o.p2p_coord32_tol = float(p2p_coord32_tol)
     if m2l_fp64_levels is not None:
         lv = [int(v) for v in m2l_fp64_levels]
         if any(v < 1 or v >= L.MAX_LEVELS for v in lv):

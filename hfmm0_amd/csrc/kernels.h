@@ -122,10 +122,21 @@ struct P2PArgs {
     const int* perm;              // sorted -> input order
     int out_mode;                 // bit 0: conjugate, bit 1: zero imaginary part
     double2* out;                 // input order
+    const unsigned char* coord32; // per leaf (mixed form, FP32 phase): 1 = coordinate differences
+                                  // and r^2 in FP32 (null: FP64 at every leaf)
 };
 // lowk: every near pair of the plan has k r <= kP2PLowKr (mixed form only); tile: targets
 // per lane of the mixed form (1, 2 or 4)
 void launch_p2p(const P2PArgs& a, int precision, int n_leaves, int n_fp32, bool lowk, int tile, hipStream_t s);
+// Per leaf l < n (leaf order of a): flag[l] = 1 where FP32 coordinates keep the estimated
+// relative near-field error of every target of the leaf at or below tol:
+//   2^-21 c_max sqrt(sum_j r_j^-4 / sum_j r_j^-2)  <= tol,
+// c_max the largest coordinate magnitude relative to the leaf centre over its targets and
+// sources (2^-21 c_max bounds the error of an FP32 coordinate difference), the sums over the
+// target's sources beyond the coincidence floor (the rounding error of pair j perturbs its
+// weight by about r_j^-2 times that error; with random charges the perturbations add in
+// quadrature against a potential of about sqrt(sum r_j^-2)).  FP64 arithmetic; setup only.
+void launch_p2p_coord_check(const P2PArgs& a, int n, double tol, unsigned char* flag, hipStream_t s);
 
 void launch_direct(const double* X, const double* Y, const double* Z, const double2* q, int64_t n, double k,
                    const double* tx, const double* ty, const double* tz, int64_t nt, double2* out, hipStream_t s);

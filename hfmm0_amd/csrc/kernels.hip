@@ -910,7 +910,8 @@ __device__ inline int p2p_chunk(int remaining) {
 // scans every G-th source (G = 64 / L groups) of each 64-point batch staged in LDS, so
 // that every source read from LDS feeds R independent pair evaluations.  Coordinates are
 // relative to the target leaf centre and scaled by a.scale = k / pi, in FP64: the
-// coordinate differences and r^2 are FP64 (exact to rounding for close pairs), the
+// coordinate differences and r^2 are FP64 (exact to rounding for close pairs; FP32 at the
+// FP32-phase leaves flagged in a.coord32, p2p_mixed_body_c32), the
 // weights e^{i pi u}/u (u = k r / pi) FP32 (halfturn_weight; PH: phase in FP64), the
 // 64-source batch sums FP32, added into FP64 sums per target.  LOWK (k r below 1e-4 over
 // every pair of the plan, k = 0 included): coordinates unscaled and the weights
@@ -928,8 +929,112 @@ union P2PStage {
     } fp64;
 };
 
+// The FP32-phase mixed near field of a leaf whose coordinate differences and r^2 may be
+// FP32 (a.coord32): the same schedule as p2p_mixed_body, with the scaled coordinates
+// relative to the leaf centre rounded to FP32 once (targets in registers, sources staged
+// in LDS as (x, y, z) and the charge).
+template <int R>
+__device__ __forceinline__ void p2p_mixed_body_c32(const P2PArgs& a, const int leaf, P2PStage& st) {
+    float4* sxyz = (float4*)st.mixed.xy;
+    float2* sq = (float2*)st.mixed.zq;
+    const int b = a.leaf_box[leaf];
+    const int tstart = a.bx.pstart[b], tcnt = a.bx.count[b];
+    const double sc = a.scale;
+    const double cx = a.bx.centre[3 * b], cy = a.bx.centre[3 * b + 1], cz = a.bx.centre[3 * b + 2];
+    const int r0 = a.rng_ptr[leaf], r1 = a.rng_ptr[leaf + 1];
+    const int S = a.leaf_src_total[leaf];
+    // the floor in scaled coordinates, at least 2^-120 (an FP32 normal number)
+    const float fl2 = (float)fmax(a.floor2[leaf] * sc * sc, 0x1p-120);
+    const int lane = threadIdx.x;
+    for (int t0 = 0, tc = 0; t0 < tcnt; t0 += tc) {
+        tc = p2p_chunk<R>(tcnt - t0);
+        const int need = (tc + R - 1) / R;
+        int L = 1;
+        while (L < need) L <<= 1;
+        const int G = 64 / L;
+        const int li = lane & (L - 1), g = lane / L;
+        float xt[R], yt[R], zt[R];
+        double sr[R], si[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int it = tstart + t0 + min(li + r * L, tc - 1);
+            xt[r] = (float)((a.X[it] - cx) * sc);
+            yt[r] = (float)((a.Y[it] - cy) * sc);
+            zt[r] = (float)((a.Z[it] - cz) * sc);
+            sr[r] = 0.0;
+            si[r] = 0.0;
+        }
+        int ri = r0;
+        for (int base = 0; base < S; base += 64) {
+            const int v = base + lane;
+            if (v < S) {
+                const int idx = p2p_source_index(v, r1, a.rng_start, a.rng_pref, ri);
+                const double2 qq = a.q[idx];
+                sxyz[lane] = make_float4((float)((a.X[idx] - cx) * sc), (float)((a.Y[idx] - cy) * sc),
+                                         (float)((a.Z[idx] - cz) * sc), 0.0f);
+                sq[lane] = make_float2((float)qq.x, (float)qq.y);
+            }
+            __syncthreads();
+            const int nb = min(64, S - base);
+            float ar[R], ai[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) ar[r] = ai[r] = 0.0f;
+            for (int j = g; j < nb; j += G) {
+                const float4 p = sxyz[j];
+                const float2 qj = sq[j];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const float dx = xt[r] - p.x, dy = yt[r] - p.y, dz = zt[r] - p.z;
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    float c, s, rs;
+                    nearfield::halfturn_parts(fmaxf(r2, 0x1p-120f), c, s, rs);
+                    rs = r2 > fl2 ? rs : 0.0f;
+                    const float wr = c * rs, wi = s * rs;
+                    ar[r] = fmaf(wr, qj.x, ar[r]);
+                    ar[r] = fmaf(-wi, qj.y, ar[r]);
+                    ai[r] = fmaf(wr, qj.y, ai[r]);
+                    ai[r] = fmaf(wi, qj.x, ai[r]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                sr[r] += (double)ar[r];
+                si[r] += (double)ai[r];
+            }
+            __syncthreads();
+        }
+        for (int off = L; off < 64; off <<= 1)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                sr[r] += __shfl_xor(sr[r], off);
+                si[r] += __shfl_xor(si[r], off);
+            }
+        if (g == 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int ti = li + r * L;
+                if (ti < tc) {
+                    const int it = tstart + t0 + ti;
+                    double2 f = a.far[it];
+                    f.x = fma(sr[r], a.wscale, f.x);
+                    f.y = fma(si[r], a.wscale, f.y);
+                    if (a.out_mode & 1) f.y = -f.y;   // k < 0: conjugate of the |k| result
+                    if (a.out_mode & 2) f.y = 0.0;    // k == 0 with real charges: the result is real
+                    a.out[a.perm[it]] = f;
+                }
+            }
+        }
+    }
+}
+
 template <bool PH, bool LOWK, int R>
 __device__ __forceinline__ void p2p_mixed_body(const P2PArgs& a, const int leaf, P2PStage& st) {
+    if constexpr (!PH && !LOWK) {
+        if (a.coord32 && a.coord32[leaf]) {   // uniform: one leaf per workgroup
+            p2p_mixed_body_c32<R>(a, leaf, st);
+            return;
+        }
+    }
     double2* sxy = st.mixed.xy;
     double2* szq = st.mixed.zq;
     const int b = a.leaf_box[leaf];
@@ -1550,6 +1655,52 @@ void launch_p2p(const P2PArgs& a0, int precision, int n_leaves, int n_fp32, bool
     else
         launch_p2p_mixed<2>(a, n_leaves, n_fp32, lowk, s);
     check_launch("p2p");
+}
+
+// One wave per leaf: lane t holds target t of a 64-target chunk; every source of the leaf is
+// read by all lanes (uniform loads), FP64 throughout.
+__global__ __launch_bounds__(64) void p2p_coord_check_kernel(P2PArgs a, double tol, unsigned char* flag) {
+    const int leaf = blockIdx.x;
+    const int b = a.leaf_box[leaf];
+    const int tstart = a.bx.pstart[b], tcnt = a.bx.count[b];
+    const double cx = a.bx.centre[3 * b], cy = a.bx.centre[3 * b + 1], cz = a.bx.centre[3 * b + 2];
+    const int r0 = a.rng_ptr[leaf], r1 = a.rng_ptr[leaf + 1];
+    const int S = a.leaf_src_total[leaf];
+    const double fl2 = a.floor2[leaf];
+    const int lane = threadIdx.x;
+    double cmax = 0.0, ratio = 0.0;
+    for (int t0 = 0; t0 < tcnt; t0 += 64) {
+        const bool has = t0 + lane < tcnt;
+        const int it = tstart + t0 + (has ? lane : 0);
+        const double xt = a.X[it] - cx, yt = a.Y[it] - cy, zt = a.Z[it] - cz;
+        if (has) cmax = fmax(cmax, fmax(fabs(xt), fmax(fabs(yt), fabs(zt))));
+        double s2 = 0.0, s4 = 0.0;
+        int ri = r0;
+        for (int v = 0; v < S; ++v) {
+            const int idx = p2p_source_index(v, r1, a.rng_start, a.rng_pref, ri);
+            const double xs = a.X[idx] - cx, ys = a.Y[idx] - cy, zs = a.Z[idx] - cz;
+            if (t0 == 0) cmax = fmax(cmax, fmax(fabs(xs), fmax(fabs(ys), fabs(zs))));
+            const double dx = xt - xs, dy = yt - ys, dz = zt - zs;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > fl2) {
+                const double w = 1.0 / r2;
+                s2 += w;
+                s4 += w * w;
+            }
+        }
+        if (has && s2 > 0.0) ratio = fmax(ratio, sqrt(s4 / s2));
+    }
+    for (int off = 1; off < 64; off <<= 1) {
+        cmax = fmax(cmax, __shfl_xor(cmax, off));
+        ratio = fmax(ratio, __shfl_xor(ratio, off));
+    }
+    if (lane == 0) flag[leaf] = 0x1p-21 * cmax * ratio <= tol ? 1 : 0;
+}
+
+void launch_p2p_coord_check(const P2PArgs& a, int n, double tol, unsigned char* flag, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(p2p_coord_check_kernel, dim3(n), dim3(64), 0, s, a, tol, flag);
+    check_launch("p2p coordinate check");
 }
 
 void launch_direct(const double* X, const double* Y, const double* Z, const double2* q, int64_t n, double k,

@@ -130,6 +130,9 @@ struct hfmm_plan {
     M2LBlockLevel m2mb[kMaxLevels], l2lb[kMaxLevels];   // M2M / L2L on matrix cores
     int m2l_kernel = 0;
     int p2p_tile = 2;               // targets per lane of the mixed near field
+    double p2p_coord32_tol = 0.0;   // estimated error up to which a leaf takes FP32 coordinates (<= 0: none)
+    unsigned char* d_coord32 = nullptr;   // per leaf: FP32 coordinate differences (mixed, FP32 phase)
+    int n_leaves_coord32 = 0;
     size_t max_lds = 65536;
     // rotation blocks as MFMA A fragments (E^T for the forward, E for the back rotation)
     double *d_rf1 = nullptr, *d_rf3 = nullptr;
@@ -853,6 +856,8 @@ void build_tables(hfmm_plan& P) {
     }
 }
 
+P2PArgs p2p_args(const hfmm_plan& P, int out_mode, double2* out);
+
 void build_p2p(hfmm_plan& P) {
     const Tree& t = P.tree;
     const Lists& ls = P.lists;
@@ -945,6 +950,17 @@ void build_p2p(hfmm_plan& P) {
     P.d_leaf_src_total = P.upload(total);
     P.d_floor2 = P.upload(floor2);
     P.box_maxabs = std::vector<double>();
+    // FP32-phase leaves whose estimated error from FP32 coordinate differences is within
+    // p2p_coord32_tol take them (the check runs once, on the GPU, in FP64)
+    if (P.precision == HFMM_PREC_MIXED && !P.p2p_lowk && P.p2p_coord32_tol > 0.0 && P.n_p2p_fp32 > 0) {
+        P.d_coord32 = P.alloc<unsigned char>(P.n_leaves);
+        HIPCHK(hipMemset(P.d_coord32, 0, P.n_leaves));
+        launch_p2p_coord_check(p2p_args(P, 0, nullptr), P.n_p2p_fp32, P.p2p_coord32_tol, P.d_coord32, P.stream);
+        std::vector<unsigned char> f(P.n_leaves);
+        HIPCHK(hipMemcpyAsync(f.data(), P.d_coord32, P.n_leaves, hipMemcpyDeviceToHost, P.stream));
+        HIPCHK(hipStreamSynchronize(P.stream));
+        for (unsigned char v : f) P.n_leaves_coord32 += v;
+    }
 }
 
 void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm_opts& o) {
@@ -972,6 +988,8 @@ void build_plan(hfmm_plan& P, const double* xyz, int64_t n, double k, const hfmm
     P.use_graph = o.use_graph;
     P.m2l_kernel = o.m2l_kernel;
     P.p2p_tile = o.p2p_tile == 0 ? 2 : o.p2p_tile;
+    if (!std::isfinite(o.p2p_coord32_tol)) fail(HFMM_ERR_ARG, "p2p_coord32_tol must be a finite number");
+    P.p2p_coord32_tol = o.p2p_coord32_tol == 0.0 ? HFMM_P2P_COORD32_TOL : o.p2p_coord32_tol;
     {
         int dev = 0, lds = 65536;
         HIPCHK(hipGetDevice(&dev));
@@ -1193,6 +1211,7 @@ P2PArgs p2p_args(const hfmm_plan& P, int out_mode, double2* out) {
     pa.perm = P.d_perm;
     pa.out_mode = out_mode;
     pa.out = out;
+    pa.coord32 = P.d_coord32;
     return pa;
 }
 
@@ -1431,6 +1450,7 @@ int hfmm_plan_info_get(const hfmm_plan* P, hfmm_plan_info* info) {
             if (P->m2lb[L].use && P->m2lb[L].fp32) info->m2l_fp32_levels |= 1 << L;
             if (P->m2mb[L].use && P->m2mb[L].fp32) info->m2m_fp32_levels |= 1 << L;
             if (P->l2lb[L].use && P->l2lb[L].fp32) info->l2l_fp32_levels |= 1 << L;
+            info->n_leaves_coord32 = P->n_leaves_coord32;
         }
         info->extent = P->tree.extent;
         for (int a = 0; a < 3; ++a) info->pmin[a] = P->tree.pmin[a];

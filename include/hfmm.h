@@ -31,9 +31,11 @@ extern "C" {
 
 /* precision of the evaluate path */
 #define HFMM_PREC_FP64 0       /* FP64 everywhere (the plain FMM) */
-#define HFMM_PREC_MIXED 1      /* near field: coordinate differences and r^2 in FP64, weights
-                                  e^{ikr}/r in FP32 (the phase k r in FP64 for leaves whose pairs
-                                  reach k r > 8), 64-source sums in FP32 added in FP64; M2L in
+#define HFMM_PREC_MIXED 1      /* near field: coordinate differences and r^2 in FP32 at the
+                                  leaves whose estimated error from them is within
+                                  hfmm_opts.p2p_coord32_tol, FP64 elsewhere, weights e^{ikr}/r in
+                                  FP32 (the phase k r in FP64 for leaves whose pairs reach
+                                  k r > 8), 64-source sums in FP32 added in FP64; M2L in
                                   FP32 on matrix cores (at the levels hfmm_opts.m2l_fp64_levels
                                   leaves in FP32); M2M and L2L in FP32 on matrix cores where the
                                   M2L of every level of the chain is FP32 (hfmm_plan_info says
@@ -69,7 +71,14 @@ typedef struct hfmm_opts {
                                        box diagonal D_l has k D_l > HFMM_MIXED_FP32_MAX_KD */
     int p2p_tile;                   /* HFMM_PREC_MIXED near field: targets per lane (1, 2 or 4);
                                        0: the default (2) */
+    double p2p_coord32_tol;         /* HFMM_PREC_MIXED near field, leaves with the phase in FP32:
+                                       a leaf takes its coordinate differences and r^2 in FP32
+                                       where the estimated relative error of its targets'
+                                       near field from that rounding (measured at setup) is at
+                                       most this; 0: HFMM_P2P_COORD32_TOL; < 0: FP64 everywhere */
 } hfmm_opts;
+
+#define HFMM_P2P_COORD32_TOL 3.0517578125e-05   /* 2^-15 */
 
 /* Largest k D_l (D_l = 2 sqrt(3) extent 2^-(l+1), the box diagonal of level l) whose M2L the
  * automatic rule of HFMM_PREC_MIXED runs in FP32.  A 1e-6 result needs about 0.75 k D + 2 > 26
@@ -96,7 +105,7 @@ typedef struct hfmm_plan_info {
     int m2l_fp32_levels;            /* bit l set: the M2L of level l runs in FP32 */
     int n_leaves_phase64;           /* HFMM_PREC_MIXED: leaves whose FP32 near-field weights take
                                        the phase k r in FP64 (pairs reaching k r > 8, where an FP32
-                                       phase errs by ~k r 2^-23 rad); every leaf takes coordinate
+                                       phase errs by ~k r 2^-23 rad); these take coordinate
                                        differences and r^2 in FP64 */
     int m2m_fp32_levels;            /* bit l set: the M2M into level l runs FP32 MFMA arithmetic
                                        (FP64 results); mixed precision, where the M2L of every
@@ -105,6 +114,8 @@ typedef struct hfmm_plan_info {
     int l2l_fp32_levels;            /* bit l set: the L2L out of level l runs FP32 MFMA arithmetic
                                        (FP64 results), where the M2L of every level from 2 to l+1
                                        is FP32 */
+    int n_leaves_coord32;           /* HFMM_PREC_MIXED: leaves whose near field takes coordinate
+                                       differences and r^2 in FP32 (hfmm_opts.p2p_coord32_tol) */
 } hfmm_plan_info;
 
 void hfmm_opts_default(hfmm_opts* o);

@@ -386,3 +386,31 @@ def test_mixed_near_field_tiles_agree(hf, k):
         st = hf.setup(x, k, p=14, N_max=64, precision="mixed")
         assert np.count_nonzero(hf.evaluate(st, q.real).imag) == 0
         st.close()
+
+
+def test_fp32_coordinate_leaves(hf):
+    """Mixed near field: FP32-phase leaves whose estimated relative error from FP32
+    coordinate differences is within p2p_coord32_tol take them.  On a uniform problem most
+    leaves do, and the result stays within that error of the FP64-coordinate one; points
+    placed 1e-9 from others keep their leaves in FP64."""
+    say("case: fp32 coordinate leaves N=200000 k=20 p=12 leaf 64")
+    x, q = D.uniform_problem(200000)
+    idx = D.sample_targets(200000, 400)
+    ref = D.direct_c(x, q, 20.0, idx)
+    st = hf.setup(x, 20.0, p=12, N_max=64, precision="mixed")
+    st64 = hf.setup(x, 20.0, p=12, N_max=64, precision="mixed", p2p_coord32_tol=-1)
+    n_leaves = st.info.n_leaves
+    say(f"  leaves {n_leaves}: FP32 coordinates at {st.n_leaves_coord32}, with tol < 0 at {st64.n_leaves_coord32}")
+    assert st64.n_leaves_coord32 == 0
+    assert st.n_leaves_coord32 > 0.5 * n_leaves
+    a, b = hf.evaluate(st, q), hf.evaluate(st64, q)
+    d = D.relative_error(a, b)
+    ea, eb = D.relative_error(a[idx], ref), D.relative_error(b[idx], ref)
+    say(f"  FP32 vs FP64 coordinates {d:.2e}; vs direct {ea:.3e} / {eb:.3e}")
+    assert d < 2e-7
+    assert ea < eb + 1e-7
+    x2 = x.copy()
+    x2[:2000] = x[2000:4000] + 1e-9   # near duplicates, far above the coincidence floor
+    st2 = hf.setup(x2, 20.0, p=12, N_max=64, precision="mixed")
+    say(f"  with 2000 near duplicates: FP32 coordinates at {st2.n_leaves_coord32} of {st2.info.n_leaves}")
+    assert st2.n_leaves_coord32 < st.n_leaves_coord32 - 500
