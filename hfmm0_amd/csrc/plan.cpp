@@ -1450,8 +1450,8 @@ int hfmm_plan_info_get(const hfmm_plan* P, hfmm_plan_info* info) {
             if (P->m2lb[L].use && P->m2lb[L].fp32) info->m2l_fp32_levels |= 1 << L;
             if (P->m2mb[L].use && P->m2mb[L].fp32) info->m2m_fp32_levels |= 1 << L;
             if (P->l2lb[L].use && P->l2lb[L].fp32) info->l2l_fp32_levels |= 1 << L;
-            info->n_leaves_coord32 = P->n_leaves_coord32;
         }
+        info->n_leaves_coord32 = P->n_leaves_coord32;
         info->extent = P->tree.extent;
         for (int a = 0; a < 3; ++a) info->pmin[a] = P->tree.pmin[a];
     });
